@@ -1,0 +1,258 @@
+/*
+ * vof_solver.h -- C ABI of the MI355X (gfx950) SimpleVOFSolver time step.
+ *
+ * Drop-in for the reference's solver interface: the same struct layouts and
+ * entry-point names as the reference's include/ headers, compiled once per
+ * dimensionality because the structs are NDIMS-sized:
+ *   libvofhip2d.so  (-DNDIMS=2)   libvofhip3d.so  (-DNDIMS=3)
+ *
+ * Differences a caller must know:
+ *  - every array_t.data is a DEVICE pointer (hipMalloc'd by *_init) with the
+ *    reference's index map (include/array_macros/ of the reference;
+ *    tools/define_arrays.py:201-229 for 3D).  Use vof_array_to_host /
+ *    vof_array_from_host to move whole padded arrays.
+ *  - domain_t.info, an sdecomp_info_t* in the reference, is an opaque solver
+ *    context here (grid metrics on the device, scratch buffers, HIP stream).
+ *  - one process, one GPU: no MPI, no domain decomposition (mysizes ==
+ *    glsizes, offsets == 0).
+ *  - return codes: 0 = success, nonzero = failure; vof_last_error() gives
+ *    the message (the reference prints to stdout/stderr and returns 1).
+ *  - integrate() and the fluid_* / interface_* steps swap array_t.data
+ *    pointers of the RK source terms exactly like the reference
+ *    (predict/main.c:9-29, interface/update/main.c:28-45); integrate() may
+ *    also swap interface->vof.data with an internal buffer.  Do not cache
+ *    data pointers across calls.
+ *  - all work is queued on the context's HIP stream; entry points that hand
+ *    values to the host (fluid_decide_dt, integrate, *_save,
+ *    vof_array_to_host) synchronise it.
+ */
+#ifndef VOF_SOLVER_H
+#define VOF_SOLVER_H
+
+#include <stddef.h>
+#include <stdbool.h>
+
+#if !defined(NDIMS)
+#error "define NDIMS as 2 or 3 before including vof_solver.h"
+#endif
+#if NDIMS != 2 && NDIMS != 3
+#error "NDIMS must be 2 or 3"
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- structs ------------------------------------------------------------ */
+
+/* replaces include/domain.h:19-28 (3D adds dz after dy) */
+typedef struct {
+  void * info;                 /* opaque solver context (was sdecomp_info_t *) */
+  size_t glsizes[NDIMS];
+  size_t mysizes[NDIMS];
+  size_t offsets[NDIMS];
+  double lengths[NDIMS];
+  double * xf, * xc;           /* host copies, as the reference keeps them */
+  double * dxf, * dxc;         /* host copies */
+  double dy;
+#if NDIMS == 3
+  double dz;
+#endif
+} domain_t;
+
+/* replaces include/array.h:10-23; data is a device pointer */
+typedef struct {
+  size_t size;
+  int (* nadds)[2];
+  size_t datasize;
+  void * data;
+} array_t;
+
+/* replaces include/fluid.h:20-31 (3D adds uz after uy, srcuz after srcuy) */
+typedef struct {
+  array_t ux;
+  array_t uy;
+#if NDIMS == 3
+  array_t uz;
+#endif
+  array_t p;
+  array_t psi;
+  array_t t;
+  array_t srcux[3];
+  array_t srcuy[3];
+#if NDIMS == 3
+  array_t srcuz[3];
+#endif
+  array_t srct[3];
+  double Ra, Pr;
+  double m_dif, t_dif;
+} fluid_t;
+
+/* replaces include/interface.h:13-24 (3D adds ifrcz, flxz) */
+typedef double vector_t[NDIMS];
+typedef double normal_t[NDIMS + 1];
+typedef struct {
+  array_t vof;
+  array_t ifrcx;
+  array_t ifrcy;
+#if NDIMS == 3
+  array_t ifrcz;
+#endif
+  array_t dvof;
+  array_t normal;
+  array_t curv;
+  array_t flxx;
+  array_t flxy;
+#if NDIMS == 3
+  array_t flxz;
+#endif
+  array_t src[2];
+  double tension;
+} interface_t;
+
+/* ---- domain (replaces include/domain.h:30-50) ---------------------------- */
+
+/* src/domain.c:170-209: loads glsizes/lengths/xf/xc NPY, derives dxf, dxc,
+ * dy (dz), and creates the device context */
+int domain_init(const char dirname_ic[], domain_t * domain);
+/* src/domain.c:52-73 */
+int domain_save(const char dirname[], const domain_t * domain);
+/* src/domain.c:75-98 */
+int domain_check_x_grid_is_uniform(const domain_t * domain, bool * x_grid_is_uniform);
+
+/* ---- fluid (replaces include/fluid_solver.h:10-99) ----------------------- */
+
+/* src/fluid/init.c:72-96: allocates device arrays, loads ux uy (uz) p t,
+ * applies boundary conditions, reads Pr and Ra (environment, as
+ * config.get_double; vof_set_param overrides) */
+int fluid_init(const char dirname_ic[], const domain_t * domain, fluid_t * fluid);
+/* src/fluid/save.c:8-27 */
+int fluid_save(const char dirname[], const domain_t * domain, const fluid_t * fluid);
+/* src/fluid/decide_dt.c:104-145 (reads coef_dt_adv, coef_dt_dif) */
+int fluid_decide_dt(const domain_t * domain, const fluid_t * fluid, double * dt);
+/* src/fluid/predict/main.c:62-79 */
+int fluid_predict_field(const domain_t * domain, const size_t rkstep, const double dt,
+                        fluid_t * fluid, const interface_t * interface);
+/* src/fluid/compute_potential/dft.c:406-468 */
+int fluid_compute_potential_dft(const domain_t * domain, const size_t rkstep,
+                                const double dt, fluid_t * fluid);
+/* src/fluid/compute_potential/dct.c:373-421; the same discrete Poisson
+ * problem, solved by the same FFT-in-y(z) / Thomas-in-x kernels */
+int fluid_compute_potential_dct(const domain_t * domain, const size_t rkstep,
+                                const double dt, fluid_t * fluid);
+/* src/fluid/correct_velocity/main.c:15-27 */
+int fluid_correct_velocity(const domain_t * domain, const size_t rkstep,
+                           const double dt, fluid_t * fluid);
+/* src/fluid/update_pressure.c:87-109 */
+int fluid_update_pressure(const domain_t * domain, const size_t rkstep,
+                          const double dt, fluid_t * fluid);
+/* src/fluid/boundary/{ux,uy,p,psi,t}.c */
+int fluid_update_boundaries_ux(const domain_t * domain, array_t * ux);
+int fluid_update_boundaries_uy(const domain_t * domain, array_t * uy);
+#if NDIMS == 3
+int fluid_update_boundaries_uz(const domain_t * domain, array_t * uz);
+#endif
+int fluid_update_boundaries_p(const domain_t * domain, array_t * p);
+int fluid_update_boundaries_psi(const domain_t * domain, array_t * psi);
+int fluid_update_boundaries_t(const domain_t * domain, array_t * t);
+
+/* ---- interface (replaces include/interface_solver.h:8-41) ---------------- */
+
+/* src/interface/init.c:55-71 (reads We) */
+int interface_init(const char dirname_ic[], const domain_t * domain, interface_t * interface);
+/* src/interface/save.c:7-22 */
+int interface_save(const char dirname[], const domain_t * domain, const interface_t * interface);
+/* src/interface/curvature_tensor.c:178-186 */
+int interface_compute_curvature_tensor(const domain_t * domain, interface_t * interface);
+/* src/interface/force.c:69-76 */
+int interface_compute_force(const domain_t * domain, interface_t * interface);
+/* src/interface/update/main.c:112-126 */
+int interface_update_vof(const domain_t * domain, const size_t rkstep, const double dt,
+                         const fluid_t * fluid, interface_t * interface);
+/* src/interface/boundary.c:27-39 */
+int interface_update_boundaries_vof(const domain_t * domain, array_t * vof);
+
+/* ---- integrator (replaces include/integrate.h:5-10) ---------------------- */
+
+/* src/integrate.c:11-68: one RK3 time step; dt is returned on the host */
+int integrate(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt);
+
+/* ---- tri-diagonal solver (replaces include/tdm.h:6-45) ------------------- */
+
+/* l, c, u are HOST arrays owned by the info object (filled by the caller
+ * through get_l/get_c/get_u, as in the reference); solve() takes a DEVICE
+ * pointer to nrhs contiguous lines of `size` values (double, or interleaved
+ * (re, im) pairs when is_complex) and runs the batched GPU Thomas /
+ * Sherman-Morrison kernels on the context-free default stream. */
+typedef struct tdm_info_t_ tdm_info_t;
+typedef struct {
+  int (* const construct)(const int size, const int nrhs, const bool is_periodic,
+                          const bool is_complex, tdm_info_t ** info);
+  int (* const get_l)(const tdm_info_t * info, double ** l);
+  int (* const get_c)(const tdm_info_t * info, double ** c);
+  int (* const get_u)(const tdm_info_t * info, double ** u);
+  int (* const get_size)(const tdm_info_t * info, int * size);
+  int (* const get_nrhs)(const tdm_info_t * info, int * nrhs);
+  int (* const solve)(const tdm_info_t * info, void * data);
+  int (* const destruct)(tdm_info_t * info);
+} tdm_t;
+extern const tdm_t tdm;
+
+/* ---- linear system (replaces include/linear_system.h:37-46) -------------- */
+
+/* src/linear_system.c:28-85.  x1pncl is a device buffer of prod(glsizes)
+ * doubles; tdm_x is the non-periodic x solver (implicit[0]).  Implicit
+ * treatment in y/z is not provided (the reference's default
+ * param/implicit.c:4,7 is explicit) and returns an error. */
+typedef struct {
+  bool is_initialised;
+  bool implicit[NDIMS];
+  double * x1pncl;
+  double * y1pncl;
+  size_t x1pncl_mysizes[NDIMS];
+  size_t y1pncl_mysizes[NDIMS];
+  tdm_info_t * tdm_x;
+  tdm_info_t * tdm_y;
+  void * transposer_x1_to_y1;
+  void * transposer_y1_to_x1;
+} linear_system_t;
+int linear_system_init(const void * info, const bool implicit[NDIMS],
+                       const size_t glsizes[NDIMS], linear_system_t * linear_system);
+int linear_system_finalise(linear_system_t * linear_system);
+
+/* ---- extensions (no counterpart in the reference) ------------------------ */
+
+/* last error message of this thread ("" if none) */
+const char * vof_last_error(void);
+/* override a run-time parameter instead of the environment variable the
+ * reference reads: "Ra", "Pr", "We", "coef_dt_adv", "coef_dt_dif", or one
+ * of the reference's compile-time constants (src/param/ files):
+ * "add_buoyancy" (0/1), "uy_xm", "uy_xp", "uz_xm", "uz_xp", "t_xm", "t_xp".
+ * Must be called after domain_init and before fluid_init / interface_init
+ * for Ra, Pr, We. */
+int vof_set_param(domain_t * domain, const char name[], double value);
+int vof_get_param(const domain_t * domain, const char name[], double * value);
+/* whole padded arrays (datasize bytes) between device and host */
+int vof_array_to_host(const domain_t * domain, const array_t * array, void * host);
+int vof_array_from_host(const domain_t * domain, array_t * array, const void * host);
+/* wait for all queued work of this domain */
+int vof_synchronize(const domain_t * domain);
+/* free device memory (the reference never frees) */
+int interface_finalise(const domain_t * domain, interface_t * interface);
+int fluid_finalise(const domain_t * domain, fluid_t * fluid);
+int domain_finalise(domain_t * domain);
+/* select the per-step kernel schedule of integrate():
+ * 0 = one launch sequence per reference function (the per-function entry
+ *     points), 1 = fused kernels (default) */
+int vof_set_schedule(domain_t * domain, int schedule);
+/* device diagnostics (logging/divergence.c:37-55, logging/interface.c:29-51,
+ * logging/energy.c:43-65): out = {max|div u|, vof min, vof max,
+ * vof volume fraction, kinetic energy x, y[, z], thermal energy} */
+int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
+                    const interface_t * interface, double * out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOF_SOLVER_H */

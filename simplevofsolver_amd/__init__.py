@@ -1,0 +1,292 @@
+"""MI355X (gfx950) SimpleVOFSolver time step behind the reference's C ABI.
+
+The product is the pair of HIP libraries ``libvofhip2d.so`` / ``libvofhip3d.so``
+(sources in ``csrc/``, interface in ``include/vof_solver.h``).  This module is
+a thin ctypes binding used by the tests, the benchmark and Python callers; it
+mirrors the reference's structs field by field and never falls back to a CPU
+path: if a library is missing or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ic  # noqa: F401  (re-export: seeded initial conditions)
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+
+_LIBS: dict = {}
+
+
+class Array(C.Structure):
+    """array_t (include/array.h:10-23); data is a device pointer."""
+    _fields_ = [("size", C.c_size_t), ("nadds", C.c_void_p), ("datasize", C.c_size_t),
+                ("data", C.c_void_p)]
+
+
+def _structs(ndims: int):
+    dom = [("info", C.c_void_p), ("glsizes", C.c_size_t * ndims), ("mysizes", C.c_size_t * ndims),
+           ("offsets", C.c_size_t * ndims), ("lengths", C.c_double * ndims),
+           ("xf", C.POINTER(C.c_double)), ("xc", C.POINTER(C.c_double)),
+           ("dxf", C.POINTER(C.c_double)), ("dxc", C.POINTER(C.c_double)), ("dy", C.c_double)]
+    if ndims == 3:
+        dom.append(("dz", C.c_double))
+    fl = [("ux", Array), ("uy", Array)]
+    if ndims == 3:
+        fl.append(("uz", Array))
+    fl += [("p", Array), ("psi", Array), ("t", Array), ("srcux", Array * 3), ("srcuy", Array * 3)]
+    if ndims == 3:
+        fl.append(("srcuz", Array * 3))
+    fl += [("srct", Array * 3), ("Ra", C.c_double), ("Pr", C.c_double), ("m_dif", C.c_double),
+           ("t_dif", C.c_double)]
+    itf = [("vof", Array), ("ifrcx", Array), ("ifrcy", Array)]
+    if ndims == 3:
+        itf.append(("ifrcz", Array))
+    itf += [("dvof", Array), ("normal", Array), ("curv", Array), ("flxx", Array), ("flxy", Array)]
+    if ndims == 3:
+        itf.append(("flxz", Array))
+    itf += [("src", Array * 2), ("tension", C.c_double)]
+
+    class Domain(C.Structure):
+        _fields_ = dom
+
+    class Fluid(C.Structure):
+        _fields_ = fl
+
+    class Interface(C.Structure):
+        _fields_ = itf
+
+    return Domain, Fluid, Interface
+
+
+# every exported entry point of include/vof_solver.h, per dimensionality
+ABI_SYMBOLS_COMMON = [
+    "domain_init", "domain_save", "domain_check_x_grid_is_uniform",
+    "fluid_init", "fluid_save", "fluid_decide_dt", "fluid_predict_field",
+    "fluid_compute_potential_dft", "fluid_compute_potential_dct", "fluid_correct_velocity",
+    "fluid_update_pressure", "fluid_update_boundaries_ux", "fluid_update_boundaries_uy",
+    "fluid_update_boundaries_p", "fluid_update_boundaries_psi", "fluid_update_boundaries_t",
+    "interface_init", "interface_save", "interface_compute_curvature_tensor",
+    "interface_compute_force", "interface_update_vof", "interface_update_boundaries_vof",
+    "integrate", "tdm", "linear_system_init", "linear_system_finalise",
+    "vof_last_error", "vof_set_param", "vof_get_param", "vof_array_to_host",
+    "vof_array_from_host", "vof_synchronize", "interface_finalise", "fluid_finalise",
+    "domain_finalise", "vof_set_schedule", "vof_diagnostics",
+]
+ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
+
+
+def lib_path(ndims: int) -> str:
+    return os.path.join(PKG, f"libvofhip{ndims}d.so")
+
+
+def load(ndims: int):
+    """Load (never build) the HIP library; raises if it is missing."""
+    if ndims in _LIBS:
+        return _LIBS[ndims]
+    path = lib_path(ndims)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -m simplevofsolver_amd.build`")
+    lib = C.CDLL(path)
+    Domain, Fluid, Interface = _structs(ndims)
+    vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
+    lib.vof_last_error.restype = C.c_char_p
+    for fn in ("domain_init",):
+        getattr(lib, fn).argtypes = [C.c_char_p, vp]
+    for fn in ("fluid_init", "interface_init", "fluid_save", "interface_save"):
+        getattr(lib, fn).argtypes = [C.c_char_p, vp, vp]
+    lib.domain_save.argtypes = [C.c_char_p, vp]
+    lib.domain_check_x_grid_is_uniform.argtypes = [vp, C.POINTER(C.c_bool)]
+    lib.fluid_decide_dt.argtypes = [vp, vp, dp]
+    lib.fluid_predict_field.argtypes = [vp, sz, C.c_double, vp, vp]
+    for fn in ("fluid_compute_potential_dft", "fluid_compute_potential_dct",
+               "fluid_correct_velocity", "fluid_update_pressure"):
+        getattr(lib, fn).argtypes = [vp, sz, C.c_double, vp]
+    lib.interface_update_vof.argtypes = [vp, sz, C.c_double, vp, vp]
+    for fn in ("interface_compute_curvature_tensor", "interface_compute_force",
+               "interface_update_boundaries_vof", "fluid_update_boundaries_ux",
+               "fluid_update_boundaries_uy", "fluid_update_boundaries_p",
+               "fluid_update_boundaries_psi", "fluid_update_boundaries_t",
+               "interface_finalise", "fluid_finalise"):
+        getattr(lib, fn).argtypes = [vp, vp]
+    if ndims == 3:
+        lib.fluid_update_boundaries_uz.argtypes = [vp, vp]
+    lib.integrate.argtypes = [vp, vp, vp, dp]
+    lib.vof_set_param.argtypes = [vp, C.c_char_p, C.c_double]
+    lib.vof_get_param.argtypes = [vp, C.c_char_p, dp]
+    lib.vof_array_to_host.argtypes = [vp, vp, vp]
+    lib.vof_array_from_host.argtypes = [vp, vp, vp]
+    lib.vof_synchronize.argtypes = [vp]
+    lib.domain_finalise.argtypes = [vp]
+    lib.vof_set_schedule.argtypes = [vp, C.c_int]
+    lib.vof_diagnostics.argtypes = [vp, vp, vp, dp]
+    lib._structs = (Domain, Fluid, Interface)
+    _LIBS[ndims] = lib
+    return lib
+
+
+class VofError(RuntimeError):
+    pass
+
+
+PARAM_NAMES = ("Ra", "Pr", "We", "coef_dt_adv", "coef_dt_dif", "add_buoyancy", "uy_xm",
+               "uy_xp", "uz_xm", "uz_xp", "t_xm", "t_xp")
+
+
+class Solver:
+    """One grid on the GPU, initialised from an NPY directory through the
+    reference's entry points (domain_init, fluid_init, interface_init)."""
+
+    def __init__(self, dirname: str, ndims: int, params: dict | None = None, schedule: int | None = None):
+        self.ndims = ndims
+        self.lib = load(ndims)
+        Domain, Fluid, Interface = self.lib._structs
+        self.domain, self.fluid, self.interface = Domain(), Fluid(), Interface()
+        self._ok = False
+        self._chk(self.lib.domain_init(dirname.encode(), C.byref(self.domain)), "domain_init")
+        for k, v in (params or {}).items():
+            self._chk(self.lib.vof_set_param(C.byref(self.domain), k.encode(), float(v)), "vof_set_param")
+        if schedule is not None:
+            self._chk(self.lib.vof_set_schedule(C.byref(self.domain), int(schedule)), "vof_set_schedule")
+        self._chk(self.lib.fluid_init(dirname.encode(), C.byref(self.domain), C.byref(self.fluid)), "fluid_init")
+        self._chk(self.lib.interface_init(dirname.encode(), C.byref(self.domain), C.byref(self.interface)),
+                  "interface_init")
+        self._ok = True
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.vof_last_error().decode(errors="replace")
+            raise VofError(f"{what} failed: {msg}")
+
+    # -- arrays ---------------------------------------------------------------
+    def _array(self, name: str) -> Array:
+        """name: a fluid_t / interface_t member, with [n] for the RK slots
+        ('srcux_a' = srcux[0], 'src_b' = src[1], ...)."""
+        slot = {"a": 0, "b": 1, "g": 2}
+        if "_" in name and name.split("_")[-1] in slot:
+            base, s = name.rsplit("_", 1)
+            owner = self.interface if base == "src" else self.fluid
+            return getattr(owner, base)[slot[s]]
+        if hasattr(self.fluid, name) and name not in ("Ra", "Pr", "m_dif", "t_dif"):
+            return getattr(self.fluid, name)
+        return getattr(self.interface, name)
+
+    def get(self, name: str) -> np.ndarray:
+        a = self._array(name)
+        out = np.empty(a.datasize // 8, dtype=np.float64)
+        self._chk(self.lib.vof_array_to_host(C.byref(self.domain), C.byref(a), out.ctypes.data_as(C.c_void_p)),
+                  "vof_array_to_host")
+        return out
+
+    def set(self, name: str, values) -> None:
+        a = self._array(name)
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if v.size * 8 != a.datasize:
+            raise ValueError(f"{name}: {v.size * 8} bytes given, {a.datasize} expected")
+        self._chk(self.lib.vof_array_from_host(C.byref(self.domain), C.byref(a), v.ctypes.data_as(C.c_void_p)),
+                  "vof_array_from_host")
+
+    # -- solver entry points --------------------------------------------------
+    def integrate(self) -> float:
+        dt = C.c_double(0.0)
+        self._chk(self.lib.integrate(C.byref(self.domain), C.byref(self.fluid), C.byref(self.interface),
+                                     C.byref(dt)), "integrate")
+        return dt.value
+
+    def decide_dt(self) -> float:
+        dt = C.c_double(0.0)
+        self._chk(self.lib.fluid_decide_dt(C.byref(self.domain), C.byref(self.fluid), C.byref(dt)),
+                  "fluid_decide_dt")
+        return dt.value
+
+    def curvature_tensor(self):
+        self._chk(self.lib.interface_compute_curvature_tensor(C.byref(self.domain), C.byref(self.interface)),
+                  "interface_compute_curvature_tensor")
+
+    def force(self):
+        self._chk(self.lib.interface_compute_force(C.byref(self.domain), C.byref(self.interface)),
+                  "interface_compute_force")
+
+    def update_vof(self, rk: int, dt: float):
+        self._chk(self.lib.interface_update_vof(C.byref(self.domain), rk, dt, C.byref(self.fluid),
+                                                C.byref(self.interface)), "interface_update_vof")
+
+    def predict(self, rk: int, dt: float):
+        self._chk(self.lib.fluid_predict_field(C.byref(self.domain), rk, dt, C.byref(self.fluid),
+                                               C.byref(self.interface)), "fluid_predict_field")
+
+    def potential(self, rk: int, dt: float, dct: bool = False):
+        fn = self.lib.fluid_compute_potential_dct if dct else self.lib.fluid_compute_potential_dft
+        self._chk(fn(C.byref(self.domain), rk, dt, C.byref(self.fluid)), "fluid_compute_potential")
+
+    def correct(self, rk: int, dt: float):
+        self._chk(self.lib.fluid_correct_velocity(C.byref(self.domain), rk, dt, C.byref(self.fluid)),
+                  "fluid_correct_velocity")
+
+    def update_pressure(self, rk: int, dt: float):
+        self._chk(self.lib.fluid_update_pressure(C.byref(self.domain), rk, dt, C.byref(self.fluid)),
+                  "fluid_update_pressure")
+
+    def boundaries(self, name: str):
+        if name == "vof":
+            self._chk(self.lib.interface_update_boundaries_vof(C.byref(self.domain),
+                                                               C.byref(self.interface.vof)), "bc vof")
+        else:
+            fn = getattr(self.lib, f"fluid_update_boundaries_{name}")
+            self._chk(fn(C.byref(self.domain), C.byref(getattr(self.fluid, name))), f"bc {name}")
+
+    def synchronize(self):
+        self._chk(self.lib.vof_synchronize(C.byref(self.domain)), "vof_synchronize")
+
+    def diagnostics(self) -> dict:
+        out = (C.c_double * 16)()
+        self._chk(self.lib.vof_diagnostics(C.byref(self.domain), C.byref(self.fluid), C.byref(self.interface),
+                                           out), "vof_diagnostics")
+        keys = ["max_div", "vof_min", "vof_max", "vof_volume", "ke_x", "ke_y"]
+        if self.ndims == 3:
+            keys.append("ke_z")
+        keys.append("thermal")
+        return {k: out[i] for i, k in enumerate(keys)}
+
+    def save(self, dirname: str, step: int = 0, time: float = 0.0):
+        """save_entrypoint (src/main.c:17-32)."""
+        os.makedirs(dirname, exist_ok=True)
+        np.save(os.path.join(dirname, "step.npy"), np.array(step, dtype=np.uint64))
+        np.save(os.path.join(dirname, "time.npy"), np.array(time, dtype=np.float64))
+        self._chk(self.lib.domain_save(dirname.encode(), C.byref(self.domain)), "domain_save")
+        self._chk(self.lib.fluid_save(dirname.encode(), C.byref(self.domain), C.byref(self.fluid)), "fluid_save")
+        self._chk(self.lib.interface_save(dirname.encode(), C.byref(self.domain), C.byref(self.interface)),
+                  "interface_save")
+
+    @property
+    def x_uniform(self) -> bool:
+        b = C.c_bool(False)
+        self.lib.domain_check_x_grid_is_uniform(C.byref(self.domain), C.byref(b))
+        return bool(b.value)
+
+    def close(self):
+        if getattr(self, "_ok", False):
+            self.lib.interface_finalise(C.byref(self.domain), C.byref(self.interface))
+            self.lib.fluid_finalise(C.byref(self.domain), C.byref(self.fluid))
+            self.lib.domain_finalise(C.byref(self.domain))
+            self._ok = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solver_from_ic(ic_obj, dirname: str, schedule: int | None = None, params: dict | None = None) -> "Solver":
+    """Write ``ic_obj`` (simplevofsolver_amd.ic.InitialCondition) to ``dirname``
+    and initialise a GPU solver from it; its ``params`` are applied through
+    vof_set_param."""
+    from .ic import write_ic
+    write_ic(dirname, ic_obj)
+    prm = dict(ic_obj.params)
+    if params:
+        prm.update(params)
+    return Solver(dirname, ic_obj.ndims, prm, schedule)

@@ -1,0 +1,1270 @@
+// vof_kernels.h -- gfx950 device kernels of the SimpleVOFSolver time step.
+//
+// Compiled once per dimensionality (NDIMS = 2 or 3) inside vof_lib.hip.
+// All arithmetic is FP64 and follows the reference's evaluation order; the
+// library is built with -ffp-contract=off so that no FMA contraction changes
+// the rounding relative to the reference's -std=c99 build.
+//
+// Array index maps are the reference's (include/array_macros/ of the
+// reference for 2D, tools/define_arrays.py:201-229 for 3D): x fastest, then
+// y, then z; each array carries its own halo padding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+
+#if NDIMS == 2
+#define LIN(I, J, K, i0, n0, j0, n1, k0) \
+  ((size_t)((I) - (i0)) + (size_t)(n0) * (size_t)((J) - (j0)))
+#else
+#define LIN(I, J, K, i0, n0, j0, n1, k0) \
+  ((size_t)((I) - (i0)) + (size_t)(n0) * ((size_t)((J) - (j0)) + (size_t)(n1) * (size_t)((K) - (k0))))
+#endif
+// these expect ints nx, ny in scope
+#define I_UX(I, J, K)   LIN(I, J, K, 1, nx + 1, 0, ny + 2, 0)
+#define I_C1(I, J, K)   LIN(I, J, K, 0, nx + 2, 0, ny + 2, 0)   /* uy uz p psi t curv */
+#define I_VOF(I, J, K)  LIN(I, J, K, 0, nx + 2, -1, ny + 4, -1)
+#define I_DVOF(I, J, K) LIN(I, J, K, 1, nx + 1, 0, ny + 3, 0)
+#define I_NRM(I, J, K)  LIN(I, J, K, 1, nx, 0, ny + 2, 0)
+#define I_FX(I, J, K)   LIN(I, J, K, 1, nx + 1, 1, ny, 1)
+#define I_FY(I, J, K)   LIN(I, J, K, 1, nx, 1, ny + 1, 1)
+#define I_FZ(I, J, K)   LIN(I, J, K, 1, nx, 1, ny, 1)
+#define I_SX(I, J, K)   LIN(I, J, K, 2, nx - 1, 1, ny, 1)        /* srcux, ifrcx */
+#define I_SC(I, J, K)   LIN(I, J, K, 1, nx, 1, ny, 1)            /* srcuy/z/t, ifrcy/z, src */
+
+namespace vofk {
+
+// interface/common.c:4-14
+#define NGAUSS 2
+__device__ __constant__ double kGaussP[NGAUSS] = {-0.2886751345948129, +0.2886751345948129};
+__device__ __constant__ double kGaussW[NGAUSS] = {+0.5, +0.5};
+constexpr double kVofBeta = 1.;
+constexpr double kVofMin = 1.e-8;
+
+struct Grid {
+  int nx, ny, nz;
+  double dy, dz;
+  const double * __restrict__ dxf;  // [nx]   DXF(i) = dxf[i-1]
+  const double * __restrict__ dxc;  // [nx+1] DXC(i) = dxc[i-1]
+};
+
+#if NDIMS == 2
+#define KIDX 1
+#define THREAD_IJK(ilo, jlo, klo)                               \
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (ilo); \
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (jlo); \
+  const int k = 1; (void)k;
+#else
+#define THREAD_IJK(ilo, jlo, klo)                               \
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (ilo); \
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (jlo); \
+  const int k = (int)blockIdx.z + (klo);
+#endif
+
+// ---------------------------------------------------------------------------
+// boundary conditions (src/halo.c:18-76, fluid/boundary/*.c,
+// interface/boundary.c:7-39)
+
+// Every halo element of a padded array takes its periodic image.  The halo
+// region is enumerated as (A) the y-halo rows of every plane and, in 3D,
+// (B) the z-halo planes restricted to interior rows; each element is written
+// once from its fully wrapped interior source, which this kernel never
+// writes, so the copy is race free and equals the reference's y-then-z
+// exchange.
+__global__ void k_halo(double * __restrict__ a, int n0, int ny, int nz, int h) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = ny + 2 * h;
+#if NDIMS == 2
+  const int np = 1;
+#else
+  const int np = nz + 2 * h;
+#endif
+  const size_t cntA = (size_t)n0 * (2 * h) * np;
+#if NDIMS == 2
+  const size_t cntB = 0;
+#else
+  const size_t cntB = (size_t)n0 * ny * (2 * h);
+#endif
+  int x, r, p;
+  if (t < cntA) {
+    x = (int)(t % n0);
+    const size_t rem = t / n0;
+    const int rr = (int)(rem % (2 * h));
+    p = (int)(rem / (2 * h));
+    r = rr < h ? rr : ny + rr;
+  } else if (t < cntA + cntB) {
+    const size_t u = t - cntA;
+    x = (int)(u % n0);
+    const size_t rem = u / n0;
+    r = h + (int)(rem % ny);
+    const int pp = (int)(rem / ny);
+    p = pp < h ? pp : nz + pp;
+  } else {
+    return;
+  }
+  const int rs = r < h ? r + ny : (r >= ny + h ? r - ny : r);
+#if NDIMS == 2
+  const int ps = 0;
+  (void)nz;
+#else
+  const int ps = p < h ? p + nz : (p >= nz + h ? p - nz : p);
+#endif
+  a[x + (size_t)n0 * (r + (size_t)n1 * p)] = a[x + (size_t)n0 * (rs + (size_t)n1 * ps)];
+}
+
+// x walls on rows [r0, r0+nr) of planes [p0, p0+np): mode 0 sets Dirichlet
+// values (vlo at column xlo, vhi at xhi), mode 1 copies the neighbour
+// (Neumann, p.c:15-16)
+__global__ void k_walls(double * __restrict__ a, int n0, int n1, int r0, int nr, int p0,
+                        int np, int xlo, int xhi, int mode, double vlo, double vhi) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= nr * np) return;
+  const int r = r0 + t % nr;
+  const int p = p0 + t / nr;
+  double * row = a + (size_t)n0 * (r + (size_t)n1 * p);
+  if (mode == 0) {
+    row[xlo] = vlo;
+    row[xhi] = vhi;
+  } else {
+    row[xlo] = row[xlo + 1];
+    row[xhi] = row[xhi - 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// interface_compute_curvature_tensor: src/interface/curvature_tensor.c
+
+// compute_gradient (:14-48): corners i = 1..nx+1, j = 0..ny+2 (k = 0..nz+2)
+__global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __restrict__ dvof) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 0, 0)
+  if (i > nx + 1 || j > ny + 2) return;
+  const double dx = g.dxc[i - 1];
+#if NDIMS == 2
+  const double a = vof[I_VOF(i - 1, j - 1, k)], b = vof[I_VOF(i, j - 1, k)];
+  const double c = vof[I_VOF(i - 1, j, k)], d = vof[I_VOF(i, j, k)];
+  const double gx = 1. / dx * (-a + b - c + d);
+  const double gy = 1. / g.dy * (-a - b + c + d);
+  const double norm = sqrt(gx * gx + gy * gy);
+  const double ninv = 1. / fmax(norm, DBL_EPSILON);
+  double * o = dvof + 2 * I_DVOF(i, j, k);
+  o[0] = gx * ninv;
+  o[1] = gy * ninv;
+#else
+  if (k > g.nz + 2) return;
+  const double a0 = vof[I_VOF(i - 1, j - 1, k - 1)], b0 = vof[I_VOF(i, j - 1, k - 1)];
+  const double c0 = vof[I_VOF(i - 1, j, k - 1)], d0 = vof[I_VOF(i, j, k - 1)];
+  const double a1 = vof[I_VOF(i - 1, j - 1, k)], b1 = vof[I_VOF(i, j - 1, k)];
+  const double c1 = vof[I_VOF(i - 1, j, k)], d1 = vof[I_VOF(i, j, k)];
+  const double gx = 1. / dx * (-a0 + b0 - c0 + d0 - a1 + b1 - c1 + d1);
+  const double gy = 1. / g.dy * (-a0 - b0 + c0 + d0 - a1 - b1 + c1 + d1);
+  const double gz = 1. / g.dz * (-a0 - b0 - c0 - d0 + a1 + b1 + c1 + d1);
+  const double norm = sqrt(gx * gx + gy * gy + gz * gz);
+  const double ninv = 1. / fmax(norm, DBL_EPSILON);
+  double * o = dvof + 3 * I_DVOF(i, j, k);
+  o[0] = gx * ninv;
+  o[1] = gy * ninv;
+  o[2] = gz * ninv;
+#endif
+}
+
+// compute_intercept (:50-92): Newton on D = exp(-2 beta d), at most 8
+// iterations, residual tested after the update
+__device__ inline double intercept(const double vof, const double * n) {
+  const int cntmax = 8;
+  const double resmax = 1.e-12;
+#if NDIMS == 2
+  double e[4];
+  for (int jj = 0; jj < NGAUSS; jj++)
+    for (int ii = 0; ii < NGAUSS; ii++)
+      e[jj * NGAUSS + ii] = exp(-2. * kVofBeta * (n[0] * kGaussP[ii] + n[1] * kGaussP[jj]));
+#else
+  double e[8];
+  for (int kk = 0; kk < NGAUSS; kk++)
+    for (int jj = 0; jj < NGAUSS; jj++)
+      for (int ii = 0; ii < NGAUSS; ii++)
+        e[(kk * NGAUSS + jj) * NGAUSS + ii] =
+            exp(-2. * kVofBeta * (n[0] * kGaussP[ii] + n[1] * kGaussP[jj] + n[2] * kGaussP[kk]));
+#endif
+  double val = 1. / vof - 1.;
+  for (int cnt = 0; cnt < cntmax; cnt++) {
+    double f0 = -1. * vof;
+    double f1 = 0.;
+#if NDIMS == 2
+    for (int jj = 0; jj < NGAUSS; jj++)
+      for (int ii = 0; ii < NGAUSS; ii++) {
+        const double w = kGaussW[ii] * kGaussW[jj];
+        const double p = e[jj * NGAUSS + ii];
+#else
+    for (int kk = 0; kk < NGAUSS; kk++)
+      for (int jj = 0; jj < NGAUSS; jj++)
+        for (int ii = 0; ii < NGAUSS; ii++) {
+          const double w = kGaussW[ii] * kGaussW[jj] * kGaussW[kk];
+          const double p = e[(kk * NGAUSS + jj) * NGAUSS + ii];
+#endif
+        const double den = 1. / (1. + p * val);
+        f0 += w * den;
+        f1 -= w * p * den * den;
+      }
+    val -= f0 / f1;
+    if (fabs(f0) < resmax) break;
+  }
+  return -0.5 / kVofBeta * log(val);
+}
+
+// compute_normal (:94-145) + compute_curvature (:147-176), cells
+// i = 1..nx, j = 0..ny+1 (k = 0..nz+1); both read the same corner normals
+__global__ void k_normal_curv(Grid g, const double * __restrict__ vof,
+                              const double * __restrict__ dvof, double * __restrict__ normal,
+                              double * __restrict__ curv) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 0, 0)
+  if (i > nx || j > ny + 1) return;
+#if NDIMS == 3
+  if (k > g.nz + 1) return;
+#endif
+  const double dx = g.dxf[i - 1];
+#if NDIMS == 2
+  const double * d00 = dvof + 2 * I_DVOF(i, j, k);
+  const double * d10 = dvof + 2 * I_DVOF(i + 1, j, k);
+  const double * d01 = dvof + 2 * I_DVOF(i, j + 1, k);
+  const double * d11 = dvof + 2 * I_DVOF(i + 1, j + 1, k);
+  const double dnxdx = 1. / dx * (-d00[0] + d10[0] - d01[0] + d11[0]);
+  const double dnydy = 1. / g.dy * (-d00[1] - d10[1] + d01[1] + d11[1]);
+  curv[I_C1(i, j, k)] = 0.5 * (-dnxdx - dnydy);
+  const double lvof = vof[I_VOF(i, j, k)];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return;
+  double nxv = d00[0] + d10[0] + d01[0] + d11[0];
+  double nyv = d00[1] + d10[1] + d01[1] + d11[1];
+  nxv /= dx;
+  nyv /= g.dy;
+  const double norm = sqrt(nxv * nxv + nyv * nyv);
+  const double ninv = 1. / fmax(norm, DBL_EPSILON);
+  nxv *= ninv;
+  nyv *= ninv;
+  const double nn[2] = {nxv, nyv};
+  const double seg = intercept(lvof, nn);
+  double * o = normal + 3 * I_NRM(i, j, k);
+  o[0] = nxv;
+  o[1] = nyv;
+  o[2] = seg;
+#else
+  const double * d000 = dvof + 3 * I_DVOF(i, j, k);
+  const double * d100 = dvof + 3 * I_DVOF(i + 1, j, k);
+  const double * d010 = dvof + 3 * I_DVOF(i, j + 1, k);
+  const double * d110 = dvof + 3 * I_DVOF(i + 1, j + 1, k);
+  const double * d001 = dvof + 3 * I_DVOF(i, j, k + 1);
+  const double * d101 = dvof + 3 * I_DVOF(i + 1, j, k + 1);
+  const double * d011 = dvof + 3 * I_DVOF(i, j + 1, k + 1);
+  const double * d111 = dvof + 3 * I_DVOF(i + 1, j + 1, k + 1);
+  const double dnxdx = 1. / dx * (-d000[0] + d100[0] - d010[0] + d110[0]
+                                  - d001[0] + d101[0] - d011[0] + d111[0]);
+  const double dnydy = 1. / g.dy * (-d000[1] - d100[1] + d010[1] + d110[1]
+                                    - d001[1] - d101[1] + d011[1] + d111[1]);
+  const double dnzdz = 1. / g.dz * (-d000[2] - d100[2] - d010[2] - d110[2]
+                                    + d001[2] + d101[2] + d011[2] + d111[2]);
+  curv[I_C1(i, j, k)] = 0.25 * (-dnxdx - dnydy - dnzdz);
+  const double lvof = vof[I_VOF(i, j, k)];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return;
+  double nv[3];
+  for (int d = 0; d < 3; d++) {
+    nv[d] = d000[d] + d100[d] + d010[d] + d110[d] + d001[d] + d101[d] + d011[d] + d111[d];
+  }
+  nv[0] /= dx;
+  nv[1] /= g.dy;
+  nv[2] /= g.dz;
+  const double norm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  const double ninv = 1. / fmax(norm, DBL_EPSILON);
+  nv[0] *= ninv;
+  nv[1] *= ninv;
+  nv[2] *= ninv;
+  const double seg = intercept(lvof, nv);
+  double * o = normal + 4 * I_NRM(i, j, k);
+  o[0] = nv[0];
+  o[1] = nv[1];
+  o[2] = nv[2];
+  o[3] = seg;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// interface_compute_force: src/interface/force.c:11-76 (cells 1..nx x 1..ny)
+__global__ void k_force(Grid g, double tension, const double * __restrict__ vof,
+                        const double * __restrict__ curv, double * __restrict__ ifrcx,
+                        double * __restrict__ ifrcy, double * __restrict__ ifrcz) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double v = vof[I_VOF(i, j, k)];
+  const double cv = curv[I_C1(i, j, k)];
+  if (i >= 2) {
+    const double dx = g.dxc[i - 1];
+    const double grad = 1. / dx * (-vof[I_VOF(i - 1, j, k)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i - 1, j, k)] + cv);
+    ifrcx[I_SX(i, j, k)] = tension * grad * kappa;
+  }
+  {
+    const double grad = 1. / g.dy * (-vof[I_VOF(i, j - 1, k)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i, j - 1, k)] + cv);
+    ifrcy[I_SC(i, j, k)] = tension * grad * kappa;
+  }
+#if NDIMS == 3
+  {
+    const double grad = 1. / g.dz * (-vof[I_VOF(i, j, k - 1)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i, j, k - 1)] + cv);
+    ifrcz[I_SC(i, j, k)] = tension * grad * kappa;
+  }
+#else
+  (void)ifrcz;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// interface_update_vof: src/interface/update/{main,flxx,flxy}.c
+
+// indicator (update/main.c:16-26)
+__device__ inline double indicator(const double * n, const double * x) {
+#if NDIMS == 2
+  return 1. / (1. + exp(-2. * kVofBeta * (n[0] * x[0] + n[1] * x[1] + n[2])));
+#else
+  return 1. / (1. + exp(-2. * kVofBeta * (n[0] * x[0] + n[1] * x[1] + n[2] * x[2] + n[3])));
+#endif
+}
+
+// compute_flux_x (flxx.c:11-44), compute_flux_y (flxy.c:11-44) and the z
+// counterpart; thread (i, j, k) over 1..nx x 1..ny+1 x 1..nz+1 owns
+// FLXX(i), FLXY(j), FLXZ(k) where those exist
+__global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                       const double * __restrict__ uz, const double * __restrict__ vof,
+                       const double * __restrict__ normal, double * __restrict__ flxx,
+                       double * __restrict__ flxy, double * __restrict__ flxz) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny + 1) return;
+#if NDIMS == 3
+  const int nz = g.nz;
+  if (k > nz + 1) return;
+  const bool kin = k <= nz;
+#else
+  const bool kin = true;
+  (void)uz; (void)flxz;
+#endif
+  constexpr int NC = NDIMS + 1;
+  if (i >= 2 && j <= ny && kin) {
+    const double vel = ux[I_UX(i, j, k)];
+    const int ii = vel < 0. ? i : i - 1;
+    const double x = vel < 0. ? -0.5 : +0.5;
+    const double lvof = vof[I_VOF(ii, j, k)];
+    double out;
+    if (lvof < kVofMin || 1. - kVofMin < lvof) {
+      out = vel * lvof;
+    } else {
+      const double * n = normal + NC * I_NRM(ii, j, k);
+      double flux = 0.;
+#if NDIMS == 2
+      for (int jj = 0; jj < NGAUSS; jj++) {
+        const double pt[2] = {x, kGaussP[jj]};
+        flux += kGaussW[jj] * indicator(n, pt);
+      }
+#else
+      for (int kk = 0; kk < NGAUSS; kk++)
+        for (int jj = 0; jj < NGAUSS; jj++) {
+          const double w = kGaussW[jj] * kGaussW[kk];
+          const double pt[3] = {x, kGaussP[jj], kGaussP[kk]};
+          flux += w * indicator(n, pt);
+        }
+#endif
+      out = vel * flux;
+    }
+    flxx[I_FX(i, j, k)] = out;
+  }
+  if (kin) {
+    const double vel = uy[I_C1(i, j, k)];
+    const int jj0 = vel < 0. ? j : j - 1;
+    const double y = vel < 0. ? -0.5 : +0.5;
+    const double lvof = vof[I_VOF(i, jj0, k)];
+    double out;
+    if (lvof < kVofMin || 1. - kVofMin < lvof) {
+      out = vel * lvof;
+    } else {
+      const double * n = normal + NC * I_NRM(i, jj0, k);
+      double flux = 0.;
+#if NDIMS == 2
+      for (int ii = 0; ii < NGAUSS; ii++) {
+        const double pt[2] = {kGaussP[ii], y};
+        flux += kGaussW[ii] * indicator(n, pt);
+      }
+#else
+      for (int kk = 0; kk < NGAUSS; kk++)
+        for (int ii = 0; ii < NGAUSS; ii++) {
+          const double w = kGaussW[ii] * kGaussW[kk];
+          const double pt[3] = {kGaussP[ii], y, kGaussP[kk]};
+          flux += w * indicator(n, pt);
+        }
+#endif
+      out = vel * flux;
+    }
+    flxy[I_FY(i, j, k)] = out;
+  }
+#if NDIMS == 3
+  if (j <= ny) {
+    const double vel = uz[I_C1(i, j, k)];
+    const int kk0 = vel < 0. ? k : k - 1;
+    const double z = vel < 0. ? -0.5 : +0.5;
+    const double lvof = vof[I_VOF(i, j, kk0)];
+    double out;
+    if (lvof < kVofMin || 1. - kVofMin < lvof) {
+      out = vel * lvof;
+    } else {
+      const double * n = normal + NC * I_NRM(i, j, kk0);
+      double flux = 0.;
+      for (int jj = 0; jj < NGAUSS; jj++)
+        for (int ii = 0; ii < NGAUSS; ii++) {
+          const double w = kGaussW[ii] * kGaussW[jj];
+          const double pt[3] = {kGaussP[ii], kGaussP[jj], z};
+          flux += w * indicator(n, pt);
+        }
+      out = vel * flux;
+    }
+    flxz[I_FZ(i, j, k)] = out;
+  }
+#endif
+}
+
+// interface_compute_rhs (update/main.c:47-78) + interface_advect_vof
+// (:80-110): src_a starts from zero (reset_srcs :28-45)
+__global__ void k_vof_update(Grid g, const double * __restrict__ flxx,
+                             const double * __restrict__ flxy, const double * __restrict__ flxz,
+                             double * __restrict__ srca, const double * __restrict__ srcb,
+                             double * __restrict__ vof, double dta, double dtb, int use_b) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  double s = 0.;
+  s += 1. / g.dxf[i - 1] * (flxx[I_FX(i, j, k)] - flxx[I_FX(i + 1, j, k)]);
+  s += 1. / g.dy * (flxy[I_FY(i, j, k)] - flxy[I_FY(i, j + 1, k)]);
+#if NDIMS == 3
+  s += 1. / g.dz * (flxz[I_FZ(i, j, k)] - flxz[I_FZ(i, j, k + 1)]);
+#else
+  (void)flxz;
+#endif
+  const size_t is = I_SC(i, j, k);
+  srca[is] = s;
+  double v = vof[I_VOF(i, j, k)];
+  v += dta * s;
+  if (use_b) v += dtb * srcb[is];
+  vof[I_VOF(i, j, k)] = v;
+}
+
+// ---------------------------------------------------------------------------
+// fluid_predict_field: src/fluid/predict/{ux,uy,t}.c.  Each thread forms the
+// explicit (srca) and implicit (srcg) source of one DOF term by term in the
+// reference's accumulation order.
+
+struct RhsArgs {
+  const double * __restrict__ ux;
+  const double * __restrict__ uy;
+  const double * __restrict__ uz;
+  const double * __restrict__ p;
+  const double * __restrict__ t;
+  const double * __restrict__ ifrc;
+  const double * __restrict__ lapx;   // 3 per x point
+  double * __restrict__ srca;
+  double * __restrict__ srcg;
+  double dif;
+  int add_buoyancy;
+};
+
+// compute_rhs_ux (ux.c:71-249), DOFs i = 2..nx
+__global__ void k_rhs_ux(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(2, 1, 1)
+  if (i > nx || j > ny) return;
+  const double * ux = a.ux;
+  const double * uy = a.uy;
+  const double dxci = g.dxc[i - 1];
+  const double u_c = ux[I_UX(i, j, k)];
+  const double u_xm = ux[I_UX(i - 1, j, k)], u_xp = ux[I_UX(i + 1, j, k)];
+  const double u_ym = ux[I_UX(i, j - 1, k)], u_yp = ux[I_UX(i, j + 1, k)];
+  double sa = 0., sg = 0.;
+  {  // advection_x
+    const double ux_l = +0.5 * u_xm + 0.5 * u_c;
+    const double ux_u = +0.5 * u_c + 0.5 * u_xp;
+    const double l = +0.5 / dxci * ux_l;
+    const double u = -0.5 / dxci * ux_u;
+    const double c = -l - u;
+    sa += +l * u_xm + c * u_c + u * u_xp;
+  }
+  const double w_xm = 0.5 * g.dxf[i - 2] / dxci;
+  const double w_xp = 0.5 * g.dxf[i - 1] / dxci;
+  {  // advection_y
+    const double uy_l = w_xm * uy[I_C1(i - 1, j, k)] + w_xp * uy[I_C1(i, j, k)];
+    const double uy_u = w_xm * uy[I_C1(i - 1, j + 1, k)] + w_xp * uy[I_C1(i, j + 1, k)];
+    const double l = +0.5 / g.dy * uy_l;
+    const double u = -0.5 / g.dy * uy_u;
+    const double c = -l - u;
+    sa += +l * u_ym + c * u_c + u * u_yp;
+  }
+#if NDIMS == 3
+  const double u_zm = ux[I_UX(i, j, k - 1)], u_zp = ux[I_UX(i, j, k + 1)];
+  {  // advection_z
+    const double * uz = a.uz;
+    const double uz_l = w_xm * uz[I_C1(i - 1, j, k)] + w_xp * uz[I_C1(i, j, k)];
+    const double uz_u = w_xm * uz[I_C1(i - 1, j, k + 1)] + w_xp * uz[I_C1(i, j, k + 1)];
+    const double l = +0.5 / g.dz * uz_l;
+    const double u = -0.5 / g.dz * uz_u;
+    const double c = -l - u;
+    sa += +l * u_zm + c * u_c + u * u_zp;
+  }
+#endif
+  {  // diffusion_x -> srcg
+    const double * L = a.lapx + 3 * (i - 2);
+    sg += a.dif * (L[0] * u_xm + L[1] * u_c + L[2] * u_xp);
+  }
+  {  // diffusion_y -> srca
+    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
+    sa += a.dif * (ly * u_ym + cy * u_c + ly * u_yp);
+  }
+#if NDIMS == 3
+  {
+    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
+    sa += a.dif * (lz * u_zm + cz * u_c + lz * u_zp);
+  }
+#endif
+  sg -= 1. / dxci * (-a.p[I_C1(i - 1, j, k)] + a.p[I_C1(i, j, k)]);
+  if (a.add_buoyancy) sa += +0.5 * a.t[I_C1(i - 1, j, k)] + 0.5 * a.t[I_C1(i, j, k)];
+  const size_t is = I_SX(i, j, k);
+  sa += a.ifrc[is];
+  a.srca[is] = sa;
+  a.srcg[is] = sg;
+}
+
+// compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx
+__global__ void k_rhs_uy(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double * ux = a.ux;
+  const double * uy = a.uy;
+  const double dxfi = g.dxf[i - 1];
+  const double v_c = uy[I_C1(i, j, k)];
+  const double v_xm = uy[I_C1(i - 1, j, k)], v_xp = uy[I_C1(i + 1, j, k)];
+  const double v_ym = uy[I_C1(i, j - 1, k)], v_yp = uy[I_C1(i, j + 1, k)];
+  double sa = 0., sg = 0.;
+  {  // advection_x
+    const double ux_l = +0.5 * ux[I_UX(i, j - 1, k)] + 0.5 * ux[I_UX(i, j, k)];
+    const double ux_u = +0.5 * ux[I_UX(i + 1, j - 1, k)] + 0.5 * ux[I_UX(i + 1, j, k)];
+    const double l = +0.5 / dxfi * ux_l;
+    const double u = -0.5 / dxfi * ux_u;
+    const double c = -l - u;
+    sa += +l * v_xm + c * v_c + u * v_xp;
+  }
+  {  // advection_y
+    const double uy_l = +0.5 * v_ym + 0.5 * v_c;
+    const double uy_u = +0.5 * v_c + 0.5 * v_yp;
+    const double l = +0.5 / g.dy * uy_l;
+    const double u = -0.5 / g.dy * uy_u;
+    const double c = -l - u;
+    sa += +l * v_ym + c * v_c + u * v_yp;
+  }
+#if NDIMS == 3
+  const double v_zm = uy[I_C1(i, j, k - 1)], v_zp = uy[I_C1(i, j, k + 1)];
+  {  // advection_z
+    const double * uz = a.uz;
+    const double uz_l = +0.5 * uz[I_C1(i, j - 1, k)] + 0.5 * uz[I_C1(i, j, k)];
+    const double uz_u = +0.5 * uz[I_C1(i, j - 1, k + 1)] + 0.5 * uz[I_C1(i, j, k + 1)];
+    const double l = +0.5 / g.dz * uz_l;
+    const double u = -0.5 / g.dz * uz_u;
+    const double c = -l - u;
+    sa += +l * v_zm + c * v_c + u * v_zp;
+  }
+#endif
+  {
+    const double * L = a.lapx + 3 * (i - 1);
+    sg += a.dif * (L[0] * v_xm + L[1] * v_c + L[2] * v_xp);
+  }
+  {
+    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
+    sa += a.dif * (ly * v_ym + cy * v_c + ly * v_yp);
+  }
+#if NDIMS == 3
+  {
+    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
+    sa += a.dif * (lz * v_zm + cz * v_c + lz * v_zp);
+  }
+#endif
+  sg -= 1. / g.dy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
+  const size_t is = I_SC(i, j, k);
+  sa += a.ifrc[is];
+  a.srca[is] = sa;
+  a.srcg[is] = sg;
+}
+
+#if NDIMS == 3
+// compute_rhs_uz: the z counterpart of compute_rhs_uy, DOFs i = 1..nx
+__global__ void k_rhs_uz(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double * ux = a.ux;
+  const double * uy = a.uy;
+  const double * uz = a.uz;
+  const double dxfi = g.dxf[i - 1];
+  const double w_c = uz[I_C1(i, j, k)];
+  const double w_xm = uz[I_C1(i - 1, j, k)], w_xp = uz[I_C1(i + 1, j, k)];
+  const double w_ym = uz[I_C1(i, j - 1, k)], w_yp = uz[I_C1(i, j + 1, k)];
+  const double w_zm = uz[I_C1(i, j, k - 1)], w_zp = uz[I_C1(i, j, k + 1)];
+  double sa = 0., sg = 0.;
+  {
+    const double ux_l = +0.5 * ux[I_UX(i, j, k - 1)] + 0.5 * ux[I_UX(i, j, k)];
+    const double ux_u = +0.5 * ux[I_UX(i + 1, j, k - 1)] + 0.5 * ux[I_UX(i + 1, j, k)];
+    const double l = +0.5 / dxfi * ux_l;
+    const double u = -0.5 / dxfi * ux_u;
+    const double c = -l - u;
+    sa += +l * w_xm + c * w_c + u * w_xp;
+  }
+  {
+    const double uy_l = +0.5 * uy[I_C1(i, j, k - 1)] + 0.5 * uy[I_C1(i, j, k)];
+    const double uy_u = +0.5 * uy[I_C1(i, j + 1, k - 1)] + 0.5 * uy[I_C1(i, j + 1, k)];
+    const double l = +0.5 / g.dy * uy_l;
+    const double u = -0.5 / g.dy * uy_u;
+    const double c = -l - u;
+    sa += +l * w_ym + c * w_c + u * w_yp;
+  }
+  {
+    const double uz_l = +0.5 * w_zm + 0.5 * w_c;
+    const double uz_u = +0.5 * w_c + 0.5 * w_zp;
+    const double l = +0.5 / g.dz * uz_l;
+    const double u = -0.5 / g.dz * uz_u;
+    const double c = -l - u;
+    sa += +l * w_zm + c * w_c + u * w_zp;
+  }
+  {
+    const double * L = a.lapx + 3 * (i - 1);
+    sg += a.dif * (L[0] * w_xm + L[1] * w_c + L[2] * w_xp);
+  }
+  {
+    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
+    sa += a.dif * (ly * w_ym + cy * w_c + ly * w_yp);
+  }
+  {
+    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
+    sa += a.dif * (lz * w_zm + cz * w_c + lz * w_zp);
+  }
+  sg -= 1. / g.dz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
+  const size_t is = I_SC(i, j, k);
+  sa += a.ifrc[is];
+  a.srca[is] = sa;
+  a.srcg[is] = sg;
+}
+#endif
+
+// compute_rhs_t (t.c:68-182), DOFs i = 1..nx
+__global__ void k_rhs_t(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double * ux = a.ux;
+  const double * uy = a.uy;
+  const double * t = a.t;
+  const double dxfi = g.dxf[i - 1];
+  const double t_c = t[I_C1(i, j, k)];
+  const double t_xm = t[I_C1(i - 1, j, k)], t_xp = t[I_C1(i + 1, j, k)];
+  const double t_ym = t[I_C1(i, j - 1, k)], t_yp = t[I_C1(i, j + 1, k)];
+  double sa = 0., sg = 0.;
+  {
+    const double l = +0.5 / dxfi * ux[I_UX(i, j, k)];
+    const double u = -0.5 / dxfi * ux[I_UX(i + 1, j, k)];
+    const double c = -l - u;
+    sa += +l * t_xm + c * t_c + u * t_xp;
+  }
+  {
+    const double l = +0.5 / g.dy * uy[I_C1(i, j, k)];
+    const double u = -0.5 / g.dy * uy[I_C1(i, j + 1, k)];
+    const double c = -l - u;
+    sa += +l * t_ym + c * t_c + u * t_yp;
+  }
+#if NDIMS == 3
+  const double t_zm = t[I_C1(i, j, k - 1)], t_zp = t[I_C1(i, j, k + 1)];
+  {
+    const double * uz = a.uz;
+    const double l = +0.5 / g.dz * uz[I_C1(i, j, k)];
+    const double u = -0.5 / g.dz * uz[I_C1(i, j, k + 1)];
+    const double c = -l - u;
+    sa += +l * t_zm + c * t_c + u * t_zp;
+  }
+#endif
+  {
+    const double * L = a.lapx + 3 * (i - 1);
+    sg += a.dif * (L[0] * t_xm + L[1] * t_c + L[2] * t_xp);
+  }
+  {
+    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
+    sa += a.dif * (ly * t_ym + cy * t_c + ly * t_yp);
+  }
+#if NDIMS == 3
+  {
+    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
+    sa += a.dif * (lz * t_zm + cz * t_c + lz * t_zp);
+  }
+#endif
+  const size_t is = I_SC(i, j, k);
+  a.srca[is] = sa;
+  a.srcg[is] = sg;
+}
+
+// ---------------------------------------------------------------------------
+// increments (ux.c:335-344), implicit x solve of the predictor (tdm.c:17-50
+// via solve_in_x, ux.c:251-272) and the field update (ux.c:376-386).
+//
+// The tri-diagonal matrix (I - prefactor Lx) is the same for every line, so
+// the elimination factors the reference recomputes per line are prepared
+// once on the host with the same arithmetic: tab = {c0, rlast, flag, -,
+// l[n], val[n], v[n]}.  A work-group of 64 lanes solves 64 consecutive
+// lines; x is streamed through LDS in chunks so that global accesses stay
+// contiguous while each lane walks its own line.
+constexpr int kTLines = 64;
+constexpr int kTChunk = 32;
+
+__global__ __launch_bounds__(kTLines) void k_thomas_lines(
+    const double * __restrict__ tab, const double * __restrict__ srca,
+    const double * __restrict__ srcb, const double * __restrict__ srcg, double ca, double cb,
+    double cg, double * __restrict__ q, double * __restrict__ field, int n, int nlines,
+    int fld_i0, int fld_n0, int fld_n1, int fld_j0, int ny) {
+  __shared__ double tile[kTLines][kTChunk + 1];
+  const int lane = threadIdx.x;
+  const int line0 = blockIdx.x * kTLines;
+  const int myline = line0 + lane;
+  const double c0 = tab[0], rlast = tab[1];
+  const int flag = tab[2] != 0.;
+  const double * L = tab + 4;
+  const double * VAL = tab + 4 + n;
+  const double * V = tab + 4 + 2 * n;
+  // forward elimination
+  double qprev = 0.;
+  for (int c = 0; c < n; c += kTChunk) {
+    const int w = min(kTChunk, n - c);
+    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
+      const int ll = e / kTChunk, m = e % kTChunk;
+      if (m < w && line0 + ll < nlines) {
+        const size_t g = (size_t)(line0 + ll) * n + c + m;
+        tile[ll][m] = +ca * srca[g] + cb * srcb[g] + cg * srcg[g];
+      }
+    }
+    __syncthreads();
+    if (myline < nlines) {
+      for (int m = 0; m < w; m++) {
+        const int ii = c + m;
+        double qi = tile[lane][m];
+        if (ii == 0) {
+          qi = qi / c0;
+        } else if (ii < n - 1) {
+          qi = VAL[ii] * (qi - L[ii] * qprev);
+        } else {
+          qi = flag ? rlast * (qi - L[ii] * qprev) : 0.;
+        }
+        tile[lane][m] = qi;
+        qprev = qi;
+      }
+    }
+    __syncthreads();
+    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
+      const int ll = e / kTChunk, m = e % kTChunk;
+      if (m < w && line0 + ll < nlines) q[(size_t)(line0 + ll) * n + c + m] = tile[ll][m];
+    }
+    __syncthreads();
+  }
+  // backward substitution fused with field += q
+  __shared__ double ftile[kTLines][kTChunk + 1];
+  double qnext = 0.;
+  const int last_c = ((n - 1) / kTChunk) * kTChunk;
+  for (int c = last_c; c >= 0; c -= kTChunk) {
+    const int w = min(kTChunk, n - c);
+    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
+      const int ll = e / kTChunk, m = e % kTChunk;
+      const int line = line0 + ll;
+      if (m < w && line < nlines) {
+        tile[ll][m] = q[(size_t)line * n + c + m];
+        const int jj = line % ny, kk = line / ny;
+        ftile[ll][m] = field[(size_t)(fld_i0 + c + m) + (size_t)fld_n0 * ((size_t)(jj + fld_j0) + (size_t)fld_n1 * (size_t)(kk + fld_j0))];
+      }
+    }
+    __syncthreads();
+    if (myline < nlines) {
+      for (int m = w - 1; m >= 0; m--) {
+        const int ii = c + m;
+        double qi = tile[lane][m];
+        if (ii < n - 1) qi -= V[ii] * qnext;
+        qnext = qi;
+        ftile[lane][m] += qi;
+      }
+    }
+    __syncthreads();
+    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
+      const int ll = e / kTChunk, m = e % kTChunk;
+      const int line = line0 + ll;
+      if (m < w && line < nlines) {
+        const int jj = line % ny, kk = line / ny;
+        field[(size_t)(fld_i0 + c + m) + (size_t)fld_n0 * ((size_t)(jj + fld_j0) + (size_t)fld_n1 * (size_t)(kk + fld_j0))] = ftile[ll][m];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson solver: FFT in y (and z), Thomas in x (compute_potential/dft.c).
+//
+// Spectral layout spec[x][kz][ky] (complex, ky fastest, nh = ny/2+1): the
+// y transform writes contiguous ky rows per x column, the z transform reads
+// contiguous ky runs per kz, and the x solve walks x with one lane per
+// (ky, kz) line so that a wavefront touches consecutive addresses.  The
+// reference's pencil transposes (SimpleDecomp) are folded into the LDS
+// staging of these kernels.
+
+struct FftPlan {
+  int n;
+  int nst;
+  int rad[16];
+};
+
+// Stockham autosort FFT of `ncols` LDS columns of length P.n, in place
+// between buffers a and b; returns the buffer holding the result.
+// out[k] = sum_j in[j] exp(sign 2 pi i j k / n); tw[m] = (cos, sin)(2 pi m / n)
+__device__ double2 * lds_fft(double2 * a, double2 * b, int ncols, const FftPlan & P,
+                             const double2 * __restrict__ tw, double sign) {
+  const int N = P.n;
+  int Ns = 1;
+  for (int s = 0; s < P.nst; s++) {
+    const int R = P.rad[s];
+    const int nb = N / R;
+    const int tstride = N / (Ns * R);
+    for (int t = threadIdx.x; t < ncols * nb; t += blockDim.x) {
+      const int col = t / nb, jj = t - col * nb;
+      const double2 * in = a + (size_t)col * N;
+      double2 * out = b + (size_t)col * N;
+      const int kk = jj % Ns;
+      double2 v[5];
+      for (int r = 0; r < R; r++) {
+        const double2 x = in[jj + r * nb];
+        if (r == 0 || kk == 0) {
+          v[r] = x;
+        } else {
+          const double2 w = tw[(kk * r * tstride) % N];
+          const double ws = sign * w.y;
+          v[r] = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
+        }
+      }
+      double2 y[5];
+      if (R == 2) {
+        y[0] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
+        y[1] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
+      } else if (R == 4) {
+        const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+        // (sign i) * d13
+        const double2 jd = make_double2(-sign * d13.y, sign * d13.x);
+        y[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+        y[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+        y[1] = make_double2(d02.x + jd.x, d02.y + jd.y);
+        y[3] = make_double2(d02.x - jd.x, d02.y - jd.y);
+      } else if (R == 3) {
+        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+        const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
+        const double2 t2 = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
+        const double2 m = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+        // s3 * i * t2
+        const double2 r2 = make_double2(-s3 * t2.y, s3 * t2.x);
+        y[0] = make_double2(v[0].x + t1.x, v[0].y + t1.y);
+        y[1] = make_double2(m.x + r2.x, m.y + r2.y);
+        y[2] = make_double2(m.x - r2.x, m.y - r2.y);
+      } else {  // R == 5
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
+        const double2 a1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
+        const double2 b1 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
+        const double2 a2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
+        const double2 b2 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
+        y[0] = make_double2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+        const double2 m1 = make_double2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        const double2 m2 = make_double2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        // i * (s1 b1 + s2 b2) and i * (s2 b1 - s1 b2)
+        const double2 n1 = make_double2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+        const double2 n2 = make_double2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+        y[1] = make_double2(m1.x + n1.x, m1.y + n1.y);
+        y[4] = make_double2(m1.x - n1.x, m1.y - n1.y);
+        y[2] = make_double2(m2.x + n2.x, m2.y + n2.y);
+        y[3] = make_double2(m2.x - n2.x, m2.y - n2.y);
+      }
+      const int idxD = (jj / Ns) * Ns * R + kk;
+      for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
+    }
+    __syncthreads();
+    double2 * tmp = a; a = b; b = tmp;
+    Ns *= R;
+  }
+  return a;
+}
+
+constexpr int kFftThreads = 256;
+constexpr int kFftElems = 2048;   // complex elements per LDS buffer
+
+// assign_input (dft.c:315-345) fused with the real transform in y.  Two x
+// columns are packed into one complex column; the r2c halves are separated
+// after the transform.  Output spec[x][k][ky].
+__global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
+    Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
+    const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
+    double2 * __restrict__ spec) {
+  __shared__ double2 sa[kFftElems], sb[kFftElems];
+  const int nx = g.nx, ny = g.ny;
+  const int N = ny;
+  const int NC = kFftElems / N;            // complex columns per block
+  const int x0 = blockIdx.x * 2 * NC;      // first interior x (0-based)
+#if NDIMS == 2
+  const int k = 1;
+  const int nz = 1;
+  (void)uz;
+#else
+  const int k = (int)blockIdx.y + 1;
+  const int nz = g.nz;
+#endif
+  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
+    const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+    const int i = x0 + xl + 1;
+    double r = 0.;
+    if (i <= nx) {
+      const double dx = g.dxf[i - 1];
+      const double ux_xm = ux[I_UX(i, j, k)], ux_xp = ux[I_UX(i + 1, j, k)];
+      const double uy_ym = uy[I_C1(i, j, k)], uy_yp = uy[I_C1(i, j + 1, k)];
+#if NDIMS == 2
+      r = prefactor * ((ux_xp - ux_xm) / dx + (uy_yp - uy_ym) / g.dy);
+#else
+      const double uz_zm = uz[I_C1(i, j, k)], uz_zp = uz[I_C1(i, j, k + 1)];
+      r = prefactor * ((ux_xp - ux_xm) / dx + (uy_yp - uy_ym) / g.dy + (uz_zp - uz_zm) / g.dz);
+#endif
+    }
+    double * slot = reinterpret_cast<double *>(&sa[(xl >> 1) * N + (j - 1)]);
+    slot[xl & 1] = r;
+  }
+  __syncthreads();
+  double2 * res = lds_fft(sa, sb, NC, P, tw, -1.);
+  const int nh = N / 2 + 1;
+  for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
+    const int c = e / nh, ky = e % nh;
+    const double2 z = res[c * N + ky];
+    const double2 zc = res[c * N + (N - ky) % N];
+    // A = (Z[k] + conj Z[N-k]) / 2, B = (Z[k] - conj Z[N-k]) / (2 i)
+    const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
+    const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+    const int xa = x0 + 2 * c, xb = xa + 1;
+    if (xa < nx) spec[((size_t)xa * nz + (k - 1)) * nh + ky] = A;
+    if (xb < nx) spec[((size_t)xb * nz + (k - 1)) * nh + ky] = B;
+  }
+}
+
+#if NDIMS == 3
+// complex transform in z of spec[x][kz][ky] for a run of ky at one x
+__global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const double2 * __restrict__ tw,
+                                                      double2 * __restrict__ spec, int nh, double sign) {
+  __shared__ double2 sa[kFftElems], sb[kFftElems];
+  const int N = g.nz;
+  const int NC = kFftElems / N;
+  const int ky0 = blockIdx.x * NC;
+  const int x = blockIdx.y;
+  double2 * base = spec + (size_t)x * N * nh;
+  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
+    const int c = e % NC, kz = e / NC;
+    const int ky = ky0 + c;
+    sa[c * N + kz] = ky < nh ? base[(size_t)kz * nh + ky] : make_double2(0., 0.);
+  }
+  __syncthreads();
+  double2 * res = lds_fft(sa, sb, NC, P, tw, sign);
+  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
+    const int c = e % NC, kz = e / NC;
+    const int ky = ky0 + c;
+    if (ky < nh) base[(size_t)kz * nh + ky] = res[c * N + kz];
+  }
+}
+#endif
+
+// solve_linear_systems (dft.c:364-395): per (ky, kz) line the complex
+// tri-diagonal system in x with c = -l - u + lambda and the Neumann fold,
+// eliminated as gtsv (tdm.c:17-50).  v goes to scratch vbuf[x][line].
+__global__ void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
+                              const double * __restrict__ pu, const double * __restrict__ ey,
+                              const double * __restrict__ ez, double2 * __restrict__ spec,
+                              double * __restrict__ vbuf) {
+  const int line = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (line >= nlines) return;
+  const int ky = line % nh;
+#if NDIMS == 2
+  const double eval = ey[ky];
+  (void)ez;
+#else
+  const int kz = line / nh;
+  const double eval = ey[ky] + ez[kz];
+#endif
+  const size_t st = (size_t)nlines;
+  double2 * q = spec + line;
+  double * v = vbuf + line;
+  const int n = nx;
+  // row 0 (with the Neumann fold c[0] += l[0])
+  double cc = -pl[0] - pu[0] + eval;
+  cc += pl[0];
+  double vprev = pu[0] / cc;
+  double2 qprev = q[0];
+  qprev.x = qprev.x / cc;
+  qprev.y = qprev.y / cc;
+  q[0] = qprev;
+  v[0] = vprev;
+  for (int i = 1; i < n - 1; i++) {
+    const double c = -pl[i] - pu[i] + eval;
+    const double val = 1. / (c - pl[i] * vprev);
+    const double vi = val * (pu[i]);
+    double2 qi = q[(size_t)i * st];
+    qi.x = val * (qi.x - pl[i] * qprev.x);
+    qi.y = val * (qi.y - pl[i] * qprev.y);
+    q[(size_t)i * st] = qi;
+    v[(size_t)i * st] = vi;
+    qprev = qi;
+    vprev = vi;
+  }
+  {
+    const int i = n - 1;
+    double c = -pl[i] - pu[i] + eval;
+    c += pu[i];
+    const double val = c - pl[i] * vprev;
+    double2 qi = q[(size_t)i * st];
+    if (fabs(val) > DBL_EPSILON) {
+      const double r = 1. / val;
+      qi.x = r * (qi.x - pl[i] * qprev.x);
+      qi.y = r * (qi.y - pl[i] * qprev.y);
+    } else {
+      qi.x = 0.;
+      qi.y = 0.;
+    }
+    q[(size_t)i * st] = qi;
+    qprev = qi;
+  }
+  for (int i = n - 2; i >= 0; i--) {
+    double2 qi = q[(size_t)i * st];
+    const double vi = v[(size_t)i * st];
+    qi.x -= vi * qprev.x;
+    qi.y -= vi * qprev.y;
+    q[(size_t)i * st] = qi;
+    qprev = qi;
+  }
+}
+
+// inverse real transform in y (c2r: the imaginary parts of the k = 0 and
+// k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
+__global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
+    Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
+    double * __restrict__ psi) {
+  __shared__ double2 sa[kFftElems], sb[kFftElems];
+  const int nx = g.nx, ny = g.ny;
+  const int N = ny;
+  const int NC = kFftElems / N;
+  const int x0 = blockIdx.x * 2 * NC;
+  const int nh = N / 2 + 1;
+#if NDIMS == 2
+  const int k = 1;
+  const int nz = 1;
+#else
+  const int k = (int)blockIdx.y + 1;
+  const int nz = g.nz;
+#endif
+  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
+    const int c = e / N, ky = e % N;
+    const int xa = x0 + 2 * c, xb = xa + 1;
+    const int kk = ky <= N / 2 ? ky : N - ky;
+    const double cs = ky <= N / 2 ? 1. : -1.;   // conjugate for the upper half
+    double2 A = make_double2(0., 0.), B = make_double2(0., 0.);
+    if (xa < nx) A = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
+    if (xb < nx) B = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
+    if (kk == 0 || 2 * kk == N) { A.y = 0.; B.y = 0.; }
+    A.y *= cs;
+    B.y *= cs;
+    // Z = A + i B
+    sa[c * N + ky] = make_double2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  double2 * res = lds_fft(sa, sb, NC, P, tw, +1.);
+  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
+    const int xl = e % (2 * NC), j = e / (2 * NC);
+    const int i = x0 + xl + 1;
+    if (i <= nx) {
+      const double2 z = res[(xl >> 1) * N + j];
+      psi[I_C1(i, j + 1, k)] = (xl & 1) ? z.y : z.x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fluid_correct_velocity (correct_velocity/{ux,uy}.c) for one component
+__global__ void k_correct(Grid g, double prefactor, const double * __restrict__ psi,
+                          double * __restrict__ ux, double * __restrict__ uy,
+                          double * __restrict__ uz) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double pc = psi[I_C1(i, j, k)];
+  if (i >= 2) {
+    const double dx = g.dxc[i - 1];
+    ux[I_UX(i, j, k)] -= prefactor / dx * (+pc - psi[I_C1(i - 1, j, k)]);
+  }
+  uy[I_C1(i, j, k)] -= prefactor / g.dy * (+pc - psi[I_C1(i, j - 1, k)]);
+#if NDIMS == 3
+  uz[I_C1(i, j, k)] -= prefactor / g.dz * (+pc - psi[I_C1(i, j, k - 1)]);
+#else
+  (void)uz;
+#endif
+}
+
+// fluid_update_pressure (update_pressure.c:12-109): p += psi, then the
+// implicit-x correction
+__global__ void k_pressure(Grid g, double prefactor, const double * __restrict__ psi,
+                           double * __restrict__ p) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double pc = psi[I_C1(i, j, k)];
+  const double pm = psi[I_C1(i - 1, j, k)], pp = psi[I_C1(i + 1, j, k)];
+  double v = p[I_C1(i, j, k)];
+  v += pc;
+  const double dpsidx_xm = (-pm + pc) / g.dxc[i - 1];
+  const double dpsidx_xp = (-pc + pp) / g.dxc[i];
+  v -= prefactor / g.dxf[i - 1] * (-dpsidx_xm + dpsidx_xp);
+  p[I_C1(i, j, k)] = v;
+}
+
+// ---------------------------------------------------------------------------
+// fluid_decide_dt (decide_dt.c:28-64): min over DOFs of spacing / (|u| + 1e-8)
+constexpr int kRedThreads = 256;
+
+__device__ inline double block_min(double v) {
+  __shared__ double red[kRedThreads];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double * __restrict__ ux,
+                                                            const double * __restrict__ uy,
+                                                            const double * __restrict__ uz,
+                                                            double * __restrict__ part) {
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 2
+  const size_t ncell = (size_t)nx * ny;
+  (void)uz;
+#else
+  const size_t ncell = (size_t)nx * ny * g.nz;
+#endif
+  const double small = 1.e-8;
+  double m = 1.;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nx) + 1;
+    const size_t r = t / nx;
+    const int j = (int)(r % ny) + 1;
+#if NDIMS == 2
+    const int k = 1;
+#else
+    const int k = (int)(r / ny) + 1;
+#endif
+    if (i >= 2) m = fmin(m, g.dxc[i - 1] / (fabs(ux[I_UX(i, j, k)]) + small));
+    m = fmin(m, g.dy / (fabs(uy[I_C1(i, j, k)]) + small));
+#if NDIMS == 3
+    m = fmin(m, g.dz / (fabs(uz[I_C1(i, j, k)]) + small));
+#endif
+  }
+  m = block_min(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_min_final(const double * __restrict__ part, int n,
+                                                           double * __restrict__ out) {
+  double m = 1.;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmin(m, part[t]);
+  m = block_min(m);
+  if (threadIdx.x == 0) out[0] = m;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics (logging/divergence.c, interface.c, energy.c): per-block
+// partials {max|div|, vof min, vof max, sum vof dV, sum dV, KE x, y, z, TE}
+constexpr int kDiagN = 9;
+
+__global__ __launch_bounds__(kRedThreads) void k_diag_partial(
+    Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+    const double * __restrict__ uz, const double * __restrict__ t,
+    const double * __restrict__ vof, double * __restrict__ part) {
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 2
+  const size_t ncell = (size_t)nx * ny;
+  const double dyz = g.dy;
+  (void)uz;
+#else
+  const size_t ncell = (size_t)nx * ny * g.nz;
+  const double dyz = g.dy * g.dz;
+#endif
+  double q[kDiagN] = {0., 1., 0., 0., 0., 0., 0., 0., 0.};
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < ncell;
+       s += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(s % nx) + 1;
+    const size_t r = s / nx;
+    const int j = (int)(r % ny) + 1;
+#if NDIMS == 2
+    const int k = 1;
+#else
+    const int k = (int)(r / ny) + 1;
+#endif
+    const double dx = g.dxf[i - 1];
+    double div = (ux[I_UX(i + 1, j, k)] - ux[I_UX(i, j, k)]) / dx +
+                 (uy[I_C1(i, j + 1, k)] - uy[I_C1(i, j, k)]) / g.dy;
+#if NDIMS == 3
+    div += (uz[I_C1(i, j, k + 1)] - uz[I_C1(i, j, k)]) / g.dz;
+    const double w = uz[I_C1(i, j, k)];
+    q[7] += 0.5 * (w * w) * (dx * dyz);
+#endif
+    q[0] = fmax(q[0], fabs(div));
+    const double lv = vof[I_VOF(i, j, k)];
+    q[1] = fmin(q[1], lv);
+    q[2] = fmax(q[2], lv);
+    q[3] += lv * (dx * dyz);
+    q[4] += dx * dyz;
+    if (i >= 2) {
+      const double u = ux[I_UX(i, j, k)];
+      q[5] += 0.5 * (u * u) * (g.dxc[i - 1] * dyz);
+    }
+    const double v = uy[I_C1(i, j, k)];
+    q[6] += 0.5 * (v * v) * (dx * dyz);
+    const double tt = t[I_C1(i, j, k)];
+    q[8] += 0.5 * (tt * tt) * (dx * dyz);
+  }
+  __shared__ double red[kDiagN][kRedThreads];
+  for (int m = 0; m < kDiagN; m++) red[m][threadIdx.x] = q[m];
+  __syncthreads();
+  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const int o = threadIdx.x + st;
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][o]);
+      red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][o]);
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][o]);
+      for (int m = 3; m < kDiagN; m++) red[m][threadIdx.x] += red[m][o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int m = 0; m < kDiagN; m++) part[(size_t)blockIdx.x * kDiagN + m] = red[m][0];
+}
+
+}  // namespace vofk

@@ -1,0 +1,1536 @@
+// vof_lib.hip -- host side of the gfx950 SimpleVOFSolver time step: the
+// reference's entry points (include/vof_solver.h) over device-resident
+// arrays, the NPY codec, and the launch sequences of the kernels in
+// vof_kernels.h.  Built once per NDIMS into libvofhip{2,3}d.so.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <float.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <errno.h>
+
+#include <string>
+#include <vector>
+
+#include "vof_solver.h"
+#include "vof_kernels.h"
+
+using vofk::Grid;
+using vofk::FftPlan;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+
+static int fail(const char * fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return 1;
+}
+
+#define HIPCHK(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) return fail("%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                                      hipGetErrorString(e_));                      \
+  } while (0)
+
+#define LAUNCHCHK()                                                                \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) return fail("%s:%d kernel launch: %s", __FILE__, __LINE__, \
+                                      hipGetErrorString(e_));                      \
+  } while (0)
+
+extern "C" const char * vof_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// NPY v1 codec (the format checks of src/fileio.c:185-245: dtype, shape,
+// C order)
+
+namespace npy {
+
+static std::string path_of(const char * dir, const char * name) {
+  std::string p(dir);
+  if (!p.empty() && p.back() != '/') p += '/';
+  p += name;
+  p += ".npy";
+  return p;
+}
+
+// returns 0 and fills data (nbytes) when the header matches exactly
+static int read(const char * dir, const char * name, const char * dtype,
+                const std::vector<size_t> & shape, void * data, size_t elsize) {
+  const std::string p = path_of(dir, name);
+  FILE * fp = fopen(p.c_str(), "rb");
+  if (!fp) return fail("%s: %s", p.c_str(), strerror(errno));
+  unsigned char pre[10];
+  if (fread(pre, 1, 10, fp) != 10 || memcmp(pre, "\x93NUMPY", 6) != 0) {
+    fclose(fp);
+    return fail("%s: not an NPY file", p.c_str());
+  }
+  const int major = pre[6];
+  size_t hlen = 0, pre_len = 10;
+  if (major == 1) {
+    hlen = (size_t)pre[8] | ((size_t)pre[9] << 8);
+  } else if (major == 2 || major == 3) {
+    unsigned char b[2];
+    if (fread(b, 1, 2, fp) != 2) { fclose(fp); return fail("%s: short header", p.c_str()); }
+    hlen = (size_t)pre[8] | ((size_t)pre[9] << 8) | ((size_t)b[0] << 16) | ((size_t)b[1] << 24);
+    pre_len = 12;
+  } else {
+    fclose(fp);
+    return fail("%s: unsupported NPY version %d", p.c_str(), major);
+  }
+  std::string hdr(hlen, '\0');
+  if (fread(&hdr[0], 1, hlen, fp) != hlen) { fclose(fp); return fail("%s: short header", p.c_str()); }
+  // descr
+  const size_t d = hdr.find("'descr'");
+  const size_t f = hdr.find("'fortran_order'");
+  const size_t s = hdr.find("'shape'");
+  if (d == std::string::npos || f == std::string::npos || s == std::string::npos) {
+    fclose(fp);
+    return fail("%s: malformed header", p.c_str());
+  }
+  {
+    const size_t q0 = hdr.find('\'', d + 7);
+    const size_t q1 = hdr.find('\'', q0 + 1);
+    const std::string dt = hdr.substr(q0, q1 - q0 + 1);
+    if (dt != dtype) {
+      fclose(fp);
+      return fail("%s: dtype %s expected, %s obtained", p.c_str(), dtype, dt.c_str());
+    }
+  }
+  {
+    const size_t c = hdr.find(':', f);
+    size_t v = hdr.find_first_not_of(" ", c + 1);
+    if (hdr.compare(v, 5, "False") != 0) {
+      fclose(fp);
+      return fail("%s: Fortran order is not supported", p.c_str());
+    }
+  }
+  {
+    const size_t o = hdr.find('(', s);
+    const size_t e = hdr.find(')', o);
+    std::vector<size_t> got;
+    size_t pos = o + 1;
+    while (pos < e) {
+      while (pos < e && (hdr[pos] == ' ' || hdr[pos] == ',')) pos++;
+      if (pos >= e) break;
+      got.push_back((size_t)strtoull(hdr.c_str() + pos, nullptr, 10));
+      while (pos < e && hdr[pos] != ',') pos++;
+    }
+    if (got != shape) {
+      std::string a, b;
+      for (size_t x : shape) a += std::to_string(x) + ",";
+      for (size_t x : got) b += std::to_string(x) + ",";
+      fclose(fp);
+      return fail("%s: shape (%s) expected, (%s) obtained", p.c_str(), a.c_str(), b.c_str());
+    }
+  }
+  (void)pre_len;
+  size_t n = 1;
+  for (size_t x : shape) n *= x;
+  if (fread(data, elsize, n, fp) != n) {
+    fclose(fp);
+    return fail("%s: data truncated", p.c_str());
+  }
+  fclose(fp);
+  return 0;
+}
+
+static int write(const char * dir, const char * name, const char * dtype,
+                 const std::vector<size_t> & shape, const void * data, size_t elsize) {
+  const std::string p = path_of(dir, name);
+  std::string sh = "(";
+  for (size_t m = 0; m < shape.size(); m++) {
+    sh += std::to_string(shape[m]);
+    if (shape.size() == 1 || m + 1 < shape.size()) sh += ",";
+    if (m + 1 < shape.size()) sh += " ";
+  }
+  sh += ")";
+  std::string hdr = std::string("{'descr': ") + dtype + ", 'fortran_order': False, 'shape': " + sh + ", }";
+  // pad with spaces so that magic + length + header + '\n' is a multiple of 64
+  const size_t base = 10 + hdr.size() + 1;
+  hdr.append((64 - base % 64) % 64, ' ');
+  hdr += '\n';
+  FILE * fp = fopen(p.c_str(), "wb");
+  if (!fp) return fail("%s: %s", p.c_str(), strerror(errno));
+  unsigned char pre[10] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0, 0, 0};
+  pre[8] = (unsigned char)(hdr.size() & 0xff);
+  pre[9] = (unsigned char)((hdr.size() >> 8) & 0xff);
+  size_t n = 1;
+  for (size_t x : shape) n *= x;
+  const bool ok = fwrite(pre, 1, 10, fp) == 10 && fwrite(hdr.data(), 1, hdr.size(), fp) == hdr.size() &&
+                  fwrite(data, elsize, n, fp) == n;
+  fclose(fp);
+  if (!ok) return fail("%s: write failed", p.c_str());
+  return 0;
+}
+
+}  // namespace npy
+
+static const char kNpyDouble[] = "'<f8'";
+static const char kNpySizeT[] = "'<u8'";
+
+// ---------------------------------------------------------------------------
+// solver context (lives behind domain_t.info)
+
+enum {
+  P_RA, P_PR, P_WE, P_COEF_ADV, P_COEF_DIF, P_BUOY, P_UY_XM, P_UY_XP, P_UZ_XM, P_UZ_XP,
+  P_T_XM, P_T_XP, P_COUNT
+};
+static const char * const kParamNames[P_COUNT] = {
+  "Ra", "Pr", "We", "coef_dt_adv", "coef_dt_dif", "add_buoyancy", "uy_xm", "uy_xp",
+  "uz_xm", "uz_xp", "t_xm", "t_xp",
+};
+
+struct vof_ctx {
+  int nx, ny, nz;              // nz = 1 in 2D
+  double dy, dz;
+  bool x_uniform;
+  hipStream_t stream;
+  double prm[P_COUNT];
+  bool prm_set[P_COUNT];
+  int schedule;
+  Grid grid;
+  double * d_dxf;
+  double * d_dxc;
+  std::vector<double> lapx_ux, lapx_c;   // 3 per point, predict/{ux,uy,t}.c init_lap
+  double * d_lapx_ux;
+  double * d_lapx_c;
+  // predictor Thomas tables: one slot per (field, rkstep)
+  size_t tab_stride;
+  double * d_tab;
+  double * h_tab;                        // pinned
+  hipEvent_t tab_done[4 * 3];
+  // Poisson
+  int nh;
+  FftPlan plan_y, plan_z;
+  double2 * d_twy;
+  double2 * d_twz;
+  double * d_pl;                         // x operator lower / upper (dft.c:160-190)
+  double * d_pu;
+  double * d_ey;                         // eigenvalues in y, z (dft.c:263-291)
+  double * d_ez;
+  double2 * d_spec;
+  double * d_vbuf;
+  // scratch
+  double * d_delta;                      // x1 pencil of increments
+  double * d_part;
+  int npart;
+  double * h_scalar;                     // pinned
+};
+
+static vof_ctx * ctx_of(const domain_t * domain) {
+  return domain ? (vof_ctx *)domain->info : nullptr;
+}
+
+#define CTX_OR_FAIL(dom)                                                           \
+  vof_ctx * ctx = ctx_of(dom);                                                     \
+  if (!ctx) return fail("%s: domain is not initialised", __func__)
+
+static int param_get(vof_ctx * ctx, int id, double * out) {
+  if (ctx->prm_set[id]) {
+    *out = ctx->prm[id];
+    return 0;
+  }
+  // config.get_double (src/config.c:14-42)
+  const char * s = getenv(kParamNames[id]);
+  if (!s) return fail("%s not found (environment variable or vof_set_param)", kParamNames[id]);
+  errno = 0;
+  const double v = strtod(s, nullptr);
+  if (errno != 0) return fail("%s: invalid value as double", kParamNames[id]);
+  *out = v;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+
+static dim3 grid3(int ni, int nj, int nk, dim3 blk) {
+  return dim3((unsigned)((ni + blk.x - 1) / blk.x), (unsigned)((nj + blk.y - 1) / blk.y),
+              (unsigned)(NDIMS == 3 ? nk : 1));
+}
+
+static const dim3 kBlk(64, 4, 1);
+
+// ---------------------------------------------------------------------------
+// boundary conditions: halo copy, then x walls (src/halo.c:18-76 followed by
+// assign_boundary_conditions_in_x of each fluid/boundary/*.c)
+
+enum { W_DIRICHLET = 0, W_NEUMANN = 1 };
+
+static int bc_apply(vof_ctx * ctx, double * a, int n0, int h, int walls_all_rows, int mode,
+                    double vlo, double vhi, int xlo, int xhi) {
+  const int ny = ctx->ny, nz = ctx->nz;
+  const int n1 = ny + 2 * h;
+#if NDIMS == 2
+  const size_t cnt = (size_t)n0 * 2 * h;
+#else
+  const size_t cnt = (size_t)n0 * 2 * h * (nz + 2 * h) + (size_t)n0 * ny * 2 * h;
+#endif
+  {
+    const unsigned nb = (unsigned)((cnt + 255) / 256);
+    hipLaunchKernelGGL(vofk::k_halo, dim3(nb), dim3(256), 0, ctx->stream, a, n0, ny, nz, h);
+    LAUNCHCHK();
+  }
+  int r0, nr, p0, np;
+  if (walls_all_rows) {
+    r0 = 0; nr = n1;
+#if NDIMS == 2
+    p0 = 0; np = 1;
+#else
+    p0 = 0; np = nz + 2 * h;
+#endif
+  } else {
+    r0 = h; nr = ny;
+#if NDIMS == 2
+    p0 = 0; np = 1;
+#else
+    p0 = h; np = nz;
+#endif
+  }
+  const int tot = nr * np;
+  hipLaunchKernelGGL(vofk::k_walls, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, a, n0,
+                     NDIMS == 3 ? n1 : 0, r0, nr, p0, np, xlo, xhi, mode, vlo, vhi);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int bc_ux(vof_ctx * ctx, double * ux) {
+  return bc_apply(ctx, ux, ctx->nx + 1, 1, 0, W_DIRICHLET, 0., 0., 0, ctx->nx);
+}
+static int bc_uy(vof_ctx * ctx, double * uy) {
+  return bc_apply(ctx, uy, ctx->nx + 2, 1, 0, W_DIRICHLET, ctx->prm[P_UY_XM], ctx->prm[P_UY_XP],
+                  0, ctx->nx + 1);
+}
+static int bc_uz(vof_ctx * ctx, double * uz) {
+  return bc_apply(ctx, uz, ctx->nx + 2, 1, 0, W_DIRICHLET, ctx->prm[P_UZ_XM], ctx->prm[P_UZ_XP],
+                  0, ctx->nx + 1);
+}
+static int bc_neumann(vof_ctx * ctx, double * p) {
+  return bc_apply(ctx, p, ctx->nx + 2, 1, 0, W_NEUMANN, 0., 0., 0, ctx->nx + 1);
+}
+static int bc_t(vof_ctx * ctx, double * t) {
+  return bc_apply(ctx, t, ctx->nx + 2, 1, 0, W_DIRICHLET, ctx->prm[P_T_XM], ctx->prm[P_T_XP], 0,
+                  ctx->nx + 1);
+}
+static int bc_vof(vof_ctx * ctx, double * vof) {
+  return bc_apply(ctx, vof, ctx->nx + 2, 2, 1, W_DIRICHLET, 0., 0., 0, ctx->nx + 1);
+}
+
+// ---------------------------------------------------------------------------
+// arrays (src/array.c:10-46): padded sizes from the nadds tables
+
+struct Nadds { int v[3][2]; };
+
+static Nadds nadds_of(const char * name) {
+  // tools/define_arrays.py:201-229
+  struct E { const char * n; int a[3][2]; };
+  static const E tab[] = {
+    {"ux", {{0, 1}, {1, 1}, {1, 1}}},     {"uy", {{1, 1}, {1, 1}, {1, 1}}},
+    {"uz", {{1, 1}, {1, 1}, {1, 1}}},     {"p", {{1, 1}, {1, 1}, {1, 1}}},
+    {"t", {{1, 1}, {1, 1}, {1, 1}}},      {"psi", {{1, 1}, {1, 1}, {1, 1}}},
+    {"srcux", {{-1, 0}, {0, 0}, {0, 0}}}, {"srcuy", {{0, 0}, {0, 0}, {0, 0}}},
+    {"srcuz", {{0, 0}, {0, 0}, {0, 0}}},  {"srct", {{0, 0}, {0, 0}, {0, 0}}},
+    {"ifrcx", {{-1, 0}, {0, 0}, {0, 0}}}, {"ifrcy", {{0, 0}, {0, 0}, {0, 0}}},
+    {"ifrcz", {{0, 0}, {0, 0}, {0, 0}}},  {"vof", {{1, 1}, {2, 2}, {2, 2}}},
+    {"curv", {{1, 1}, {1, 1}, {1, 1}}},   {"dvof", {{0, 1}, {1, 2}, {1, 2}}},
+    {"flxx", {{0, 1}, {0, 0}, {0, 0}}},   {"flxy", {{0, 0}, {0, 1}, {0, 0}}},
+    {"flxz", {{0, 0}, {0, 0}, {0, 1}}},   {"normal", {{0, 0}, {1, 1}, {1, 1}}},
+    {"src", {{0, 0}, {0, 0}, {0, 0}}},
+  };
+  Nadds r{};
+  for (const E & e : tab)
+    if (!strcmp(e.n, name)) {
+      memcpy(r.v, e.a, sizeof r.v);
+      return r;
+    }
+  return r;
+}
+
+static int array_prepare(const domain_t * domain, const char * name, size_t elsize, array_t * a) {
+  vof_ctx * ctx = ctx_of(domain);
+  const Nadds na = nadds_of(name);
+  size_t nitems = 1;
+  a->nadds = (int (*)[2])calloc(NDIMS, sizeof(int[2]));
+  if (!a->nadds) return fail("calloc failed");
+  for (int d = 0; d < NDIMS; d++) {
+    nitems *= domain->mysizes[d] + na.v[d][0] + na.v[d][1];
+    a->nadds[d][0] = na.v[d][0];
+    a->nadds[d][1] = na.v[d][1];
+  }
+  a->size = elsize;
+  a->datasize = nitems * elsize;
+  a->data = nullptr;
+  HIPCHK(hipMalloc(&a->data, a->datasize));
+  HIPCHK(hipMemsetAsync(a->data, 0, a->datasize, ctx->stream));
+  return 0;
+}
+
+static int array_free(array_t * a) {
+  if (a->data) hipFree(a->data);
+  free(a->nadds);
+  a->data = nullptr;
+  a->nadds = nullptr;
+  return 0;
+}
+
+static size_t padded_x(const array_t * a, const domain_t * d) {
+  return d->mysizes[0] + a->nadds[0][0] + a->nadds[0][1];
+}
+
+// array.load / array.dump (src/array.c:80-206): the NPY holds the interior
+// y (z) rows with the x padding
+static int array_load(const domain_t * domain, const char * dir, const char * name, array_t * a) {
+  vof_ctx * ctx = ctx_of(domain);
+  const size_t n0 = padded_x(a, domain);
+  std::vector<size_t> shape;
+#if NDIMS == 3
+  shape.push_back(domain->glsizes[2]);
+#endif
+  shape.push_back(domain->glsizes[1]);
+  shape.push_back(n0);
+  size_t nrow = domain->glsizes[1];
+#if NDIMS == 3
+  nrow *= domain->glsizes[2];
+#endif
+  std::vector<double> buf(n0 * nrow);
+  if (npy::read(dir, name, kNpyDouble, shape, buf.data(), sizeof(double))) return 1;
+  std::vector<double> full(a->datasize / sizeof(double), 0.);
+  HIPCHK(hipMemcpyAsync(full.data(), a->data, a->datasize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const size_t n1 = domain->mysizes[1] + a->nadds[1][0] + a->nadds[1][1];
+  const int hy = a->nadds[1][0];
+#if NDIMS == 3
+  const int hz = a->nadds[2][0];
+  const size_t nzz = domain->glsizes[2];
+#else
+  const int hz = 0;
+  const size_t nzz = 1;
+#endif
+  for (size_t k = 0; k < nzz; k++)
+    for (size_t j = 0; j < domain->glsizes[1]; j++)
+      memcpy(&full[n0 * ((j + hy) + n1 * (k + hz))], &buf[n0 * (j + domain->glsizes[1] * k)],
+             n0 * sizeof(double));
+  HIPCHK(hipMemcpyAsync(a->data, full.data(), a->datasize, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+static int array_dump(const domain_t * domain, const char * dir, const char * name, const array_t * a) {
+  vof_ctx * ctx = ctx_of(domain);
+  const size_t n0 = padded_x(a, domain);
+  std::vector<double> full(a->datasize / sizeof(double));
+  HIPCHK(hipMemcpyAsync(full.data(), a->data, a->datasize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<size_t> shape;
+#if NDIMS == 3
+  shape.push_back(domain->glsizes[2]);
+#endif
+  shape.push_back(domain->glsizes[1]);
+  shape.push_back(n0);
+  size_t nrow = domain->glsizes[1];
+#if NDIMS == 3
+  nrow *= domain->glsizes[2];
+  const int hz = a->nadds[2][0];
+  const size_t nzz = domain->glsizes[2];
+#else
+  const int hz = 0;
+  const size_t nzz = 1;
+#endif
+  std::vector<double> buf(n0 * nrow);
+  const size_t n1 = domain->mysizes[1] + a->nadds[1][0] + a->nadds[1][1];
+  const int hy = a->nadds[1][0];
+  for (size_t k = 0; k < nzz; k++)
+    for (size_t j = 0; j < domain->glsizes[1]; j++)
+      memcpy(&buf[n0 * (j + domain->glsizes[1] * k)], &full[n0 * ((j + hy) + n1 * (k + hz))],
+             n0 * sizeof(double));
+  return npy::write(dir, name, kNpyDouble, shape, buf.data(), sizeof(double));
+}
+
+extern "C" int vof_array_to_host(const domain_t * domain, const array_t * a, void * host) {
+  CTX_OR_FAIL(domain);
+  if (!a || !a->data || !host) return fail("vof_array_to_host: null argument");
+  HIPCHK(hipMemcpyAsync(host, a->data, a->datasize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int vof_array_from_host(const domain_t * domain, array_t * a, const void * host) {
+  CTX_OR_FAIL(domain);
+  if (!a || !a->data || !host) return fail("vof_array_from_host: null argument");
+  HIPCHK(hipMemcpyAsync(a->data, host, a->datasize, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int vof_synchronize(const domain_t * domain) {
+  CTX_OR_FAIL(domain);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FFT plans and twiddles
+
+static int make_plan(int n, FftPlan * P) {
+  P->n = n;
+  P->nst = 0;
+  int m = n;
+  const int rads[4] = {4, 2, 3, 5};
+  for (int r : rads) {
+    while (m % r == 0) {
+      if (P->nst >= 16) return fail("FFT length %d has too many factors", n);
+      P->rad[P->nst++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) return fail("FFT length %d must factor into 2, 3 and 5", n);
+  if (n > vofk::kFftElems) return fail("FFT length %d exceeds %d", n, vofk::kFftElems);
+  return 0;
+}
+
+// (cos, sin)(2 pi m / n) reduced to the first octant so that symmetric
+// angles give identical magnitudes and the axes are exact
+static std::vector<double2> make_twiddles(int n) {
+  std::vector<double2> tw(n);
+  const double pi = 3.14159265358979324;
+  for (int k = 0; k < n; k++) {
+    const long m8 = 8L * k;
+    double c, s;
+    if (m8 % n == 0) {
+      static const double cs[8][2] = {
+        {1., 0.}, {0.70710678118654752440, 0.70710678118654752440}, {0., 1.},
+        {-0.70710678118654752440, 0.70710678118654752440}, {-1., 0.},
+        {-0.70710678118654752440, -0.70710678118654752440}, {0., -1.},
+        {0.70710678118654752440, -0.70710678118654752440},
+      };
+      const int oct = (int)(m8 / n);
+      c = cs[oct][0];
+      s = cs[oct][1];
+    } else {
+      const long kk = 4L * k;
+      const int q = (int)(kk / n);
+      const long rem = kk - (long)q * n;
+      double cr, sr;
+      if (2 * rem <= n) {
+        const double r = 0.5 * pi * (double)rem / n;
+        cr = cos(r); sr = sin(r);
+      } else {
+        const double r = 0.5 * pi * (double)(n - rem) / n;
+        cr = sin(r); sr = cos(r);
+      }
+      switch (q) {
+        case 0: c = cr; s = sr; break;
+        case 1: c = -sr; s = cr; break;
+        case 2: c = -cr; s = -sr; break;
+        default: c = sr; s = -cr; break;
+      }
+    }
+    tw[k] = make_double2(c, s);
+  }
+  return tw;
+}
+
+// ---------------------------------------------------------------------------
+// domain
+
+static int ctx_create(domain_t * domain) {
+  vof_ctx * ctx = new vof_ctx();
+  domain->info = ctx;
+  ctx->nx = (int)domain->glsizes[0];
+  ctx->ny = (int)domain->glsizes[1];
+#if NDIMS == 3
+  ctx->nz = (int)domain->glsizes[2];
+  ctx->dz = domain->dz;
+#else
+  ctx->nz = 1;
+  ctx->dz = 0.;
+#endif
+  ctx->dy = domain->dy;
+  ctx->schedule = 1;
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  if (nx < 4 || ny < 4 || (NDIMS == 3 && nz < 4))
+    return fail("grid %dx%dx%d too small (at least 4 cells per direction)", nx, ny, nz);
+  // reference compile-time parameters (src/param/*.c)
+  for (int m = 0; m < P_COUNT; m++) { ctx->prm[m] = 0.; ctx->prm_set[m] = false; }
+  ctx->prm[P_BUOY] = 1.;  ctx->prm_set[P_BUOY] = true;
+  ctx->prm[P_UY_XM] = 0.; ctx->prm_set[P_UY_XM] = true;
+  ctx->prm[P_UY_XP] = 0.; ctx->prm_set[P_UY_XP] = true;
+  ctx->prm[P_UZ_XM] = 0.; ctx->prm_set[P_UZ_XM] = true;
+  ctx->prm[P_UZ_XP] = 0.; ctx->prm_set[P_UZ_XP] = true;
+  ctx->prm[P_T_XM] = +0.5; ctx->prm_set[P_T_XM] = true;
+  ctx->prm[P_T_XP] = -0.5; ctx->prm_set[P_T_XP] = true;
+  HIPCHK(hipStreamCreate(&ctx->stream));
+  // metrics
+  HIPCHK(hipMalloc(&ctx->d_dxf, sizeof(double) * nx));
+  HIPCHK(hipMalloc(&ctx->d_dxc, sizeof(double) * (nx + 1)));
+  HIPCHK(hipMemcpy(ctx->d_dxf, domain->dxf, sizeof(double) * nx, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_dxc, domain->dxc, sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
+  ctx->grid = Grid{nx, ny, nz, ctx->dy, ctx->dz, ctx->d_dxf, ctx->d_dxc};
+  // Laplacians in x (predict/ux.c:35-62, uy.c:35-62, t.c:33-60)
+  const double * dxf = domain->dxf;
+  const double * dxc = domain->dxc;
+#define DXF(I) (dxf[(I) - 1])
+#define DXC(I) (dxc[(I) - 1])
+  ctx->lapx_ux.resize(3 * (nx - 1));
+  for (int i = 2; i <= nx; i++) {
+    const double l = 1. / DXF(i - 1) / DXC(i);
+    const double u = 1. / DXF(i) / DXC(i);
+    const double c = -l - u;
+    ctx->lapx_ux[3 * (i - 2) + 0] = l;
+    ctx->lapx_ux[3 * (i - 2) + 1] = c;
+    ctx->lapx_ux[3 * (i - 2) + 2] = u;
+  }
+  ctx->lapx_c.resize(3 * nx);
+  for (int i = 1; i <= nx; i++) {
+    const double l = 1. / DXC(i) / DXF(i);
+    const double u = 1. / DXC(i + 1) / DXF(i);
+    const double c = -l - u;
+    ctx->lapx_c[3 * (i - 1) + 0] = l;
+    ctx->lapx_c[3 * (i - 1) + 1] = c;
+    ctx->lapx_c[3 * (i - 1) + 2] = u;
+  }
+  HIPCHK(hipMalloc(&ctx->d_lapx_ux, sizeof(double) * 3 * (nx - 1)));
+  HIPCHK(hipMalloc(&ctx->d_lapx_c, sizeof(double) * 3 * nx));
+  HIPCHK(hipMemcpy(ctx->d_lapx_ux, ctx->lapx_ux.data(), sizeof(double) * 3 * (nx - 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_lapx_c, ctx->lapx_c.data(), sizeof(double) * 3 * nx, hipMemcpyHostToDevice));
+  // Poisson operator in x (dft.c:160-190) and eigenvalues (dft.c:263-291)
+  {
+    std::vector<double> l(nx), u(nx);
+    for (int i = 1; i <= nx; i++) {
+      l[i - 1] = 1. / DXC(i) / DXF(i);
+      u[i - 1] = 1. / DXC(i + 1) / DXF(i);
+    }
+    HIPCHK(hipMalloc(&ctx->d_pl, sizeof(double) * nx));
+    HIPCHK(hipMalloc(&ctx->d_pu, sizeof(double) * nx));
+    HIPCHK(hipMemcpy(ctx->d_pl, l.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_pu, u.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+  }
+#undef DXF
+#undef DXC
+  const double g_pi = 3.14159265358979324;
+  ctx->nh = ny / 2 + 1;
+  {
+    std::vector<double> ey(ctx->nh);
+    const double gy = domain->lengths[1] / ny;
+    for (int m = 0; m < ctx->nh; m++) {
+      const double s = sin(g_pi * m / (double)ny);
+      ey[m] = -4. / (gy * gy) * (s * s);
+    }
+    HIPCHK(hipMalloc(&ctx->d_ey, sizeof(double) * ctx->nh));
+    HIPCHK(hipMemcpy(ctx->d_ey, ey.data(), sizeof(double) * ctx->nh, hipMemcpyHostToDevice));
+  }
+  {
+    std::vector<double> ez(nz, 0.);
+#if NDIMS == 3
+    const double gz = domain->lengths[2] / nz;
+    for (int m = 0; m < nz; m++) {
+      const double s = sin(g_pi * m / (double)nz);
+      ez[m] = -4. / (gz * gz) * (s * s);
+    }
+#endif
+    HIPCHK(hipMalloc(&ctx->d_ez, sizeof(double) * nz));
+    HIPCHK(hipMemcpy(ctx->d_ez, ez.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  }
+  if (make_plan(ny, &ctx->plan_y)) return 1;
+  {
+    std::vector<double2> tw = make_twiddles(ny);
+    HIPCHK(hipMalloc(&ctx->d_twy, sizeof(double2) * ny));
+    HIPCHK(hipMemcpy(ctx->d_twy, tw.data(), sizeof(double2) * ny, hipMemcpyHostToDevice));
+  }
+#if NDIMS == 3
+  if (make_plan(nz, &ctx->plan_z)) return 1;
+  {
+    std::vector<double2> tw = make_twiddles(nz);
+    HIPCHK(hipMalloc(&ctx->d_twz, sizeof(double2) * nz));
+    HIPCHK(hipMemcpy(ctx->d_twz, tw.data(), sizeof(double2) * nz, hipMemcpyHostToDevice));
+  }
+#else
+  ctx->d_twz = nullptr;
+#endif
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nline = (size_t)ctx->nh * nz;
+  HIPCHK(hipMalloc(&ctx->d_spec, sizeof(double2) * nx * nline));
+  HIPCHK(hipMalloc(&ctx->d_vbuf, sizeof(double) * nx * nline));
+  HIPCHK(hipMalloc(&ctx->d_delta, sizeof(double) * ncell));
+  ctx->npart = 1024;
+  HIPCHK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->npart * vofk::kDiagN));
+  HIPCHK(hipHostMalloc(&ctx->h_scalar, sizeof(double) * 64));
+  // Thomas tables: header 4 + l, val, v
+  ctx->tab_stride = 4 + 3 * (size_t)nx;
+  HIPCHK(hipMalloc(&ctx->d_tab, sizeof(double) * ctx->tab_stride * 12));
+  HIPCHK(hipHostMalloc(&ctx->h_tab, sizeof(double) * ctx->tab_stride * 12));
+  for (int m = 0; m < 12; m++) HIPCHK(hipEventCreateWithFlags(&ctx->tab_done[m], hipEventDisableTiming));
+  return 0;
+}
+
+extern "C" int domain_init(const char dirname_ic[], domain_t * domain) {
+  if (!domain) return fail("domain_init: domain is NULL");
+  size_t gl[NDIMS];
+  double len[NDIMS];
+  if (npy::read(dirname_ic, "glsizes", kNpySizeT, {(size_t)NDIMS}, gl, sizeof(size_t))) return 1;
+  if (npy::read(dirname_ic, "lengths", kNpyDouble, {(size_t)NDIMS}, len, sizeof(double))) return 1;
+  for (int d = 0; d < NDIMS; d++) {
+    domain->glsizes[d] = gl[d];
+    domain->mysizes[d] = gl[d];
+    domain->offsets[d] = 0;
+    domain->lengths[d] = len[d];
+  }
+  const size_t nx = gl[0];
+  domain->xf = (double *)calloc(nx + 1, sizeof(double));
+  domain->xc = (double *)calloc(nx + 2, sizeof(double));
+  domain->dxf = (double *)calloc(nx, sizeof(double));
+  domain->dxc = (double *)calloc(nx + 1, sizeof(double));
+  if (npy::read(dirname_ic, "xf", kNpyDouble, {nx + 1}, domain->xf, sizeof(double))) return 1;
+  if (npy::read(dirname_ic, "xc", kNpyDouble, {nx + 2}, domain->xc, sizeof(double))) return 1;
+  // domain.c:103-143
+  for (size_t i = 1; i <= nx; i++) domain->dxf[i - 1] = domain->xf[i] - domain->xf[i - 1];
+  for (size_t i = 1; i <= nx + 1; i++) domain->dxc[i - 1] = domain->xc[i] - domain->xc[i - 1];
+  domain->dy = domain->lengths[1] / domain->glsizes[1];
+#if NDIMS == 3
+  domain->dz = domain->lengths[2] / domain->glsizes[2];
+#endif
+  if (ctx_create(domain)) return 1;
+  bool uni = false;
+  domain_check_x_grid_is_uniform(domain, &uni);
+  ctx_of(domain)->x_uniform = uni;
+  return 0;
+}
+
+extern "C" int domain_save(const char dirname[], const domain_t * domain) {
+  if (!domain) return fail("domain_save: domain is NULL");
+  const size_t nx = domain->glsizes[0];
+  if (npy::write(dirname, "glsizes", kNpySizeT, {(size_t)NDIMS}, domain->glsizes, sizeof(size_t))) return 1;
+  if (npy::write(dirname, "lengths", kNpyDouble, {(size_t)NDIMS}, domain->lengths, sizeof(double))) return 1;
+  if (npy::write(dirname, "xf", kNpyDouble, {nx + 1}, domain->xf, sizeof(double))) return 1;
+  if (npy::write(dirname, "xc", kNpyDouble, {nx + 2}, domain->xc, sizeof(double))) return 1;
+  return 0;
+}
+
+extern "C" int domain_check_x_grid_is_uniform(const domain_t * domain, bool * x_grid_is_uniform) {
+  // domain.c:75-98
+  const size_t nx = domain->glsizes[0];
+  double mn = +1. * DBL_MAX, mx = -1. * DBL_MAX;
+  for (size_t i = 0; i < nx; i++) {
+    mn = fmin(mn, domain->dxf[i]);
+    mx = fmax(mx, domain->dxf[i]);
+  }
+  *x_grid_is_uniform = fabs(mx - mn) < 1.e-15;
+  return 0;
+}
+
+extern "C" int domain_finalise(domain_t * domain) {
+  vof_ctx * ctx = ctx_of(domain);
+  if (ctx) {
+    hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->d_dxf); hipFree(ctx->d_dxc); hipFree(ctx->d_lapx_ux); hipFree(ctx->d_lapx_c);
+    hipFree(ctx->d_tab); hipHostFree(ctx->h_tab);
+    for (int m = 0; m < 12; m++) hipEventDestroy(ctx->tab_done[m]);
+    hipFree(ctx->d_twy); if (ctx->d_twz) hipFree(ctx->d_twz);
+    hipFree(ctx->d_pl); hipFree(ctx->d_pu); hipFree(ctx->d_ey); hipFree(ctx->d_ez);
+    hipFree(ctx->d_spec); hipFree(ctx->d_vbuf); hipFree(ctx->d_delta); hipFree(ctx->d_part);
+    hipHostFree(ctx->h_scalar);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    domain->info = nullptr;
+  }
+  free(domain->xf); free(domain->xc); free(domain->dxf); free(domain->dxc);
+  domain->xf = domain->xc = domain->dxf = domain->dxc = nullptr;
+  return 0;
+}
+
+extern "C" int vof_set_param(domain_t * domain, const char name[], double value) {
+  CTX_OR_FAIL(domain);
+  for (int m = 0; m < P_COUNT; m++)
+    if (!strcmp(name, kParamNames[m])) {
+      ctx->prm[m] = value;
+      ctx->prm_set[m] = true;
+      return 0;
+    }
+  return fail("vof_set_param: unknown parameter %s", name);
+}
+
+extern "C" int vof_get_param(const domain_t * domain, const char name[], double * value) {
+  CTX_OR_FAIL(domain);
+  for (int m = 0; m < P_COUNT; m++)
+    if (!strcmp(name, kParamNames[m])) return param_get(ctx, m, value);
+  return fail("vof_get_param: unknown parameter %s", name);
+}
+
+extern "C" int vof_set_schedule(domain_t * domain, int schedule) {
+  CTX_OR_FAIL(domain);
+  if (schedule < 0 || schedule > 1) return fail("vof_set_schedule: %d is not 0 or 1", schedule);
+  ctx->schedule = schedule;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fluid
+
+#if NDIMS == 2
+#define FOR_FLUID_ARRAYS(X) X(ux) X(uy) X(p) X(psi) X(t)
+#else
+#define FOR_FLUID_ARRAYS(X) X(ux) X(uy) X(uz) X(p) X(psi) X(t)
+#endif
+
+extern "C" int fluid_init(const char dirname_ic[], const domain_t * domain, fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+#define PREP(n) if (array_prepare(domain, #n, sizeof(double), &fluid->n)) return 1;
+  FOR_FLUID_ARRAYS(PREP)
+#undef PREP
+  for (int m = 0; m < 3; m++) {
+    if (array_prepare(domain, "srcux", sizeof(double), &fluid->srcux[m])) return 1;
+    if (array_prepare(domain, "srcuy", sizeof(double), &fluid->srcuy[m])) return 1;
+#if NDIMS == 3
+    if (array_prepare(domain, "srcuz", sizeof(double), &fluid->srcuz[m])) return 1;
+#endif
+    if (array_prepare(domain, "srct", sizeof(double), &fluid->srct[m])) return 1;
+  }
+  if (array_load(domain, dirname_ic, "ux", &fluid->ux)) return 1;
+  if (array_load(domain, dirname_ic, "uy", &fluid->uy)) return 1;
+#if NDIMS == 3
+  if (array_load(domain, dirname_ic, "uz", &fluid->uz)) return 1;
+#endif
+  if (array_load(domain, dirname_ic, "p", &fluid->p)) return 1;
+  if (array_load(domain, dirname_ic, "t", &fluid->t)) return 1;
+  if (fluid_update_boundaries_ux(domain, &fluid->ux)) return 1;
+  if (fluid_update_boundaries_uy(domain, &fluid->uy)) return 1;
+#if NDIMS == 3
+  if (fluid_update_boundaries_uz(domain, &fluid->uz)) return 1;
+#endif
+  if (fluid_update_boundaries_p(domain, &fluid->p)) return 1;
+  if (fluid_update_boundaries_t(domain, &fluid->t)) return 1;
+  // fluid/init.c:90-93
+  if (param_get(ctx, P_PR, &fluid->Pr)) return 1;
+  if (param_get(ctx, P_RA, &fluid->Ra)) return 1;
+  fluid->m_dif = 1. * sqrt(fluid->Pr) / sqrt(fluid->Ra);
+  fluid->t_dif = 1. / sqrt(fluid->Pr) / sqrt(fluid->Ra);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int fluid_finalise(const domain_t * domain, fluid_t * fluid) {
+  if (ctx_of(domain)) hipStreamSynchronize(ctx_of(domain)->stream);
+#define FREE(n) array_free(&fluid->n);
+  FOR_FLUID_ARRAYS(FREE)
+#undef FREE
+  for (int m = 0; m < 3; m++) {
+    array_free(&fluid->srcux[m]);
+    array_free(&fluid->srcuy[m]);
+#if NDIMS == 3
+    array_free(&fluid->srcuz[m]);
+#endif
+    array_free(&fluid->srct[m]);
+  }
+  return 0;
+}
+
+extern "C" int fluid_save(const char dirname[], const domain_t * domain, const fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+  (void)ctx;
+  if (npy::write(dirname, "m_dif", kNpyDouble, {}, &fluid->m_dif, sizeof(double))) return 1;
+  if (npy::write(dirname, "t_dif", kNpyDouble, {}, &fluid->t_dif, sizeof(double))) return 1;
+  if (array_dump(domain, dirname, "ux", &fluid->ux)) return 1;
+  if (array_dump(domain, dirname, "uy", &fluid->uy)) return 1;
+#if NDIMS == 3
+  if (array_dump(domain, dirname, "uz", &fluid->uz)) return 1;
+#endif
+  if (array_dump(domain, dirname, "p", &fluid->p)) return 1;
+  if (array_dump(domain, dirname, "t", &fluid->t)) return 1;
+  return 0;
+}
+
+extern "C" int fluid_update_boundaries_ux(const domain_t * domain, array_t * ux) {
+  CTX_OR_FAIL(domain);
+  return bc_ux(ctx, (double *)ux->data);
+}
+extern "C" int fluid_update_boundaries_uy(const domain_t * domain, array_t * uy) {
+  CTX_OR_FAIL(domain);
+  return bc_uy(ctx, (double *)uy->data);
+}
+#if NDIMS == 3
+extern "C" int fluid_update_boundaries_uz(const domain_t * domain, array_t * uz) {
+  CTX_OR_FAIL(domain);
+  return bc_uz(ctx, (double *)uz->data);
+}
+#endif
+extern "C" int fluid_update_boundaries_p(const domain_t * domain, array_t * p) {
+  CTX_OR_FAIL(domain);
+  return bc_neumann(ctx, (double *)p->data);
+}
+extern "C" int fluid_update_boundaries_psi(const domain_t * domain, array_t * psi) {
+  CTX_OR_FAIL(domain);
+  return bc_neumann(ctx, (double *)psi->data);
+}
+extern "C" int fluid_update_boundaries_t(const domain_t * domain, array_t * t) {
+  CTX_OR_FAIL(domain);
+  return bc_t(ctx, (double *)t->data);
+}
+
+// fluid_decide_dt (decide_dt.c:104-145)
+static int decide_dt_enqueue(vof_ctx * ctx, const fluid_t * fluid) {
+  const size_t ncell = (size_t)ctx->nx * ctx->ny * ctx->nz;
+  int nb = (int)((ncell + vofk::kRedThreads - 1) / vofk::kRedThreads);
+  if (nb > ctx->npart) nb = ctx->npart;
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+#else
+  const double * uz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_dt_partial, dim3(nb), dim3(vofk::kRedThreads), 0, ctx->stream,
+                     ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data,
+                     uz, ctx->d_part);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(vofk::k_min_final, dim3(1), dim3(vofk::kRedThreads), 0, ctx->stream,
+                     ctx->d_part, nb, ctx->d_part + ctx->npart);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(ctx->h_scalar, ctx->d_part + ctx->npart, sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+static int decide_dt_finish(vof_ctx * ctx, const domain_t * domain, const fluid_t * fluid, double * dt) {
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double coef_adv = 0., coef_dif = 0.;
+  if (param_get(ctx, P_COEF_ADV, &coef_adv)) return 1;
+  if (param_get(ctx, P_COEF_DIF, &coef_dif)) return 1;
+  double res = ctx->h_scalar[0];
+  res *= coef_adv;
+  // decide_dt_dif: only y (and z) are explicit (param/implicit.c:3-7)
+  double hs[3];
+  hs[0] = DBL_MAX;
+  for (int i = 2; i <= ctx->nx; i++) hs[0] = fmin(hs[0], domain->dxc[i - 1]);
+  hs[1] = ctx->dy;
+  hs[2] = ctx->dz;
+  const double difs[2] = {fluid->m_dif, fluid->t_dif};
+  for (int f = 0; f < 2; f++)
+    for (int d = 1; d < NDIMS; d++) {
+      const double v = coef_dif / difs[f] * 0.5 / NDIMS * (hs[d] * hs[d]);
+      res = fmin(res, v);
+    }
+  *dt = res;
+  return 0;
+}
+
+extern "C" int fluid_decide_dt(const domain_t * domain, const fluid_t * fluid, double * dt) {
+  CTX_OR_FAIL(domain);
+  if (decide_dt_enqueue(ctx, fluid)) return 1;
+  return decide_dt_finish(ctx, domain, fluid, dt);
+}
+
+// runge_kutta.c:8-15
+static const double kRk[3][3] = {
+  {+32. / 60., 0. / 60., +32. / 60. + 0. / 60.},
+  {+25. / 60., -17. / 60., +25. / 60. + -17. / 60.},
+  {+45. / 60., -25. / 60., +45. / 60. + -25. / 60.},
+};
+
+// Thomas factors shared by every line of one predictor solve (tdm.c:17-50
+// with the coefficients of solve_in_x, ux.c:251-272)
+static void build_tab(double * tab, int n, double prefactor, const double * lapx) {
+  double * L = tab + 4;
+  double * VAL = tab + 4 + n;
+  double * V = tab + 4 + 2 * n;
+  std::vector<double> l(n), c(n), u(n);
+  for (int i = 0; i < n; i++) {
+    l[i] = -prefactor * lapx[3 * i + 0];
+    c[i] = 1. - prefactor * lapx[3 * i + 1];
+    u[i] = -prefactor * lapx[3 * i + 2];
+  }
+  for (int i = 0; i < n; i++) L[i] = l[i];
+  V[0] = u[0] / c[0];
+  VAL[0] = 0.;
+  tab[0] = c[0];
+  for (int i = 1; i < n - 1; i++) {
+    const double val = 1. / (c[i] - l[i] * V[i - 1]);
+    VAL[i] = val;
+    V[i] = val * (u[i]);
+  }
+  const double val = c[n - 1] - l[n - 1] * V[n - 2];
+  if (fabs(val) > DBL_EPSILON) {
+    tab[1] = 1. / val;
+    tab[2] = 1.;
+  } else {
+    tab[1] = 0.;
+    tab[2] = 0.;
+  }
+  tab[3] = 0.;
+  VAL[n - 1] = 0.;
+  V[n - 1] = 0.;
+}
+
+// enqueue: tables for `field` (0 ux, 1 uy, 2 uz, 3 t) at `rkstep`, copied
+// from a pinned slot once the previous copy from that slot has completed
+static int upload_tab(vof_ctx * ctx, int field, int rkstep, double prefactor, const double ** dtab) {
+  const int slot = field * 3 + rkstep;
+  double * h = ctx->h_tab + ctx->tab_stride * slot;
+  double * d = ctx->d_tab + ctx->tab_stride * slot;
+  HIPCHK(hipEventSynchronize(ctx->tab_done[slot]));
+  const int n = field == 0 ? ctx->nx - 1 : ctx->nx;
+  build_tab(h, n, prefactor, field == 0 ? ctx->lapx_ux.data() : ctx->lapx_c.data());
+  HIPCHK(hipMemcpyAsync(d, h, sizeof(double) * ctx->tab_stride, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->tab_done[slot], ctx->stream));
+  *dtab = d;
+  return 0;
+}
+
+static void swap_data(array_t * a, array_t * b) {
+  void * tmp = a->data;
+  a->data = b->data;
+  b->data = tmp;
+}
+
+// fluid_predict_field (predict/main.c:62-79): the three sources of each
+// field, increments, implicit x solve and update (ux.c:306-388 etc.)
+extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep, const double dt,
+                                   fluid_t * fluid, const interface_t * interface) {
+  CTX_OR_FAIL(domain);
+  if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  // reset_srcs (predict/main.c:9-29): the a <-> b swap; a and g are fully
+  // rewritten by the source kernels
+  if (rkstep != 0) {
+    swap_data(&fluid->srcux[0], &fluid->srcux[1]);
+    swap_data(&fluid->srcuy[0], &fluid->srcuy[1]);
+#if NDIMS == 3
+    swap_data(&fluid->srcuz[0], &fluid->srcuz[1]);
+#endif
+    swap_data(&fluid->srct[0], &fluid->srct[1]);
+  }
+  const double ca = kRk[rkstep][0] * dt, cb = kRk[rkstep][1] * dt, cg = kRk[rkstep][2] * dt;
+  const double pm = 0.5 * kRk[rkstep][2] * dt * fluid->m_dif;
+  const double pt = 0.5 * kRk[rkstep][2] * dt * fluid->t_dif;
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+  const double * ifrcz = (const double *)interface->ifrcz.data;
+#else
+  const double * uz = nullptr;
+#endif
+  vofk::RhsArgs base{};
+  base.ux = (const double *)fluid->ux.data;
+  base.uy = (const double *)fluid->uy.data;
+  base.uz = uz;
+  base.p = (const double *)fluid->p.data;
+  base.t = (const double *)fluid->t.data;
+  base.add_buoyancy = ctx->prm[P_BUOY] != 0.;
+  struct FieldJob {
+    void (*kern)(Grid, vofk::RhsArgs);
+    array_t * src;
+    array_t * field;
+    const double * ifrc;
+    const double * lapx;
+    double dif;
+    int kind;    // 0 ux, 1 uy, 2 uz, 3 t
+  };
+  FieldJob jobs[4];
+  int nj = 0;
+  jobs[nj++] = {vofk::k_rhs_ux, fluid->srcux, &fluid->ux, (const double *)interface->ifrcx.data,
+                ctx->d_lapx_ux, fluid->m_dif, 0};
+  jobs[nj++] = {vofk::k_rhs_uy, fluid->srcuy, &fluid->uy, (const double *)interface->ifrcy.data,
+                ctx->d_lapx_c, fluid->m_dif, 1};
+#if NDIMS == 3
+  jobs[nj++] = {vofk::k_rhs_uz, fluid->srcuz, &fluid->uz, ifrcz, ctx->d_lapx_c, fluid->m_dif, 2};
+#endif
+  jobs[nj++] = {vofk::k_rhs_t, fluid->srct, &fluid->t, nullptr, ctx->d_lapx_c, fluid->t_dif, 3};
+  // compute_rhs (predict/main.c:31-40): every source from the old fields
+  for (int f = 0; f < nj; f++) {
+    vofk::RhsArgs a = base;
+    a.ifrc = jobs[f].ifrc;
+    a.lapx = jobs[f].lapx;
+    a.srca = (double *)jobs[f].src[0].data;
+    a.srcg = (double *)jobs[f].src[2].data;
+    a.dif = jobs[f].dif;
+    const int ilo_n = jobs[f].kind == 0 ? nx - 1 : nx;
+    hipLaunchKernelGGL(jobs[f].kern, grid3(ilo_n, ny, nz, kBlk), kBlk, 0, ctx->stream, ctx->grid, a);
+    LAUNCHCHK();
+  }
+  // predict (predict/main.c:42-53): increments, x solve, update, boundaries
+  for (int f = 0; f < nj; f++) {
+    const int kind = jobs[f].kind;
+    const double pref = kind == 3 ? pt : pm;
+    const double * dtab = nullptr;
+    if (upload_tab(ctx, kind, (int)rkstep, pref, &dtab)) return 1;
+    const int n = kind == 0 ? nx - 1 : nx;
+    const int n0 = kind == 0 ? nx + 1 : nx + 2;
+    double * fld = (double *)jobs[f].field->data;
+    const int nlines = ny * nz;
+    const int nb = (nlines + vofk::kTLines - 1) / vofk::kTLines;
+    hipLaunchKernelGGL(vofk::k_thomas_lines, dim3(nb), dim3(vofk::kTLines), 0, ctx->stream, dtab,
+                       (const double *)jobs[f].src[0].data, (const double *)jobs[f].src[1].data,
+                       (const double *)jobs[f].src[2].data, ca, cb, cg, ctx->d_delta, fld, n,
+                       nlines, 1, n0, NDIMS == 3 ? ny + 2 : 0, 1, ny);
+    LAUNCHCHK();
+    int rc = 0;
+    if (kind == 0) rc = bc_ux(ctx, fld);
+    else if (kind == 1) rc = bc_uy(ctx, fld);
+    else if (kind == 2) rc = bc_uz(ctx, fld);
+    else rc = bc_t(ctx, fld);
+    if (rc) return 1;
+  }
+  return 0;
+}
+
+// Poisson: both reference solvers are the same discrete problem; the FFT
+// normalisation 1/(ny nz) is folded into the right-hand side as dft.c:329-330
+static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+  if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  const double norm = (double)ny * (NDIMS == 3 ? (double)nz : 1.);
+  const double prefactor = 1. / (kRk[rkstep][2] * dt) / norm;
+  const int NC = vofk::kFftElems / ny;
+  const int nbx = (nx + 2 * NC - 1) / (2 * NC);
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+#else
+  const double * uz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_div_rfft_y, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
+                     ctx->stream, ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
+                     (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
+  LAUNCHCHK();
+#if NDIMS == 3
+  const int NCz = vofk::kFftElems / nz;
+  const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
+  hipLaunchKernelGGL(vofk::k_fft_z, gz, dim3(vofk::kFftThreads), 0, ctx->stream, ctx->grid,
+                     ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, -1.);
+  LAUNCHCHK();
+#endif
+  const int nlines = ctx->nh * nz;
+  hipLaunchKernelGGL(vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), 0, ctx->stream, nx,
+                     nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec,
+                     ctx->d_vbuf);
+  LAUNCHCHK();
+#if NDIMS == 3
+  hipLaunchKernelGGL(vofk::k_fft_z, gz, dim3(vofk::kFftThreads), 0, ctx->stream, ctx->grid,
+                     ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, +1.);
+  LAUNCHCHK();
+#endif
+  hipLaunchKernelGGL(vofk::k_irfft_y_psi, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+                     0, ctx->stream, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec,
+                     (double *)fluid->psi.data);
+  LAUNCHCHK();
+  return bc_neumann(ctx, (double *)fluid->psi.data);
+}
+
+extern "C" int fluid_compute_potential_dft(const domain_t * domain, const size_t rkstep,
+                                           const double dt, fluid_t * fluid) {
+  return potential(domain, rkstep, dt, fluid);
+}
+
+extern "C" int fluid_compute_potential_dct(const domain_t * domain, const size_t rkstep,
+                                           const double dt, fluid_t * fluid) {
+  return potential(domain, rkstep, dt, fluid);
+}
+
+// correct_velocity/{main,ux,uy}.c
+extern "C" int fluid_correct_velocity(const domain_t * domain, const size_t rkstep,
+                                      const double dt, fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+  if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  const double prefactor = kRk[rkstep][2] * dt;
+#if NDIMS == 3
+  double * uz = (double *)fluid->uz.data;
+#else
+  double * uz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_correct, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+                     ctx->grid, prefactor, (const double *)fluid->psi.data,
+                     (double *)fluid->ux.data, (double *)fluid->uy.data, uz);
+  LAUNCHCHK();
+  if (bc_ux(ctx, (double *)fluid->ux.data)) return 1;
+  if (bc_uy(ctx, (double *)fluid->uy.data)) return 1;
+#if NDIMS == 3
+  if (bc_uz(ctx, uz)) return 1;
+#endif
+  return 0;
+}
+
+// update_pressure.c:87-109
+extern "C" int fluid_update_pressure(const domain_t * domain, const size_t rkstep, const double dt,
+                                     fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+  if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  const double prefactor = 0.5 * kRk[rkstep][2] * dt * fluid->m_dif;
+  hipLaunchKernelGGL(vofk::k_pressure, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+                     ctx->grid, prefactor, (const double *)fluid->psi.data, (double *)fluid->p.data);
+  LAUNCHCHK();
+  return bc_neumann(ctx, (double *)fluid->p.data);
+}
+
+// ---------------------------------------------------------------------------
+// interface
+
+extern "C" int interface_init(const char dirname_ic[], const domain_t * domain, interface_t * interface) {
+  CTX_OR_FAIL(domain);
+  if (array_prepare(domain, "vof", sizeof(double), &interface->vof)) return 1;
+  if (array_prepare(domain, "ifrcx", sizeof(double), &interface->ifrcx)) return 1;
+  if (array_prepare(domain, "ifrcy", sizeof(double), &interface->ifrcy)) return 1;
+#if NDIMS == 3
+  if (array_prepare(domain, "ifrcz", sizeof(double), &interface->ifrcz)) return 1;
+#endif
+  if (array_prepare(domain, "dvof", sizeof(vector_t), &interface->dvof)) return 1;
+  if (array_prepare(domain, "normal", sizeof(normal_t), &interface->normal)) return 1;
+  if (array_prepare(domain, "curv", sizeof(double), &interface->curv)) return 1;
+  if (array_prepare(domain, "flxx", sizeof(double), &interface->flxx)) return 1;
+  if (array_prepare(domain, "flxy", sizeof(double), &interface->flxy)) return 1;
+#if NDIMS == 3
+  if (array_prepare(domain, "flxz", sizeof(double), &interface->flxz)) return 1;
+#endif
+  for (int m = 0; m < 2; m++)
+    if (array_prepare(domain, "src", sizeof(double), &interface->src[m])) return 1;
+  if (array_load(domain, dirname_ic, "vof", &interface->vof)) return 1;
+  if (interface_update_boundaries_vof(domain, &interface->vof)) return 1;
+  double We = 0.;
+  if (param_get(ctx, P_WE, &We)) return 1;
+  interface->tension = 1. / We;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int interface_finalise(const domain_t * domain, interface_t * interface) {
+  if (ctx_of(domain)) hipStreamSynchronize(ctx_of(domain)->stream);
+  array_free(&interface->vof);
+  array_free(&interface->ifrcx);
+  array_free(&interface->ifrcy);
+#if NDIMS == 3
+  array_free(&interface->ifrcz);
+  array_free(&interface->flxz);
+#endif
+  array_free(&interface->dvof);
+  array_free(&interface->normal);
+  array_free(&interface->curv);
+  array_free(&interface->flxx);
+  array_free(&interface->flxy);
+  array_free(&interface->src[0]);
+  array_free(&interface->src[1]);
+  return 0;
+}
+
+extern "C" int interface_save(const char dirname[], const domain_t * domain, const interface_t * interface) {
+  CTX_OR_FAIL(domain);
+  (void)ctx;
+  if (npy::write(dirname, "surface_tension", kNpyDouble, {}, &interface->tension, sizeof(double))) return 1;
+  return array_dump(domain, dirname, "vof", &interface->vof);
+}
+
+extern "C" int interface_update_boundaries_vof(const domain_t * domain, array_t * vof) {
+  CTX_OR_FAIL(domain);
+  return bc_vof(ctx, (double *)vof->data);
+}
+
+extern "C" int interface_compute_curvature_tensor(const domain_t * domain, interface_t * interface) {
+  CTX_OR_FAIL(domain);
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  hipLaunchKernelGGL(vofk::k_gradient, grid3(nx + 1, ny + 3, nz + 3, kBlk), kBlk, 0, ctx->stream,
+                     ctx->grid, (const double *)interface->vof.data, (double *)interface->dvof.data);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(vofk::k_normal_curv, grid3(nx, ny + 2, nz + 2, kBlk), kBlk, 0, ctx->stream,
+                     ctx->grid, (const double *)interface->vof.data,
+                     (const double *)interface->dvof.data, (double *)interface->normal.data,
+                     (double *)interface->curv.data);
+  LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int interface_compute_force(const domain_t * domain, interface_t * interface) {
+  CTX_OR_FAIL(domain);
+#if NDIMS == 3
+  double * ifrcz = (double *)interface->ifrcz.data;
+#else
+  double * ifrcz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_force, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+                     ctx->grid, interface->tension, (const double *)interface->vof.data,
+                     (const double *)interface->curv.data, (double *)interface->ifrcx.data,
+                     (double *)interface->ifrcy.data, ifrcz);
+  LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int interface_update_vof(const domain_t * domain, const size_t rkstep, const double dt,
+                                    const fluid_t * fluid, interface_t * interface) {
+  CTX_OR_FAIL(domain);
+  if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  // reset_srcs (update/main.c:28-45): swap; src_a is fully rewritten below
+  if (rkstep != 0) swap_data(&interface->src[0], &interface->src[1]);
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+  double * flxz = (double *)interface->flxz.data;
+#else
+  const double * uz = nullptr;
+  double * flxz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_flux, grid3(nx, ny + 1, nz + 1, kBlk), kBlk, 0, ctx->stream, ctx->grid,
+                     (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz,
+                     (const double *)interface->vof.data, (const double *)interface->normal.data,
+                     (double *)interface->flxx.data, (double *)interface->flxy.data, flxz);
+  LAUNCHCHK();
+  const double dta = dt * kRk[rkstep][0];
+  const double dtb = dt * kRk[rkstep][1];
+  hipLaunchKernelGGL(vofk::k_vof_update, grid3(nx, ny, nz, kBlk), kBlk, 0, ctx->stream, ctx->grid,
+                     (const double *)interface->flxx.data, (const double *)interface->flxy.data,
+                     (const double *)flxz, (double *)interface->src[0].data,
+                     (const double *)interface->src[1].data, (double *)interface->vof.data, dta,
+                     dtb, (int)(rkstep != 0));
+  LAUNCHCHK();
+  return bc_vof(ctx, (double *)interface->vof.data);
+}
+
+// ---------------------------------------------------------------------------
+// integrate (src/integrate.c:11-68)
+
+extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
+  CTX_OR_FAIL(domain);
+  bool uniform = false;
+  domain_check_x_grid_is_uniform(domain, &uniform);
+  if (fluid_decide_dt(domain, fluid, dt)) return 1;
+  for (size_t rk = 0; rk < 3; rk++) {
+    if (interface_compute_curvature_tensor(domain, interface)) return 1;
+    if (interface_compute_force(domain, interface)) return 1;
+    if (interface_update_vof(domain, rk, *dt, fluid, interface)) return 1;
+    if (fluid_predict_field(domain, rk, *dt, fluid, interface)) return 1;
+    if (uniform) {
+      if (fluid_compute_potential_dct(domain, rk, *dt, fluid)) return 1;
+    } else {
+      if (fluid_compute_potential_dft(domain, rk, *dt, fluid)) return 1;
+    }
+    if (fluid_correct_velocity(domain, rk, *dt, fluid)) return 1;
+    if (fluid_update_pressure(domain, rk, *dt, fluid)) return 1;
+  }
+  (void)ctx;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+
+extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
+                               const interface_t * interface, double * out) {
+  CTX_OR_FAIL(domain);
+  const int nb = 256;
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+#else
+  const double * uz = nullptr;
+#endif
+  hipLaunchKernelGGL(vofk::k_diag_partial, dim3(nb), dim3(vofk::kRedThreads), 0, ctx->stream,
+                     ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz,
+                     (const double *)fluid->t.data, (const double *)interface->vof.data, ctx->d_part);
+  LAUNCHCHK();
+  std::vector<double> h((size_t)nb * vofk::kDiagN);
+  HIPCHK(hipMemcpyAsync(h.data(), ctx->d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double q[vofk::kDiagN] = {0., 1., 0., 0., 0., 0., 0., 0., 0.};
+  for (int b = 0; b < nb; b++) {
+    const double * r = &h[(size_t)b * vofk::kDiagN];
+    q[0] = fmax(q[0], r[0]);
+    q[1] = fmin(q[1], r[1]);
+    q[2] = fmax(q[2], r[2]);
+    for (int m = 3; m < vofk::kDiagN; m++) q[m] += r[m];
+  }
+  int o = 0;
+  out[o++] = q[0];
+  out[o++] = q[1];
+  out[o++] = q[2];
+  out[o++] = q[3] / q[4];
+  out[o++] = q[5];
+  out[o++] = q[6];
+#if NDIMS == 3
+  out[o++] = q[7];
+#endif
+  out[o++] = q[8];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tri-diagonal solver vtable (tdm.c:110-324) over device right-hand sides
+
+struct tdm_info_t_ {
+  int size, nrhs;
+  bool is_periodic, is_complex;
+  double * l, * c, * u;        // host
+  double * d_luc;              // device copy of l, c, u
+  double * d_v;                // per-rhs v scratch (size * nrhs)
+  double * d_q1;               // periodic correction vector (size)
+};
+
+namespace vofk {
+// gtsv on one line of length n (stride st elements) for nrhs interleaved
+// components (1 real, 2 complex); v per line in vbuf
+__device__ inline void gtsv_line(int n, const double * l, const double * c, const double * u,
+                                 double * v, double * q, int ncomp) {
+  v[0] = u[0] / c[0];
+  for (int r = 0; r < ncomp; r++) q[r] = q[r] / c[0];
+  for (int i = 1; i < n - 1; i++) {
+    const double val = 1. / (c[i] - l[i] * v[i - 1]);
+    v[i] = val * (u[i]);
+    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] = val * (q[i * ncomp + r] - l[i] * q[(i - 1) * ncomp + r]);
+  }
+  const double val = c[n - 1] - l[n - 1] * v[n - 2];
+  if (fabs(val) > DBL_EPSILON) {
+    const double rr = 1. / val;
+    for (int r = 0; r < ncomp; r++)
+      q[(n - 1) * ncomp + r] = rr * (q[(n - 1) * ncomp + r] - l[n - 1] * q[(n - 2) * ncomp + r]);
+  } else {
+    for (int r = 0; r < ncomp; r++) q[(n - 1) * ncomp + r] = 0.;
+  }
+  for (int i = n - 2; i >= 0; i--)
+    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] -= v[i] * q[(i + 1) * ncomp + r];
+}
+
+// tdm_solve (tdm.c:66-106): one lane per right-hand side; the periodic
+// correction vector q1 is solved by the first lane of every block into its
+// own copy (all copies are identical)
+__global__ void k_tdm_solve(int n, int nrhs, int periodic, int ncomp, const double * __restrict__ luc,
+                            double * __restrict__ vbuf, double * __restrict__ q1buf,
+                            double * __restrict__ data) {
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= nrhs) return;
+  const double * l = luc;
+  const double * c = luc + n;
+  const double * u = luc + 2 * n;
+  double * v = vbuf + (size_t)j * n;
+  double * q = data + (size_t)j * n * ncomp;
+  if (!periodic) {
+    gtsv_line(n, l, c, u, v, q, ncomp);
+    return;
+  }
+  double * q1 = q1buf + (size_t)j * n;
+  for (int i = 0; i < n - 1; i++) q1[i] = i == 0 ? -l[i] : i == n - 2 ? -u[i] : 0.;
+  gtsv_line(n - 1, l, c, u, v, q1, 1);
+  gtsv_line(n - 1, l, c, u, v, q, ncomp);
+  const double den = c[n - 1] + u[n - 1] * q1[0] + l[n - 1] * q1[n - 2];
+  for (int r = 0; r < ncomp; r++) {
+    const double num = q[(n - 1) * ncomp + r] - u[n - 1] * q[r] - l[n - 1] * q[(n - 2) * ncomp + r];
+    q[(n - 1) * ncomp + r] = fabs(den) < DBL_EPSILON ? 0. : num / den;
+  }
+  for (int i = 0; i < n - 1; i++)
+    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] = q[i * ncomp + r] + q[(n - 1) * ncomp + r] * q1[i];
+}
+}  // namespace vofk
+
+// host-only: the vtable object must not be emitted by the device pass
+#if !defined(__HIP_DEVICE_COMPILE__)
+static int tdm_construct(const int size, const int nrhs, const bool is_periodic, const bool is_complex,
+                         tdm_info_t ** info) {
+  if (size <= 0) { *info = nullptr; return fail("ERROR(construct): size should be positive: %d", size); }
+  if (nrhs <= 0) { *info = nullptr; return fail("ERROR(construct): nrhs should be positive: %d", nrhs); }
+  if (size < 3) { *info = nullptr; return fail("ERROR(construct): size should be at least 3: %d", size); }
+  tdm_info_t * t = new tdm_info_t_();
+  t->size = size;
+  t->nrhs = nrhs;
+  t->is_periodic = is_periodic;
+  t->is_complex = is_complex;
+  t->l = (double *)calloc(size, sizeof(double));
+  t->c = (double *)calloc(size, sizeof(double));
+  t->u = (double *)calloc(size, sizeof(double));
+  HIPCHK(hipMalloc(&t->d_luc, sizeof(double) * 3 * size));
+  HIPCHK(hipMalloc(&t->d_v, sizeof(double) * (size_t)size * nrhs));
+  HIPCHK(hipMalloc(&t->d_q1, sizeof(double) * (size_t)size * nrhs));
+  *info = t;
+  return 0;
+}
+static int tdm_get_l(const tdm_info_t * info, double ** l) {
+  if (!info) return fail("ERROR(get_l): info is NULL");
+  *l = info->l;
+  return 0;
+}
+static int tdm_get_c(const tdm_info_t * info, double ** c) {
+  if (!info) return fail("ERROR(get_c): info is NULL");
+  *c = info->c;
+  return 0;
+}
+static int tdm_get_u(const tdm_info_t * info, double ** u) {
+  if (!info) return fail("ERROR(get_u): info is NULL");
+  *u = info->u;
+  return 0;
+}
+static int tdm_get_size(const tdm_info_t * info, int * size) {
+  if (!info) return fail("ERROR(get_size): info is NULL");
+  *size = info->size;
+  return 0;
+}
+static int tdm_get_nrhs(const tdm_info_t * info, int * nrhs) {
+  if (!info) return fail("ERROR(get_nrhs): info is NULL");
+  *nrhs = info->nrhs;
+  return 0;
+}
+static int tdm_solve(const tdm_info_t * info, void * data) {
+  if (!info) return fail("ERROR(solve): info is NULL");
+  const int n = info->size;
+  std::vector<double> luc(3 * (size_t)n);
+  memcpy(&luc[0], info->l, sizeof(double) * n);
+  memcpy(&luc[n], info->c, sizeof(double) * n);
+  memcpy(&luc[2 * n], info->u, sizeof(double) * n);
+  HIPCHK(hipMemcpy(info->d_luc, luc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  const int nb = (info->nrhs + 63) / 64;
+  hipLaunchKernelGGL(vofk::k_tdm_solve, dim3(nb), dim3(64), 0, 0, n, info->nrhs, (int)info->is_periodic,
+                     info->is_complex ? 2 : 1, info->d_luc, info->d_v, info->d_q1, (double *)data);
+  LAUNCHCHK();
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+static int tdm_destruct(tdm_info_t * info) {
+  if (!info) return fail("ERROR(destruct): info is NULL");
+  free(info->l); free(info->c); free(info->u);
+  hipFree(info->d_luc); hipFree(info->d_v); hipFree(info->d_q1);
+  delete info;
+  return 0;
+}
+
+extern "C" const tdm_t tdm = {
+  tdm_construct, tdm_get_l, tdm_get_c, tdm_get_u, tdm_get_size, tdm_get_nrhs, tdm_solve, tdm_destruct,
+};
+
+#endif  // !__HIP_DEVICE_COMPILE__
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+// linear_system.c:28-85 (implicit x only)
+extern "C" int linear_system_init(const void * info, const bool implicit[NDIMS],
+                                  const size_t glsizes[NDIMS], linear_system_t * ls) {
+  (void)info;
+  if (ls->is_initialised) return fail("this linear_system object is already initialised");
+  for (int d = 1; d < NDIMS; d++)
+    if (implicit[d]) return fail("linear_system_init: implicit treatment in direction %d is not provided", d);
+  memcpy(ls->implicit, implicit, sizeof(bool) * NDIMS);
+  size_t n = 1;
+  for (int d = 0; d < NDIMS; d++) {
+    ls->x1pncl_mysizes[d] = glsizes[d];
+    ls->y1pncl_mysizes[d] = glsizes[d];
+    n *= glsizes[d];
+  }
+  ls->x1pncl = nullptr;
+  HIPCHK(hipMalloc(&ls->x1pncl, sizeof(double) * n));
+  HIPCHK(hipMemset(ls->x1pncl, 0, sizeof(double) * n));
+  ls->y1pncl = nullptr;
+  ls->tdm_y = nullptr;
+  ls->transposer_x1_to_y1 = ls->transposer_y1_to_x1 = nullptr;
+  ls->tdm_x = nullptr;
+  if (implicit[0]) {
+    if (tdm_construct((int)glsizes[0], (int)(n / glsizes[0]), false, false, &ls->tdm_x)) return 1;
+  }
+  ls->is_initialised = true;
+  return 0;
+}
+
+extern "C" int linear_system_finalise(linear_system_t * ls) {
+  if (!ls || !ls->is_initialised) return 1;
+  hipFree(ls->x1pncl);
+  if (ls->implicit[0]) tdm_destruct(ls->tdm_x);
+  ls->is_initialised = false;
+  return 0;
+}
+#endif  // !__HIP_DEVICE_COMPILE__
