@@ -1,0 +1,207 @@
+"""Benchmark: plain FP64 time step of the 3D 256^3 droplet workload on one
+MI355X, reported in Mcell-steps/s.
+
+Contract: ``python bench.py --gpus N --steps K --warmup W`` does W untimed
+steps, then times exactly K steps of integrate() (the reference's one-step
+entry point) with the device synchronised on both sides, and prints one JSON
+line.  A "step" is one full RK3 time step (dt, 3 x [curvature, force, VOF
+update, predictor, Poisson, correction, pressure]) over the whole grid with
+every field resident in HBM.
+
+Per-kernel times come from HIP events recorded on the library's own stream
+around every launch inside the timed region (vof_profile); the dominant
+kernel's algorithmic bytes per launch divided by its mean launch time gives
+``roofline.achieved``.  ``cpu_baseline`` times the CPU oracle (a single-core
+C restatement of the reference step) on one step of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s at 3D 256³ FP64 plain vs batched/mixed-prec; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
+    """Compulsory FP64 traffic of one launch: every array element the kernel
+    must read once plus every element it must write once (3D layouts)."""
+    nh = ny // 2 + 1
+    c = nx * ny * nz
+    s = nx * nh * nz
+    table = {
+        "k_gradient": (nx + 1) * (ny + 3) * (nz + 3) * (8 + 24),
+        "k_normal_curv": nx * (ny + 2) * (nz + 2) * (24 + 8 + 8),
+        "k_force": c * (8 + 8 + 24),
+        "k_flux": c * (24 + 8 + 24),
+        "k_vof_update": c * (24 + 8 + 8 + 8 + 8),
+        "k_rhs_ux": (nx - 1) * ny * nz * (48 + 16),
+        "k_rhs_uy": c * (40 + 16),
+        "k_rhs_uz": c * (40 + 16),
+        "k_rhs_t": c * (32 + 16),
+        "k_thomas_lines": (3 * nx + (nx - 1)) / 4.0 * ny * nz * (24 + 8 + 8),
+        "k_div_rfft_y": c * 24 + s * 16,
+        "k_fft_z": s * 32,
+        "k_thomas_spec": s * 32,
+        "k_irfft_y_psi": s * 16 + c * 8,
+        "k_correct": c * (8 + 24 + 24),
+        "k_pressure": c * (8 + 8 + 8),
+        "k_dt_partial": c * 24,
+    }
+    return table.get(kernel)
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (profiles/traffic.txt: `workload kernel fetch_bytes write_bytes`), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == workload and parts[1] == kernel:
+                return float(parts[2]) + float(parts[3])
+    return None
+
+
+def cpu_baseline(ic, steps: int):
+    """Time the single-core CPU oracle on `steps` steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    o = Oracle(ic)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.integrate()
+    el = time.perf_counter() - t0
+    cells = 1
+    for n in ic.glsizes:
+        cells *= n
+    return {"value": cells * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} step(s) of the same {'x'.join(map(str, ic.glsizes))} workload "
+                      f"on one host core ({el:.1f} s)"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="cells per direction (3D cube)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", type=int, default=None)
+    ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design note, not code)", file=sys.stderr)
+        return 2
+
+    import simplevofsolver_amd as sva
+    from simplevofsolver_amd.ic import droplets_3d
+
+    n = args.n
+    t_ic = time.perf_counter()
+    ic, spheres = droplets_3d(n, n, n, seed=2)
+    t_ic = time.perf_counter() - t_ic
+    print(f"[bench] IC {n}^3 generated in {t_ic:.1f} s", file=sys.stderr, flush=True)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        g = sva.solver_from_ic(ic, d, schedule=args.schedule)
+    # the library holds everything on the device from here on
+    d0 = g.diagnostics()
+    for _ in range(args.warmup):
+        g.integrate()
+    g.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    g.profile(True)
+    g.profile_reset()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.integrate()
+    g.synchronize()
+    t1 = time.perf_counter()
+    rep = g.profile_report()
+    g.profile(False)
+    d1 = g.diagnostics()
+    el = t1 - t0
+    cells = n * n * n
+    value = cells * args.steps / el / 1e6
+    kern_total = sum(ms for _, ms in rep.values())
+    dom = max(rep.items(), key=lambda kv: kv[1][1])
+    kname, (kcnt, kms) = dom
+    kavg_s = kms / kcnt * 1e-3
+    kbytes = algorithmic_bytes(kname, n, n, n)
+    achieved = kbytes / kavg_s / 1e9 if kbytes else None
+    traffic = read_traffic(kname, f"3d{n}")
+    copy_gbs = g.copy_probe(1 << 27, 10)
+    if args.kernels:
+        print(f"[bench] kernels (timed region {el * 1e3:.2f} ms, kernel sum {kern_total:.2f} ms):",
+              file=sys.stderr)
+        for k, (cnt, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
+            b = algorithmic_bytes(k, n, n, n)
+            bw = f"{b / (ms / cnt * 1e-3) / 1e9:8.1f} GB/s" if b else ""
+            print(f"  {k:18s} {cnt:5d} x {ms / cnt * 1e3:9.1f} us  total {ms:9.2f} ms {bw}",
+                  file=sys.stderr)
+        print(f"[bench] copy probe {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    g.close()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain 3D {n}x{n}x{n} FP64 droplets (8 spheres r in [0.06,0.12], "
+                        f"16-mode divergence-free velocity rms 0.1, Re=500, We=20), x walls, y/z periodic",
+            "cells": cells,
+            "form": "plain",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": kbytes,
+            "mean_launch_us": round(kavg_s * 1e6, 2),
+            "copy_probe_gbs": round(copy_gbs, 1),
+        },
+        "step_bytes_per_cell": 960,
+        "step_hbm_frac": round(960 * value * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
+        "drift": {
+            "vof_volume_rel": abs(d1["vof_volume"] - d0["vof_volume"]) / d0["vof_volume"],
+            "max_div": d1["max_div"],
+            "kinetic_energy_rel": ((d1["ke_x"] + d1["ke_y"] + d1["ke_z"]) /
+                                   (d0["ke_x"] + d0["ke_y"] + d0["ke_z"]) - 1.0),
+            "steps": args.warmup + args.steps,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -251,6 +251,16 @@ int vof_set_schedule(domain_t * domain, int schedule);
 int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
                     const interface_t * interface, double * out);
 
+/* per-kernel timing with HIP events recorded on the context's stream around
+ * every launch (on = 1 starts, on = 0 stops); the report lists
+ * "kernel launches total_ms" lines */
+int vof_profile(domain_t * domain, int on);
+int vof_profile_reset(domain_t * domain);
+int vof_profile_report(domain_t * domain, char * buf, size_t len);
+/* copy of n doubles repeated reps times on the device: mean read+write
+ * bandwidth in GB/s (calibrates hardware byte counters, kernel k_copy) */
+int vof_copy_probe(domain_t * domain, size_t n, int reps, double * gbps);
+
 #ifdef __cplusplus
 }
 #endif

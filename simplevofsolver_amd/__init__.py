@@ -73,7 +73,8 @@ ABI_SYMBOLS_COMMON = [
     "integrate", "tdm", "linear_system_init", "linear_system_finalise",
     "vof_last_error", "vof_set_param", "vof_get_param", "vof_array_to_host",
     "vof_array_from_host", "vof_synchronize", "interface_finalise", "fluid_finalise",
-    "domain_finalise", "vof_set_schedule", "vof_diagnostics",
+    "domain_finalise", "vof_set_schedule", "vof_diagnostics", "vof_profile",
+    "vof_profile_reset", "vof_profile_report", "vof_copy_probe",
 ]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
 
@@ -122,6 +123,10 @@ def load(ndims: int):
     lib.domain_finalise.argtypes = [vp]
     lib.vof_set_schedule.argtypes = [vp, C.c_int]
     lib.vof_diagnostics.argtypes = [vp, vp, vp, dp]
+    lib.vof_profile.argtypes = [vp, C.c_int]
+    lib.vof_profile_reset.argtypes = [vp]
+    lib.vof_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
+    lib.vof_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, dp]
     lib._structs = (Domain, Fluid, Interface)
     _LIBS[ndims] = lib
     return lib
@@ -259,6 +264,27 @@ class Solver:
         self._chk(self.lib.fluid_save(dirname.encode(), C.byref(self.domain), C.byref(self.fluid)), "fluid_save")
         self._chk(self.lib.interface_save(dirname.encode(), C.byref(self.domain), C.byref(self.interface)),
                   "interface_save")
+
+    def profile(self, on: bool = True):
+        self._chk(self.lib.vof_profile(C.byref(self.domain), int(on)), "vof_profile")
+
+    def profile_reset(self):
+        self._chk(self.lib.vof_profile_reset(C.byref(self.domain)), "vof_profile_reset")
+
+    def profile_report(self) -> dict:
+        """{kernel: (launches, total_ms)} from the HIP-event timers."""
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(self.lib.vof_profile_report(C.byref(self.domain), buf, len(buf)), "vof_profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    def copy_probe(self, n: int, reps: int = 10) -> float:
+        g = C.c_double(0.0)
+        self._chk(self.lib.vof_copy_probe(C.byref(self.domain), n, reps, C.byref(g)), "vof_copy_probe")
+        return g.value
 
     @property
     def x_uniform(self) -> bool:

@@ -1267,4 +1267,10 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     for (int m = 0; m < kDiagN; m++) part[(size_t)blockIdx.x * kDiagN + m] = red[m][0];
 }
 
+// plain copy, the calibration kernel for hardware byte counters
+__global__ void k_copy(const double * __restrict__ a, double * __restrict__ b, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
+}
+
 }  // namespace vofk

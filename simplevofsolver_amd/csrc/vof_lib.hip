@@ -228,6 +228,15 @@ struct vof_ctx {
   double * d_part;
   int npart;
   double * h_scalar;                     // pinned
+  // per-kernel timing with HIP events on this context's stream
+  bool prof_on;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_next;
+  struct Pending { const char * name; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<std::string> prof_names;
+  std::vector<double> prof_ms;
+  std::vector<long> prof_cnt;
 };
 
 static vof_ctx * ctx_of(const domain_t * domain) {
@@ -263,6 +272,51 @@ static dim3 grid3(int ni, int nj, int nk, dim3 blk) {
 
 static const dim3 kBlk(64, 4, 1);
 
+static int prof_flush(vof_ctx * ctx) {
+  if (ctx->pending.empty()) return 0;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (const auto & p : ctx->pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+    size_t m = 0;
+    while (m < ctx->prof_names.size() && ctx->prof_names[m] != p.name) m++;
+    if (m == ctx->prof_names.size()) {
+      ctx->prof_names.push_back(p.name);
+      ctx->prof_ms.push_back(0.);
+      ctx->prof_cnt.push_back(0);
+    }
+    ctx->prof_ms[m] += ms;
+    ctx->prof_cnt[m] += 1;
+  }
+  ctx->pending.clear();
+  ctx->ev_next = 0;
+  return 0;
+}
+
+static hipEvent_t prof_begin(vof_ctx * ctx) {
+  if (!ctx->prof_on) return nullptr;
+  if (ctx->ev_next + 2 > ctx->ev_pool.size() && prof_flush(ctx)) return nullptr;
+  hipEvent_t e = ctx->ev_pool[ctx->ev_next++];
+  if (hipEventRecord(e, ctx->stream) != hipSuccess) return nullptr;
+  return e;
+}
+
+static int prof_end(vof_ctx * ctx, const char * name, hipEvent_t e0) {
+  hipEvent_t e1 = ctx->ev_pool[ctx->ev_next++];
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  ctx->pending.push_back({name, e0, e1});
+  return 0;
+}
+
+// launch on the context stream, bracketed by events when profiling is on
+#define KLN(ctx, name, kern, g, b, ...)                                            \
+  do {                                                                             \
+    hipEvent_t e0_ = prof_begin(ctx);                                              \
+    hipLaunchKernelGGL(kern, g, b, 0, (ctx)->stream, __VA_ARGS__);                 \
+    LAUNCHCHK();                                                                   \
+    if (e0_ && prof_end(ctx, name, e0_)) return 1;                                 \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // boundary conditions: halo copy, then x walls (src/halo.c:18-76 followed by
 // assign_boundary_conditions_in_x of each fluid/boundary/*.c)
@@ -280,7 +334,7 @@ static int bc_apply(vof_ctx * ctx, double * a, int n0, int h, int walls_all_rows
 #endif
   {
     const unsigned nb = (unsigned)((cnt + 255) / 256);
-    hipLaunchKernelGGL(vofk::k_halo, dim3(nb), dim3(256), 0, ctx->stream, a, n0, ny, nz, h);
+    KLN(ctx, "k_halo", vofk::k_halo, dim3(nb), dim3(256), a, n0, ny, nz, h);
     LAUNCHCHK();
   }
   int r0, nr, p0, np;
@@ -300,7 +354,7 @@ static int bc_apply(vof_ctx * ctx, double * a, int n0, int h, int walls_all_rows
 #endif
   }
   const int tot = nr * np;
-  hipLaunchKernelGGL(vofk::k_walls, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, a, n0,
+  KLN(ctx, "k_walls", vofk::k_walls, dim3((tot + 255) / 256), dim3(256), a, n0,
                      NDIMS == 3 ? n1 : 0, r0, nr, p0, np, xlo, xhi, mode, vlo, vhi);
   LAUNCHCHK();
   return 0;
@@ -559,6 +613,8 @@ static int ctx_create(domain_t * domain) {
 #endif
   ctx->dy = domain->dy;
   ctx->schedule = 1;
+  ctx->prof_on = false;
+  ctx->ev_next = 0;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
   if (nx < 4 || ny < 4 || (NDIMS == 3 && nz < 4))
     return fail("grid %dx%dx%d too small (at least 4 cells per direction)", nx, ny, nz);
@@ -741,6 +797,8 @@ extern "C" int domain_finalise(domain_t * domain) {
     hipFree(ctx->d_pl); hipFree(ctx->d_pu); hipFree(ctx->d_ey); hipFree(ctx->d_ez);
     hipFree(ctx->d_spec); hipFree(ctx->d_vbuf); hipFree(ctx->d_delta); hipFree(ctx->d_part);
     hipHostFree(ctx->h_scalar);
+    prof_flush(ctx);
+    for (auto & e : ctx->ev_pool) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     domain->info = nullptr;
@@ -888,11 +946,11 @@ static int decide_dt_enqueue(vof_ctx * ctx, const fluid_t * fluid) {
 #else
   const double * uz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_dt_partial, dim3(nb), dim3(vofk::kRedThreads), 0, ctx->stream,
+  KLN(ctx, "k_dt_partial", vofk::k_dt_partial, dim3(nb), dim3(vofk::kRedThreads),
                      ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data,
                      uz, ctx->d_part);
   LAUNCHCHK();
-  hipLaunchKernelGGL(vofk::k_min_final, dim3(1), dim3(vofk::kRedThreads), 0, ctx->stream,
+  KLN(ctx, "k_min_final", vofk::k_min_final, dim3(1), dim3(vofk::kRedThreads),
                      ctx->d_part, nb, ctx->d_part + ctx->npart);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(ctx->h_scalar, ctx->d_part + ctx->npart, sizeof(double),
@@ -1025,6 +1083,7 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
   base.t = (const double *)fluid->t.data;
   base.add_buoyancy = ctx->prm[P_BUOY] != 0.;
   struct FieldJob {
+    const char * name;
     void (*kern)(Grid, vofk::RhsArgs);
     array_t * src;
     array_t * field;
@@ -1035,14 +1094,14 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
   };
   FieldJob jobs[4];
   int nj = 0;
-  jobs[nj++] = {vofk::k_rhs_ux, fluid->srcux, &fluid->ux, (const double *)interface->ifrcx.data,
+  jobs[nj++] = {"k_rhs_ux", vofk::k_rhs_ux, fluid->srcux, &fluid->ux, (const double *)interface->ifrcx.data,
                 ctx->d_lapx_ux, fluid->m_dif, 0};
-  jobs[nj++] = {vofk::k_rhs_uy, fluid->srcuy, &fluid->uy, (const double *)interface->ifrcy.data,
+  jobs[nj++] = {"k_rhs_uy", vofk::k_rhs_uy, fluid->srcuy, &fluid->uy, (const double *)interface->ifrcy.data,
                 ctx->d_lapx_c, fluid->m_dif, 1};
 #if NDIMS == 3
-  jobs[nj++] = {vofk::k_rhs_uz, fluid->srcuz, &fluid->uz, ifrcz, ctx->d_lapx_c, fluid->m_dif, 2};
+  jobs[nj++] = {"k_rhs_uz", vofk::k_rhs_uz, fluid->srcuz, &fluid->uz, ifrcz, ctx->d_lapx_c, fluid->m_dif, 2};
 #endif
-  jobs[nj++] = {vofk::k_rhs_t, fluid->srct, &fluid->t, nullptr, ctx->d_lapx_c, fluid->t_dif, 3};
+  jobs[nj++] = {"k_rhs_t", vofk::k_rhs_t, fluid->srct, &fluid->t, nullptr, ctx->d_lapx_c, fluid->t_dif, 3};
   // compute_rhs (predict/main.c:31-40): every source from the old fields
   for (int f = 0; f < nj; f++) {
     vofk::RhsArgs a = base;
@@ -1052,7 +1111,7 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
     a.srcg = (double *)jobs[f].src[2].data;
     a.dif = jobs[f].dif;
     const int ilo_n = jobs[f].kind == 0 ? nx - 1 : nx;
-    hipLaunchKernelGGL(jobs[f].kern, grid3(ilo_n, ny, nz, kBlk), kBlk, 0, ctx->stream, ctx->grid, a);
+    KLN(ctx, jobs[f].name, jobs[f].kern, grid3(ilo_n, ny, nz, kBlk), kBlk, ctx->grid, a);
     LAUNCHCHK();
   }
   // predict (predict/main.c:42-53): increments, x solve, update, boundaries
@@ -1066,7 +1125,7 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
     double * fld = (double *)jobs[f].field->data;
     const int nlines = ny * nz;
     const int nb = (nlines + vofk::kTLines - 1) / vofk::kTLines;
-    hipLaunchKernelGGL(vofk::k_thomas_lines, dim3(nb), dim3(vofk::kTLines), 0, ctx->stream, dtab,
+    KLN(ctx, "k_thomas_lines", vofk::k_thomas_lines, dim3(nb), dim3(vofk::kTLines), dtab,
                        (const double *)jobs[f].src[0].data, (const double *)jobs[f].src[1].data,
                        (const double *)jobs[f].src[2].data, ca, cb, cg, ctx->d_delta, fld, n,
                        nlines, 1, n0, NDIMS == 3 ? ny + 2 : 0, 1, ny);
@@ -1096,29 +1155,28 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
 #else
   const double * uz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_div_rfft_y, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
-                     ctx->stream, ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
+  KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
                      (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
   LAUNCHCHK();
 #if NDIMS == 3
   const int NCz = vofk::kFftElems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
-  hipLaunchKernelGGL(vofk::k_fft_z, gz, dim3(vofk::kFftThreads), 0, ctx->stream, ctx->grid,
+  KLN(ctx, "k_fft_z", vofk::k_fft_z, gz, dim3(vofk::kFftThreads), ctx->grid,
                      ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, -1.);
   LAUNCHCHK();
 #endif
   const int nlines = ctx->nh * nz;
-  hipLaunchKernelGGL(vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), 0, ctx->stream, nx,
+  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), nx,
                      nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec,
                      ctx->d_vbuf);
   LAUNCHCHK();
 #if NDIMS == 3
-  hipLaunchKernelGGL(vofk::k_fft_z, gz, dim3(vofk::kFftThreads), 0, ctx->stream, ctx->grid,
+  KLN(ctx, "k_fft_z", vofk::k_fft_z, gz, dim3(vofk::kFftThreads), ctx->grid,
                      ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, +1.);
   LAUNCHCHK();
 #endif
-  hipLaunchKernelGGL(vofk::k_irfft_y_psi, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
-                     0, ctx->stream, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec,
+  KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+                     ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec,
                      (double *)fluid->psi.data);
   LAUNCHCHK();
   return bc_neumann(ctx, (double *)fluid->psi.data);
@@ -1145,7 +1203,7 @@ extern "C" int fluid_correct_velocity(const domain_t * domain, const size_t rkst
 #else
   double * uz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_correct, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+  KLN(ctx, "k_correct", vofk::k_correct, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk,
                      ctx->grid, prefactor, (const double *)fluid->psi.data,
                      (double *)fluid->ux.data, (double *)fluid->uy.data, uz);
   LAUNCHCHK();
@@ -1163,7 +1221,7 @@ extern "C" int fluid_update_pressure(const domain_t * domain, const size_t rkste
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
   const double prefactor = 0.5 * kRk[rkstep][2] * dt * fluid->m_dif;
-  hipLaunchKernelGGL(vofk::k_pressure, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+  KLN(ctx, "k_pressure", vofk::k_pressure, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk,
                      ctx->grid, prefactor, (const double *)fluid->psi.data, (double *)fluid->p.data);
   LAUNCHCHK();
   return bc_neumann(ctx, (double *)fluid->p.data);
@@ -1233,10 +1291,10 @@ extern "C" int interface_update_boundaries_vof(const domain_t * domain, array_t 
 extern "C" int interface_compute_curvature_tensor(const domain_t * domain, interface_t * interface) {
   CTX_OR_FAIL(domain);
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
-  hipLaunchKernelGGL(vofk::k_gradient, grid3(nx + 1, ny + 3, nz + 3, kBlk), kBlk, 0, ctx->stream,
+  KLN(ctx, "k_gradient", vofk::k_gradient, grid3(nx + 1, ny + 3, nz + 3, kBlk), kBlk,
                      ctx->grid, (const double *)interface->vof.data, (double *)interface->dvof.data);
   LAUNCHCHK();
-  hipLaunchKernelGGL(vofk::k_normal_curv, grid3(nx, ny + 2, nz + 2, kBlk), kBlk, 0, ctx->stream,
+  KLN(ctx, "k_normal_curv", vofk::k_normal_curv, grid3(nx, ny + 2, nz + 2, kBlk), kBlk,
                      ctx->grid, (const double *)interface->vof.data,
                      (const double *)interface->dvof.data, (double *)interface->normal.data,
                      (double *)interface->curv.data);
@@ -1251,7 +1309,7 @@ extern "C" int interface_compute_force(const domain_t * domain, interface_t * in
 #else
   double * ifrcz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_force, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk, 0, ctx->stream,
+  KLN(ctx, "k_force", vofk::k_force, grid3(ctx->nx, ctx->ny, ctx->nz, kBlk), kBlk,
                      ctx->grid, interface->tension, (const double *)interface->vof.data,
                      (const double *)interface->curv.data, (double *)interface->ifrcx.data,
                      (double *)interface->ifrcy.data, ifrcz);
@@ -1273,14 +1331,14 @@ extern "C" int interface_update_vof(const domain_t * domain, const size_t rkstep
   const double * uz = nullptr;
   double * flxz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_flux, grid3(nx, ny + 1, nz + 1, kBlk), kBlk, 0, ctx->stream, ctx->grid,
+  KLN(ctx, "k_flux", vofk::k_flux, grid3(nx, ny + 1, nz + 1, kBlk), kBlk, ctx->grid,
                      (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz,
                      (const double *)interface->vof.data, (const double *)interface->normal.data,
                      (double *)interface->flxx.data, (double *)interface->flxy.data, flxz);
   LAUNCHCHK();
   const double dta = dt * kRk[rkstep][0];
   const double dtb = dt * kRk[rkstep][1];
-  hipLaunchKernelGGL(vofk::k_vof_update, grid3(nx, ny, nz, kBlk), kBlk, 0, ctx->stream, ctx->grid,
+  KLN(ctx, "k_vof_update", vofk::k_vof_update, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid,
                      (const double *)interface->flxx.data, (const double *)interface->flxy.data,
                      (const double *)flxz, (double *)interface->src[0].data,
                      (const double *)interface->src[1].data, (double *)interface->vof.data, dta,
@@ -1326,7 +1384,7 @@ extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
 #else
   const double * uz = nullptr;
 #endif
-  hipLaunchKernelGGL(vofk::k_diag_partial, dim3(nb), dim3(vofk::kRedThreads), 0, ctx->stream,
+  KLN(ctx, "k_diag_partial", vofk::k_diag_partial, dim3(nb), dim3(vofk::kRedThreads),
                      ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz,
                      (const double *)fluid->t.data, (const double *)interface->vof.data, ctx->d_part);
   LAUNCHCHK();
@@ -1352,6 +1410,78 @@ extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
   out[o++] = q[7];
 #endif
   out[o++] = q[8];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel timing and the copy probe
+
+extern "C" int vof_profile(domain_t * domain, int on) {
+  CTX_OR_FAIL(domain);
+  if (on) {
+    if (ctx->ev_pool.empty()) {
+      ctx->ev_pool.resize(8192);
+      for (auto & e : ctx->ev_pool) HIPCHK(hipEventCreate(&e));
+      ctx->ev_next = 0;
+    }
+    ctx->prof_on = true;
+  } else {
+    if (prof_flush(ctx)) return 1;
+    ctx->prof_on = false;
+  }
+  return 0;
+}
+
+extern "C" int vof_profile_reset(domain_t * domain) {
+  CTX_OR_FAIL(domain);
+  if (prof_flush(ctx)) return 1;
+  ctx->prof_names.clear();
+  ctx->prof_ms.clear();
+  ctx->prof_cnt.clear();
+  return 0;
+}
+
+// "name launches total_ms" lines
+extern "C" int vof_profile_report(domain_t * domain, char * buf, size_t len) {
+  CTX_OR_FAIL(domain);
+  if (prof_flush(ctx)) return 1;
+  std::string out;
+  for (size_t m = 0; m < ctx->prof_names.size(); m++) {
+    char line[256];
+    snprintf(line, sizeof line, "%s %ld %.9f\n", ctx->prof_names[m].c_str(), ctx->prof_cnt[m], ctx->prof_ms[m]);
+    out += line;
+  }
+  if (out.size() + 1 > len) return fail("vof_profile_report: buffer too small (%zu bytes needed)", out.size() + 1);
+  memcpy(buf, out.c_str(), out.size() + 1);
+  return 0;
+}
+
+// device copy of n doubles, `reps` times; returns the mean (read + write)
+// bandwidth in GB/s.  Calibrates hardware byte counters against a kernel
+// of known traffic.
+extern "C" int vof_copy_probe(domain_t * domain, size_t n, int reps, double * gbps) {
+  CTX_OR_FAIL(domain);
+  double * a = nullptr;
+  double * b = nullptr;
+  HIPCHK(hipMalloc(&a, sizeof(double) * n));
+  HIPCHK(hipMalloc(&b, sizeof(double) * n));
+  HIPCHK(hipMemsetAsync(a, 0, sizeof(double) * n, ctx->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const unsigned nb = 4096;
+  KLN(ctx, "k_copy", vofk::k_copy, dim3(nb), dim3(256), (const double *)a, b, n);
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; r++) KLN(ctx, "k_copy", vofk::k_copy, dim3(nb), dim3(256), (const double *)a, b, n);
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2. * sizeof(double) * (double)n * reps / (ms * 1e-3) / 1e9;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
   return 0;
 }
 

@@ -167,7 +167,9 @@ def velocity_from_streamfunction_2d(xf, ny, ly, psi_fn):
 def velocity_from_potential_3d(xf, ny, ly, nz, lz, ay_fn, az_fn, ax_fn):
     """u = curl A on the staggered grid.  A_x on (x_c, y_{j-1/2}, z_{k-1/2}),
     A_y on (x_f, y_c, z_{k-1/2}), A_z on (x_f, y_{j-1/2}, z_c); A_y = A_z = 0
-    on the x walls, so ux vanishes there.  Discretely divergence free."""
+    on the x walls, so ux vanishes there.  Discretely divergence free.
+    The potential functions take open (broadcastable) coordinate grids
+    z[:, None, None], y[None, :, None], x[None, None, :]."""
     nx = len(xf) - 1
     dy, dz = ly / ny, lz / nz
     xc = 0.5 * (xf[:-1] + xf[1:])
@@ -175,17 +177,17 @@ def velocity_from_potential_3d(xf, ny, ly, nz, lz, ay_fn, az_fn, ax_fn):
     zf = np.arange(nz) * dz
     yc = yf + 0.5 * dy
     zc = zf + 0.5 * dz
-    # A_z(i_face, j_corner, k_centre), A_y(i_face, j_centre, k_corner)
-    Zc, Yf, Xf = np.meshgrid(zc, yf, xf, indexing="ij")
-    az = az_fn(Xf, Yf, Zc)
-    Zf, Yc, Xf2 = np.meshgrid(zf, yc, xf, indexing="ij")
-    ay = ay_fn(Xf2, Yc, Zf)
+    o3 = lambda z, y, x: (z[:, None, None], y[None, :, None], x[None, None, :])  # noqa: E731
+    Z, Y, X = o3(zc, yf, xf)
+    az = np.broadcast_to(az_fn(X, Y, Z), (nz, ny, nx + 1)).copy()
+    Z, Y, X = o3(zf, yc, xf)
+    ay = np.broadcast_to(ay_fn(X, Y, Z), (nz, ny, nx + 1)).copy()
     az[:, :, 0] = 0.0
     az[:, :, -1] = 0.0
     ay[:, :, 0] = 0.0
     ay[:, :, -1] = 0.0
-    Zf3, Yf3, Xc3 = np.meshgrid(zf, yf, xc, indexing="ij")
-    ax = ax_fn(Xc3, Yf3, Zf3)
+    Z, Y, X = o3(zf, yf, xc)
+    ax = np.broadcast_to(ax_fn(X, Y, Z), (nz, ny, nx)).copy()
     roll = np.roll
     # ux(i_face, j, k) = d_y A_z - d_z A_y
     ux = (roll(az, -1, axis=1) - az) / dy - (roll(ay, -1, axis=0) - ay) / dz
@@ -215,10 +217,10 @@ def _random_modes(rng, nmodes, lx, ly, lz=None, kmax=4):
 
 def _modal_fn(modes, comp, lx, ly, lz):
     def fn(X, Y, Z):
-        out = np.zeros_like(X, dtype=np.float64)
+        out = 0.0
         for (kx, ky, kz, amp, phase) in modes:
             arg = 2.0 * np.pi * (ky * Y / ly + (kz * Z / lz if lz else 0.0)) + phase[comp]
-            out += amp[comp] * np.sin(np.pi * kx * X / lx) * np.cos(arg)
+            out = out + (amp[comp] * np.cos(arg)) * np.sin(np.pi * kx * X / lx)
         return out
     return fn
 
