@@ -42,11 +42,15 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         "k_force": c * (8 + 8 + 24),
         "k_flux": c * (24 + 8 + 24),
         "k_vof_update": c * (24 + 8 + 8 + 8 + 8),
-        "k_rhs_ux": (nx - 1) * ny * nz * (48 + 16),
-        "k_rhs_uy": c * (40 + 16),
-        "k_rhs_uz": c * (40 + 16),
-        "k_rhs_t": c * (32 + 16),
-        "k_thomas_lines": (3 * nx + (nx - 1)) / 4.0 * ny * nz * (24 + 8 + 8),
+        # fields read (ux: ux uy uz p t ifrcx; uy/uz: ux uy uz p ifrc; t: ux uy
+        # uz t) + srcb read, srca + increment written
+        "k_rhs_ux": (nx - 1) * ny * nz * (48 + 8 + 16),
+        "k_rhs_uy": c * (40 + 8 + 16),
+        "k_rhs_uz": c * (40 + 8 + 16),
+        "k_rhs_t": c * (32 + 8 + 16),
+        # four fields (ux: nx-1 DOFs per line, uy uz t: nx) in one launch:
+        # increment read, field read and write
+        "k_lines_lds": ((nx - 1) + 3 * nx) * ny * nz * (8 + 8 + 8),
         "k_div_rfft_y": c * 24 + s * 16,
         "k_fft_z": s * 32,
         "k_thomas_spec": s * 32,

@@ -797,22 +797,18 @@ static void compute_gradient(orc_t * o) {
     DVOF(i, j, k)[0] = dvofdx * norminv;
     DVOF(i, j, k)[1] = dvofdy * norminv;
 #else
+    /* per-plane partial sums in the 2D order, so that a z-uniform field
+     * reproduces the 2D corner normals exactly */
     const double dz = o->dz;
-    const double dvofdx = 1. / dx * (
-        - VOF(i - 1, j - 1, k - 1) + VOF(i, j - 1, k - 1)
-        - VOF(i - 1, j,     k - 1) + VOF(i, j,     k - 1)
-        - VOF(i - 1, j - 1, k    ) + VOF(i, j - 1, k    )
-        - VOF(i - 1, j,     k    ) + VOF(i, j,     k    ));
-    const double dvofdy = 1. / dy * (
-        - VOF(i - 1, j - 1, k - 1) - VOF(i, j - 1, k - 1)
-        + VOF(i - 1, j,     k - 1) + VOF(i, j,     k - 1)
-        - VOF(i - 1, j - 1, k    ) - VOF(i, j - 1, k    )
-        + VOF(i - 1, j,     k    ) + VOF(i, j,     k    ));
-    const double dvofdz = 1. / dz * (
-        - VOF(i - 1, j - 1, k - 1) - VOF(i, j - 1, k - 1)
-        - VOF(i - 1, j,     k - 1) - VOF(i, j,     k - 1)
-        + VOF(i - 1, j - 1, k    ) + VOF(i, j - 1, k    )
-        + VOF(i - 1, j,     k    ) + VOF(i, j,     k    ));
+    const double sx0 = -VOF(i - 1, j - 1, k - 1) + VOF(i, j - 1, k - 1) - VOF(i - 1, j, k - 1) + VOF(i, j, k - 1);
+    const double sx1 = -VOF(i - 1, j - 1, k) + VOF(i, j - 1, k) - VOF(i - 1, j, k) + VOF(i, j, k);
+    const double sy0 = -VOF(i - 1, j - 1, k - 1) - VOF(i, j - 1, k - 1) + VOF(i - 1, j, k - 1) + VOF(i, j, k - 1);
+    const double sy1 = -VOF(i - 1, j - 1, k) - VOF(i, j - 1, k) + VOF(i - 1, j, k) + VOF(i, j, k);
+    const double sz0 = VOF(i - 1, j - 1, k - 1) + VOF(i, j - 1, k - 1) + VOF(i - 1, j, k - 1) + VOF(i, j, k - 1);
+    const double sz1 = VOF(i - 1, j - 1, k) + VOF(i, j - 1, k) + VOF(i - 1, j, k) + VOF(i, j, k);
+    const double dvofdx = 1. / dx * (sx0 + sx1);
+    const double dvofdy = 1. / dy * (sy0 + sy1);
+    const double dvofdz = 1. / dz * (-sz0 + sz1);
     const double norm = sqrt(dvofdx * dvofdx + dvofdy * dvofdy + dvofdz * dvofdz);
     const double norminv = 1. / fmax(norm, DBL_EPSILON);
     DVOF(i, j, k)[0] = dvofdx * norminv;
@@ -846,15 +842,26 @@ static double compute_intercept(const double vof, const double * normal) {
     for (int jj = 0; jj < NGAUSS; jj++) for (int ii = 0; ii < NGAUSS; ii++) {
       const double weight = gauss_ws[ii] * gauss_ws[jj];
       const double p = exps[jj * NGAUSS + ii];
-#else
-    for (int kk = 0; kk < NGAUSS; kk++) for (int jj = 0; jj < NGAUSS; jj++) for (int ii = 0; ii < NGAUSS; ii++) {
-      const double weight = gauss_ws[ii] * gauss_ws[jj] * gauss_ws[kk];
-      const double p = exps[(kk * NGAUSS + jj) * NGAUSS + ii];
-#endif
       const double denom = 1. / (1. + p * val);
       f0 += weight * denom;
       f1 -= weight * p * denom * denom;
     }
+#else
+    /* z weights innermost: with a zero z normal the two z points coincide
+     * and the sums reduce exactly to the 2D ones */
+    for (int jj = 0; jj < NGAUSS; jj++) for (int ii = 0; ii < NGAUSS; ii++) {
+      const double weight = gauss_ws[ii] * gauss_ws[jj];
+      double s0 = 0., s1 = 0.;
+      for (int kk = 0; kk < NGAUSS; kk++) {
+        const double p = exps[(kk * NGAUSS + jj) * NGAUSS + ii];
+        const double denom = 1. / (1. + p * val);
+        s0 += gauss_ws[kk] * denom;
+        s1 += gauss_ws[kk] * (weight * p * denom * denom);
+      }
+      f0 += weight * s0;
+      f1 -= s1;
+    }
+#endif
     val -= f0 / f1;
     if (fabs(f0) < resmax) break;
   }
@@ -894,8 +901,8 @@ static void compute_normal(orc_t * o) {
 #else
     double n3[3];
     for (int d = 0; d < 3; d++) {
-      n3[d] = DVOF(i, j, k)[d] + DVOF(i + 1, j, k)[d] + DVOF(i, j + 1, k)[d] + DVOF(i + 1, j + 1, k)[d]
-            + DVOF(i, j, k + 1)[d] + DVOF(i + 1, j, k + 1)[d] + DVOF(i, j + 1, k + 1)[d] + DVOF(i + 1, j + 1, k + 1)[d];
+      n3[d] = (DVOF(i, j, k)[d] + DVOF(i + 1, j, k)[d] + DVOF(i, j + 1, k)[d] + DVOF(i + 1, j + 1, k)[d])
+            + (DVOF(i, j, k + 1)[d] + DVOF(i + 1, j, k + 1)[d] + DVOF(i, j + 1, k + 1)[d] + DVOF(i + 1, j + 1, k + 1)[d]);
     }
     n3[0] /= dx;
     n3[1] /= dy;
@@ -934,20 +941,14 @@ static void compute_curvature(orc_t * o) {
 #else
     const double dz = o->dz;
     const double dnxdx = 1. / dx * (
-        - DVOF(i, j, k)[0] + DVOF(i + 1, j, k)[0]
-        - DVOF(i, j + 1, k)[0] + DVOF(i + 1, j + 1, k)[0]
-        - DVOF(i, j, k + 1)[0] + DVOF(i + 1, j, k + 1)[0]
-        - DVOF(i, j + 1, k + 1)[0] + DVOF(i + 1, j + 1, k + 1)[0]);
+        (- DVOF(i, j, k)[0] + DVOF(i + 1, j, k)[0] - DVOF(i, j + 1, k)[0] + DVOF(i + 1, j + 1, k)[0])
+      + (- DVOF(i, j, k + 1)[0] + DVOF(i + 1, j, k + 1)[0] - DVOF(i, j + 1, k + 1)[0] + DVOF(i + 1, j + 1, k + 1)[0]));
     const double dnydy = 1. / dy * (
-        - DVOF(i, j, k)[1] - DVOF(i + 1, j, k)[1]
-        + DVOF(i, j + 1, k)[1] + DVOF(i + 1, j + 1, k)[1]
-        - DVOF(i, j, k + 1)[1] - DVOF(i + 1, j, k + 1)[1]
-        + DVOF(i, j + 1, k + 1)[1] + DVOF(i + 1, j + 1, k + 1)[1]);
+        (- DVOF(i, j, k)[1] - DVOF(i + 1, j, k)[1] + DVOF(i, j + 1, k)[1] + DVOF(i + 1, j + 1, k)[1])
+      + (- DVOF(i, j, k + 1)[1] - DVOF(i + 1, j, k + 1)[1] + DVOF(i, j + 1, k + 1)[1] + DVOF(i + 1, j + 1, k + 1)[1]));
     const double dnzdz = 1. / dz * (
-        - DVOF(i, j, k)[2] - DVOF(i + 1, j, k)[2]
-        - DVOF(i, j + 1, k)[2] - DVOF(i + 1, j + 1, k)[2]
-        + DVOF(i, j, k + 1)[2] + DVOF(i + 1, j, k + 1)[2]
-        + DVOF(i, j + 1, k + 1)[2] + DVOF(i + 1, j + 1, k + 1)[2]);
+        - (DVOF(i, j, k)[2] + DVOF(i + 1, j, k)[2] + DVOF(i, j + 1, k)[2] + DVOF(i + 1, j + 1, k)[2])
+        + (DVOF(i, j, k + 1)[2] + DVOF(i + 1, j, k + 1)[2] + DVOF(i, j + 1, k + 1)[2] + DVOF(i + 1, j + 1, k + 1)[2]));
     CURV(i, j, k) = 0.25 * (-dnxdx - dnydy - dnzdz);
 #endif
   }
@@ -1026,10 +1027,13 @@ static void compute_flux_x(orc_t * o) {
       flux += w * indicator(NORMAL(ii, j, k), pt);
     }
 #else
-    for (int kk = 0; kk < NGAUSS; kk++) for (int jj = 0; jj < NGAUSS; jj++) {
-      const double w = gauss_ws[jj] * gauss_ws[kk];
-      const double pt[3] = {x, gauss_ps[jj], gauss_ps[kk]};
-      flux += w * indicator(NORMAL(ii, j, k), pt);
+    for (int jj = 0; jj < NGAUSS; jj++) {
+      double sz = 0.;
+      for (int kk = 0; kk < NGAUSS; kk++) {
+        const double pt[3] = {x, gauss_ps[jj], gauss_ps[kk]};
+        sz += gauss_ws[kk] * indicator(NORMAL(ii, j, k), pt);
+      }
+      flux += gauss_ws[jj] * sz;
     }
 #endif
     FLXX(i, j, k) = vel * flux;
@@ -1056,10 +1060,13 @@ static void compute_flux_y(orc_t * o) {
       flux += w * indicator(NORMAL(i, jj, k), pt);
     }
 #else
-    for (int kk = 0; kk < NGAUSS; kk++) for (int ii = 0; ii < NGAUSS; ii++) {
-      const double w = gauss_ws[ii] * gauss_ws[kk];
-      const double pt[3] = {gauss_ps[ii], y, gauss_ps[kk]};
-      flux += w * indicator(NORMAL(i, jj, k), pt);
+    for (int ii = 0; ii < NGAUSS; ii++) {
+      double sz = 0.;
+      for (int kk = 0; kk < NGAUSS; kk++) {
+        const double pt[3] = {gauss_ps[ii], y, gauss_ps[kk]};
+        sz += gauss_ws[kk] * indicator(NORMAL(i, jj, k), pt);
+      }
+      flux += gauss_ws[ii] * sz;
     }
 #endif
     FLXY(i, j, k) = vel * flux;
@@ -1080,10 +1087,13 @@ static void compute_flux_z(orc_t * o) {
     const double lvof = VOF(i, j, kk0);
     if (lvof < vofmin || 1. - vofmin < lvof) { FLXZ(i, j, k) = vel * lvof; continue; }
     double flux = 0.;
-    for (int jj = 0; jj < NGAUSS; jj++) for (int ii = 0; ii < NGAUSS; ii++) {
-      const double w = gauss_ws[ii] * gauss_ws[jj];
-      const double pt[3] = {gauss_ps[ii], gauss_ps[jj], z};
-      flux += w * indicator(NORMAL(i, j, kk0), pt);
+    for (int ii = 0; ii < NGAUSS; ii++) {
+      double sy = 0.;
+      for (int jj = 0; jj < NGAUSS; jj++) {
+        const double pt[3] = {gauss_ps[ii], gauss_ps[jj], z};
+        sy += gauss_ws[jj] * indicator(NORMAL(i, j, kk0), pt);
+      }
+      flux += gauss_ws[ii] * sy;
     }
     FLXZ(i, j, k) = vel * flux;
   }

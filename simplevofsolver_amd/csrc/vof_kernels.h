@@ -157,9 +157,14 @@ __global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __r
   const double c0 = vof[I_VOF(i - 1, j, k - 1)], d0 = vof[I_VOF(i, j, k - 1)];
   const double a1 = vof[I_VOF(i - 1, j - 1, k)], b1 = vof[I_VOF(i, j - 1, k)];
   const double c1 = vof[I_VOF(i - 1, j, k)], d1 = vof[I_VOF(i, j, k)];
-  const double gx = 1. / dx * (-a0 + b0 - c0 + d0 - a1 + b1 - c1 + d1);
-  const double gy = 1. / g.dy * (-a0 - b0 + c0 + d0 - a1 - b1 + c1 + d1);
-  const double gz = 1. / g.dz * (-a0 - b0 - c0 - d0 + a1 + b1 + c1 + d1);
+  // per-plane partial sums in the 2D order: a z-uniform field reproduces the
+  // 2D corner normals exactly
+  const double sx0 = -a0 + b0 - c0 + d0, sx1 = -a1 + b1 - c1 + d1;
+  const double sy0 = -a0 - b0 + c0 + d0, sy1 = -a1 - b1 + c1 + d1;
+  const double sz0 = a0 + b0 + c0 + d0, sz1 = a1 + b1 + c1 + d1;
+  const double gx = 1. / dx * (sx0 + sx1);
+  const double gy = 1. / g.dy * (sy0 + sy1);
+  const double gz = 1. / g.dz * (-sz0 + sz1);
   const double norm = sqrt(gx * gx + gy * gy + gz * gz);
   const double ninv = 1. / fmax(norm, DBL_EPSILON);
   double * o = dvof + 3 * I_DVOF(i, j, k);
@@ -196,17 +201,26 @@ __device__ inline double intercept(const double vof, const double * n) {
       for (int ii = 0; ii < NGAUSS; ii++) {
         const double w = kGaussW[ii] * kGaussW[jj];
         const double p = e[jj * NGAUSS + ii];
-#else
-    for (int kk = 0; kk < NGAUSS; kk++)
-      for (int jj = 0; jj < NGAUSS; jj++)
-        for (int ii = 0; ii < NGAUSS; ii++) {
-          const double w = kGaussW[ii] * kGaussW[jj] * kGaussW[kk];
-          const double p = e[(kk * NGAUSS + jj) * NGAUSS + ii];
-#endif
         const double den = 1. / (1. + p * val);
         f0 += w * den;
         f1 -= w * p * den * den;
       }
+#else
+    // z weights innermost: with a zero z normal the sums reduce exactly to 2D
+    for (int jj = 0; jj < NGAUSS; jj++)
+      for (int ii = 0; ii < NGAUSS; ii++) {
+        const double w = kGaussW[ii] * kGaussW[jj];
+        double s0 = 0., s1 = 0.;
+        for (int kk = 0; kk < NGAUSS; kk++) {
+          const double p = e[(kk * NGAUSS + jj) * NGAUSS + ii];
+          const double den = 1. / (1. + p * val);
+          s0 += kGaussW[kk] * den;
+          s1 += kGaussW[kk] * (w * p * den * den);
+        }
+        f0 += w * s0;
+        f1 -= s1;
+      }
+#endif
     val -= f0 / f1;
     if (fabs(f0) < resmax) break;
   }
@@ -258,18 +272,18 @@ __global__ void k_normal_curv(Grid g, const double * __restrict__ vof,
   const double * d101 = dvof + 3 * I_DVOF(i + 1, j, k + 1);
   const double * d011 = dvof + 3 * I_DVOF(i, j + 1, k + 1);
   const double * d111 = dvof + 3 * I_DVOF(i + 1, j + 1, k + 1);
-  const double dnxdx = 1. / dx * (-d000[0] + d100[0] - d010[0] + d110[0]
-                                  - d001[0] + d101[0] - d011[0] + d111[0]);
-  const double dnydy = 1. / g.dy * (-d000[1] - d100[1] + d010[1] + d110[1]
-                                    - d001[1] - d101[1] + d011[1] + d111[1]);
-  const double dnzdz = 1. / g.dz * (-d000[2] - d100[2] - d010[2] - d110[2]
-                                    + d001[2] + d101[2] + d011[2] + d111[2]);
+  const double dnxdx = 1. / dx * ((-d000[0] + d100[0] - d010[0] + d110[0])
+                                  + (-d001[0] + d101[0] - d011[0] + d111[0]));
+  const double dnydy = 1. / g.dy * ((-d000[1] - d100[1] + d010[1] + d110[1])
+                                    + (-d001[1] - d101[1] + d011[1] + d111[1]));
+  const double dnzdz = 1. / g.dz * (-(d000[2] + d100[2] + d010[2] + d110[2])
+                                    + (d001[2] + d101[2] + d011[2] + d111[2]));
   curv[I_C1(i, j, k)] = 0.25 * (-dnxdx - dnydy - dnzdz);
   const double lvof = vof[I_VOF(i, j, k)];
   if (lvof < kVofMin || 1. - kVofMin < lvof) return;
   double nv[3];
   for (int d = 0; d < 3; d++) {
-    nv[d] = d000[d] + d100[d] + d010[d] + d110[d] + d001[d] + d101[d] + d011[d] + d111[d];
+    nv[d] = (d000[d] + d100[d] + d010[d] + d110[d]) + (d001[d] + d101[d] + d011[d] + d111[d]);
   }
   nv[0] /= dx;
   nv[1] /= g.dy;
@@ -368,12 +382,14 @@ __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __
         flux += kGaussW[jj] * indicator(n, pt);
       }
 #else
-      for (int kk = 0; kk < NGAUSS; kk++)
-        for (int jj = 0; jj < NGAUSS; jj++) {
-          const double w = kGaussW[jj] * kGaussW[kk];
+      for (int jj = 0; jj < NGAUSS; jj++) {
+        double sz = 0.;
+        for (int kk = 0; kk < NGAUSS; kk++) {
           const double pt[3] = {x, kGaussP[jj], kGaussP[kk]};
-          flux += w * indicator(n, pt);
+          sz += kGaussW[kk] * indicator(n, pt);
         }
+        flux += kGaussW[jj] * sz;
+      }
 #endif
       out = vel * flux;
     }
@@ -396,12 +412,14 @@ __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __
         flux += kGaussW[ii] * indicator(n, pt);
       }
 #else
-      for (int kk = 0; kk < NGAUSS; kk++)
-        for (int ii = 0; ii < NGAUSS; ii++) {
-          const double w = kGaussW[ii] * kGaussW[kk];
+      for (int ii = 0; ii < NGAUSS; ii++) {
+        double sz = 0.;
+        for (int kk = 0; kk < NGAUSS; kk++) {
           const double pt[3] = {kGaussP[ii], y, kGaussP[kk]};
-          flux += w * indicator(n, pt);
+          sz += kGaussW[kk] * indicator(n, pt);
         }
+        flux += kGaussW[ii] * sz;
+      }
 #endif
       out = vel * flux;
     }
@@ -419,12 +437,14 @@ __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __
     } else {
       const double * n = normal + NC * I_NRM(i, j, kk0);
       double flux = 0.;
-      for (int jj = 0; jj < NGAUSS; jj++)
-        for (int ii = 0; ii < NGAUSS; ii++) {
-          const double w = kGaussW[ii] * kGaussW[jj];
+      for (int ii = 0; ii < NGAUSS; ii++) {
+        double sy = 0.;
+        for (int jj = 0; jj < NGAUSS; jj++) {
           const double pt[3] = {kGaussP[ii], kGaussP[jj], z};
-          flux += w * indicator(n, pt);
+          sy += kGaussW[jj] * indicator(n, pt);
         }
+        flux += kGaussW[ii] * sy;
+      }
       out = vel * flux;
     }
     flxz[I_FZ(i, j, k)] = out;
@@ -471,10 +491,25 @@ struct RhsArgs {
   const double * __restrict__ ifrc;
   const double * __restrict__ lapx;   // 3 per x point
   double * __restrict__ srca;
-  double * __restrict__ srcg;
+  double * __restrict__ srcg;          // implicit source (reference layout) or
+  double * __restrict__ delta;         // ... the increment, when non-null
+  const double * __restrict__ srcb;    // previous explicit source (delta only)
+  double ca, cb, cg;                   // coef * dt (delta only)
   double dif;
   int add_buoyancy;
 };
+
+// store the explicit source and either the implicit source (the
+// per-function entry point keeps the reference's fluid_t state) or directly
+// the increment a dt srca + b dt srcb + g dt srcg (ux.c:335-344)
+__device__ inline void rhs_store(const RhsArgs & a, size_t is, double sa, double sg) {
+  a.srca[is] = sa;
+  if (a.delta) {
+    a.delta[is] = +a.ca * sa + a.cb * a.srcb[is] + a.cg * sg;
+  } else {
+    a.srcg[is] = sg;
+  }
+}
 
 // compute_rhs_ux (ux.c:71-249), DOFs i = 2..nx
 __global__ void k_rhs_ux(Grid g, RhsArgs a) {
@@ -536,8 +571,7 @@ __global__ void k_rhs_ux(Grid g, RhsArgs a) {
   if (a.add_buoyancy) sa += +0.5 * a.t[I_C1(i - 1, j, k)] + 0.5 * a.t[I_C1(i, j, k)];
   const size_t is = I_SX(i, j, k);
   sa += a.ifrc[is];
-  a.srca[is] = sa;
-  a.srcg[is] = sg;
+  rhs_store(a, is, sa, sg);
 }
 
 // compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx
@@ -597,8 +631,7 @@ __global__ void k_rhs_uy(Grid g, RhsArgs a) {
   sg -= 1. / g.dy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
   const size_t is = I_SC(i, j, k);
   sa += a.ifrc[is];
-  a.srca[is] = sa;
-  a.srcg[is] = sg;
+  rhs_store(a, is, sa, sg);
 }
 
 #if NDIMS == 3
@@ -655,8 +688,7 @@ __global__ void k_rhs_uz(Grid g, RhsArgs a) {
   sg -= 1. / g.dz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
   const size_t is = I_SC(i, j, k);
   sa += a.ifrc[is];
-  a.srca[is] = sa;
-  a.srcg[is] = sg;
+  rhs_store(a, is, sa, sg);
 }
 #endif
 
@@ -710,8 +742,7 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
   }
 #endif
   const size_t is = I_SC(i, j, k);
-  a.srca[is] = sa;
-  a.srcg[is] = sg;
+  rhs_store(a, is, sa, sg);
 }
 
 // ---------------------------------------------------------------------------
@@ -720,96 +751,239 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
 //
 // The tri-diagonal matrix (I - prefactor Lx) is the same for every line, so
 // the elimination factors the reference recomputes per line are prepared
-// once on the host with the same arithmetic: tab = {c0, rlast, flag, -,
-// l[n], val[n], v[n]}.  A work-group of 64 lanes solves 64 consecutive
-// lines; x is streamed through LDS in chunks so that global accesses stay
-// contiguous while each lane walks its own line.
-constexpr int kTLines = 64;
-constexpr int kTChunk = 32;
+// once on the host with the same arithmetic.
+// Multi-field line solver: one launch solves the x lines of up to four
+// fields (blockIdx.y selects the field).  256 lines per work-group, one line
+// per lane; x is streamed through LDS in chunks of CH values stored x-major
+// (tile[m][line]) so that the cooperative global accesses are contiguous
+// rows and each lane's walk along its line is bank-conflict free.
+struct LineField {
+  const double * tab;        // {c0, rlast, flag, -, l[n], val[n], v[n]}
+  const double * srca;       // increments from the three sources when srcg
+  const double * srcb;       //   is non-null (per-function entry point) ...
+  const double * srcg;
+  double * q;                // ... else q holds them; forward result goes to q
+  double * field;
+  int n, n0, i0;             // line length, padded x width, first DOF column
+};
+struct LineJob {
+  LineField f[4];
+  int nlines, ny, n1;        // lines per field, rows per plane, padded rows (0 in 2D)
+  double ca, cb, cg;
+};
 
-__global__ __launch_bounds__(kTLines) void k_thomas_lines(
-    const double * __restrict__ tab, const double * __restrict__ srca,
-    const double * __restrict__ srcb, const double * __restrict__ srcg, double ca, double cb,
-    double cg, double * __restrict__ q, double * __restrict__ field, int n, int nlines,
-    int fld_i0, int fld_n0, int fld_n1, int fld_j0, int ny) {
-  __shared__ double tile[kTLines][kTChunk + 1];
-  const int lane = threadIdx.x;
-  const int line0 = blockIdx.x * kTLines;
-  const int myline = line0 + lane;
-  const double c0 = tab[0], rlast = tab[1];
-  const int flag = tab[2] != 0.;
-  const double * L = tab + 4;
-  const double * VAL = tab + 4 + n;
-  const double * V = tab + 4 + 2 * n;
-  // forward elimination
+constexpr int kLLines = 256;
+constexpr int kLStride = 264;
+
+template <int CH>
+__global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
+  __shared__ double T[CH][kLStride];
+  __shared__ double U[CH][kLStride];
+  const LineField & F = J.f[blockIdx.y];
+  const int tid = threadIdx.x;
+  const int line0 = blockIdx.x * kLLines;
+  const int myline = line0 + tid;
+  const int n = F.n;
+  const int nlines = J.nlines;
+  const double c0 = F.tab[0], rlast = F.tab[1];
+  const bool flag = F.tab[2] != 0.;
+  const double * __restrict__ L = F.tab + 4;
+  const double * __restrict__ VAL = F.tab + 4 + n;
+  const double * __restrict__ V = F.tab + 4 + 2 * n;
   double qprev = 0.;
-  for (int c = 0; c < n; c += kTChunk) {
-    const int w = min(kTChunk, n - c);
-    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
-      const int ll = e / kTChunk, m = e % kTChunk;
-      if (m < w && line0 + ll < nlines) {
-        const size_t g = (size_t)(line0 + ll) * n + c + m;
-        tile[ll][m] = +ca * srca[g] + cb * srcb[g] + cg * srcg[g];
+  for (int c = 0; c < n; c += CH) {
+    const int w = min(CH, n - c);
+    for (int e = tid; e < kLLines * CH; e += kLLines) {
+      const int ll = e / CH, m = e % CH;
+      const int line = line0 + ll;
+      if (m < w && line < nlines) {
+        const size_t g = (size_t)line * n + c + m;
+        T[m][ll] = F.srcg ? +J.ca * F.srca[g] + J.cb * F.srcb[g] + J.cg * F.srcg[g] : F.q[g];
       }
     }
     __syncthreads();
     if (myline < nlines) {
-      for (int m = 0; m < w; m++) {
-        const int ii = c + m;
-        double qi = tile[lane][m];
-        if (ii == 0) {
-          qi = qi / c0;
-        } else if (ii < n - 1) {
-          qi = VAL[ii] * (qi - L[ii] * qprev);
-        } else {
-          qi = flag ? rlast * (qi - L[ii] * qprev) : 0.;
+#pragma unroll
+      for (int m = 0; m < CH; m++) {
+        if (m < w) {
+          const int ii = c + m;
+          double qi = T[m][tid];
+          if (ii == 0) {
+            qi = qi / c0;
+          } else if (ii < n - 1) {
+            qi = VAL[ii] * (qi - L[ii] * qprev);
+          } else {
+            qi = flag ? rlast * (qi - L[ii] * qprev) : 0.;
+          }
+          T[m][tid] = qi;
+          qprev = qi;
         }
-        tile[lane][m] = qi;
-        qprev = qi;
       }
     }
     __syncthreads();
-    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
-      const int ll = e / kTChunk, m = e % kTChunk;
-      if (m < w && line0 + ll < nlines) q[(size_t)(line0 + ll) * n + c + m] = tile[ll][m];
+    for (int e = tid; e < kLLines * CH; e += kLLines) {
+      const int ll = e / CH, m = e % CH;
+      const int line = line0 + ll;
+      if (m < w && line < nlines) F.q[(size_t)line * n + c + m] = T[m][ll];
     }
     __syncthreads();
   }
-  // backward substitution fused with field += q
-  __shared__ double ftile[kTLines][kTChunk + 1];
   double qnext = 0.;
-  const int last_c = ((n - 1) / kTChunk) * kTChunk;
-  for (int c = last_c; c >= 0; c -= kTChunk) {
-    const int w = min(kTChunk, n - c);
-    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
-      const int ll = e / kTChunk, m = e % kTChunk;
+  const int last_c = ((n - 1) / CH) * CH;
+  for (int c = last_c; c >= 0; c -= CH) {
+    const int w = min(CH, n - c);
+    for (int e = tid; e < kLLines * CH; e += kLLines) {
+      const int ll = e / CH, m = e % CH;
       const int line = line0 + ll;
       if (m < w && line < nlines) {
-        tile[ll][m] = q[(size_t)line * n + c + m];
-        const int jj = line % ny, kk = line / ny;
-        ftile[ll][m] = field[(size_t)(fld_i0 + c + m) + (size_t)fld_n0 * ((size_t)(jj + fld_j0) + (size_t)fld_n1 * (size_t)(kk + fld_j0))];
+        T[m][ll] = F.q[(size_t)line * n + c + m];
+        const int jj = line % J.ny, kk = line / J.ny;
+        U[m][ll] = F.field[(size_t)(F.i0 + c + m) + (size_t)F.n0 * ((size_t)(jj + 1) + (size_t)J.n1 * (size_t)(kk + 1))];
       }
     }
     __syncthreads();
     if (myline < nlines) {
-      for (int m = w - 1; m >= 0; m--) {
-        const int ii = c + m;
-        double qi = tile[lane][m];
-        if (ii < n - 1) qi -= V[ii] * qnext;
-        qnext = qi;
-        ftile[lane][m] += qi;
+#pragma unroll
+      for (int m = CH - 1; m >= 0; m--) {
+        if (m < w) {
+          const int ii = c + m;
+          double qi = T[m][tid];
+          if (ii < n - 1) qi -= V[ii] * qnext;
+          qnext = qi;
+          U[m][tid] += qi;
+        }
       }
     }
     __syncthreads();
-    for (int e = lane; e < kTLines * kTChunk; e += kTLines) {
-      const int ll = e / kTChunk, m = e % kTChunk;
+    for (int e = tid; e < kLLines * CH; e += kLLines) {
+      const int ll = e / CH, m = e % CH;
       const int line = line0 + ll;
       if (m < w && line < nlines) {
-        const int jj = line % ny, kk = line / ny;
-        field[(size_t)(fld_i0 + c + m) + (size_t)fld_n0 * ((size_t)(jj + fld_j0) + (size_t)fld_n1 * (size_t)(kk + fld_j0))] = ftile[ll][m];
+        const int jj = line % J.ny, kk = line / J.ny;
+        F.field[(size_t)(F.i0 + c + m) + (size_t)F.n0 * ((size_t)(jj + 1) + (size_t)J.n1 * (size_t)(kk + 1))] = U[m][ll];
       }
     }
     __syncthreads();
+  }
+}
+
+// Whole-line variant: a work-group keeps `lpb` complete lines in LDS (odd
+// row stride, so the lanes walking different lines hit different banks).
+// Phase 1 copies the increments (one contiguous block: the lines of a group
+// are adjacent) into LDS with every lane and several loads in flight,
+// phase 2 runs forward elimination and back substitution on-chip with one
+// lane per line, phase 3 adds the solution to the field rows.  Traffic per
+// DOF: increments once, field read and write once.
+constexpr int kWThreads = 256;
+constexpr int kWUnroll = 4;
+
+__global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
+  extern __shared__ double S[];
+  const LineField & F = J.f[blockIdx.y];
+  const int n = F.n;
+  const int ld = n | 1;
+  const int tid = threadIdx.x;
+  const int line0 = blockIdx.x * lpb;
+  const int nl = min(lpb, J.nlines - line0);
+  if (nl <= 0) return;
+  const int total = nl * n;
+  const size_t g0 = (size_t)line0 * n;
+  // phase 1: element e = tid + kWThreads * r; (ll, m) tracked incrementally
+  {
+    int e = tid, ll = tid / n, m = tid - (tid / n) * n;
+    while (e < total) {
+      double v[kWUnroll];
+      int lls[kWUnroll], ms[kWUnroll], cnt = 0;
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++) {
+        if (e < total) {
+          const size_t g = g0 + e;
+          v[u] = F.srcg ? +J.ca * F.srca[g] + J.cb * F.srcb[g] + J.cg * F.srcg[g] : F.q[g];
+          lls[u] = ll;
+          ms[u] = m;
+          cnt = u + 1;
+          e += kWThreads;
+          m += kWThreads;
+          while (m >= n) { m -= n; ll++; }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++)
+        if (u < cnt) S[lls[u] * ld + ms[u]] = v[u];
+    }
+  }
+  __syncthreads();
+  if (tid < nl) {
+    const double c0 = F.tab[0], rlast = F.tab[1];
+    const bool flag = F.tab[2] != 0.;
+    const double * __restrict__ L = F.tab + 4;
+    const double * __restrict__ VAL = F.tab + 4 + n;
+    const double * __restrict__ V = F.tab + 4 + 2 * n;
+    double * q = S + tid * ld;
+    double qprev = q[0] / c0;
+    q[0] = qprev;
+    int i = 1;
+    for (; i + 4 <= n - 1; i += 4) {
+      double a0 = q[i], a1 = q[i + 1], a2 = q[i + 2], a3 = q[i + 3];
+      a0 = VAL[i] * (a0 - L[i] * qprev);
+      a1 = VAL[i + 1] * (a1 - L[i + 1] * a0);
+      a2 = VAL[i + 2] * (a2 - L[i + 2] * a1);
+      a3 = VAL[i + 3] * (a3 - L[i + 3] * a2);
+      q[i] = a0; q[i + 1] = a1; q[i + 2] = a2; q[i + 3] = a3;
+      qprev = a3;
+    }
+    for (; i < n - 1; i++) {
+      const double a = VAL[i] * (q[i] - L[i] * qprev);
+      q[i] = a;
+      qprev = a;
+    }
+    {
+      const double a = flag ? rlast * (q[n - 1] - L[n - 1] * qprev) : 0.;
+      q[n - 1] = a;
+      qprev = a;
+    }
+    double qnext = qprev;
+    i = n - 2;
+    for (; i - 3 >= 0; i -= 4) {
+      double a0 = q[i], a1 = q[i - 1], a2 = q[i - 2], a3 = q[i - 3];
+      a0 -= V[i] * qnext;
+      a1 -= V[i - 1] * a0;
+      a2 -= V[i - 2] * a1;
+      a3 -= V[i - 3] * a2;
+      q[i] = a0; q[i - 1] = a1; q[i - 2] = a2; q[i - 3] = a3;
+      qnext = a3;
+    }
+    for (; i >= 0; i--) {
+      const double a = q[i] - V[i] * qnext;
+      q[i] = a;
+      qnext = a;
+    }
+  }
+  __syncthreads();
+  // phase 3: field rows += solution
+  {
+    int e = tid, ll = tid / n, m = tid - (tid / n) * n;
+    while (e < total) {
+      double * ptr[kWUnroll];
+      double val[kWUnroll];
+      int cnt = 0;
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++) {
+        if (e < total) {
+          const int line = line0 + ll;
+          const int jj = line % J.ny, kk = line / J.ny;
+          ptr[u] = F.field + (size_t)(F.i0 + m) + (size_t)F.n0 * ((size_t)(jj + 1) + (size_t)J.n1 * (size_t)(kk + 1));
+          val[u] = *ptr[u] + S[ll * ld + m];
+          cnt = u + 1;
+          e += kWThreads;
+          m += kWThreads;
+          while (m >= n) { m -= n; ll++; }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++)
+        if (u < cnt) *ptr[u] = val[u];
+    }
   }
 }
 
@@ -829,8 +1003,95 @@ struct FftPlan {
   int rad[16];
 };
 
-// Stockham autosort FFT of `ncols` LDS columns of length P.n, in place
-// between buffers a and b; returns the buffer holding the result.
+// radix-R DFT of v in place: y_k = sum_j v_j exp(sign 2 pi i j k / R)
+template <int R>
+__device__ inline void butterfly(double2 * v, double sign);
+
+template <>
+__device__ inline void butterfly<2>(double2 * v, double) {
+  const double2 a = v[0], b = v[1];
+  v[0] = make_double2(a.x + b.x, a.y + b.y);
+  v[1] = make_double2(a.x - b.x, a.y - b.y);
+}
+
+template <>
+__device__ inline void butterfly<4>(double2 * v, double sign) {
+  const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+  const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+  const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+  const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+  const double2 jd = make_double2(-sign * d13.y, sign * d13.x);   // (sign i) d13
+  v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+  v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+  v[1] = make_double2(d02.x + jd.x, d02.y + jd.y);
+  v[3] = make_double2(d02.x - jd.x, d02.y - jd.y);
+}
+
+template <>
+__device__ inline void butterfly<3>(double2 * v, double sign) {
+  const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+  const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
+  const double2 t2 = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
+  const double2 m = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+  const double2 r2 = make_double2(-s3 * t2.y, s3 * t2.x);         // s3 i t2
+  v[0] = make_double2(v[0].x + t1.x, v[0].y + t1.y);
+  v[1] = make_double2(m.x + r2.x, m.y + r2.y);
+  v[2] = make_double2(m.x - r2.x, m.y - r2.y);
+}
+
+template <>
+__device__ inline void butterfly<5>(double2 * v, double sign) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
+  const double2 a1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
+  const double2 b1 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
+  const double2 a2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
+  const double2 b2 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
+  const double2 x0 = v[0];
+  v[0] = make_double2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+  const double2 m1 = make_double2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  const double2 m2 = make_double2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  const double2 n1 = make_double2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+  const double2 n2 = make_double2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+  v[1] = make_double2(m1.x + n1.x, m1.y + n1.y);
+  v[4] = make_double2(m1.x - n1.x, m1.y - n1.y);
+  v[2] = make_double2(m2.x + n2.x, m2.y + n2.y);
+  v[3] = make_double2(m2.x - n2.x, m2.y - n2.y);
+}
+
+// one Stockham stage: combine R sub-transforms of length Ns read at stride
+// N/R (Govindaraju et al., SC'08 formulation)
+template <int R>
+__device__ inline void fft_stage(const double2 * a, double2 * b, int ncols, int N, int Ns,
+                                 const double2 * __restrict__ tw, double sign) {
+  const int nb = N / R;
+  const int tstride = N / (Ns * R);
+  for (int t = threadIdx.x; t < ncols * nb; t += blockDim.x) {
+    const int col = t / nb, jj = t - col * nb;
+    const double2 * in = a + (size_t)col * N;
+    double2 * out = b + (size_t)col * N;
+    const int kk = jj % Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const double2 x = in[jj + r * nb];
+      if (r == 0 || kk == 0) {
+        v[r] = x;
+      } else {
+        const double2 w = tw[kk * r * tstride];
+        const double ws = sign * w.y;
+        v[r] = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
+      }
+    }
+    butterfly<R>(v, sign);
+    const int idxD = (jj / Ns) * Ns * R + kk;
+#pragma unroll
+    for (int r = 0; r < R; r++) out[idxD + r * Ns] = v[r];
+  }
+}
+
+// Stockham autosort FFT of `ncols` LDS columns of length P.n between buffers
+// a and b; returns the buffer holding the result.
 // out[k] = sum_j in[j] exp(sign 2 pi i j k / n); tw[m] = (cos, sin)(2 pi m / n)
 __device__ double2 * lds_fft(double2 * a, double2 * b, int ncols, const FftPlan & P,
                              const double2 * __restrict__ tw, double sign) {
@@ -838,69 +1099,11 @@ __device__ double2 * lds_fft(double2 * a, double2 * b, int ncols, const FftPlan 
   int Ns = 1;
   for (int s = 0; s < P.nst; s++) {
     const int R = P.rad[s];
-    const int nb = N / R;
-    const int tstride = N / (Ns * R);
-    for (int t = threadIdx.x; t < ncols * nb; t += blockDim.x) {
-      const int col = t / nb, jj = t - col * nb;
-      const double2 * in = a + (size_t)col * N;
-      double2 * out = b + (size_t)col * N;
-      const int kk = jj % Ns;
-      double2 v[5];
-      for (int r = 0; r < R; r++) {
-        const double2 x = in[jj + r * nb];
-        if (r == 0 || kk == 0) {
-          v[r] = x;
-        } else {
-          const double2 w = tw[(kk * r * tstride) % N];
-          const double ws = sign * w.y;
-          v[r] = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
-        }
-      }
-      double2 y[5];
-      if (R == 2) {
-        y[0] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
-        y[1] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
-      } else if (R == 4) {
-        const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-        const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-        const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-        const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
-        // (sign i) * d13
-        const double2 jd = make_double2(-sign * d13.y, sign * d13.x);
-        y[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-        y[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-        y[1] = make_double2(d02.x + jd.x, d02.y + jd.y);
-        y[3] = make_double2(d02.x - jd.x, d02.y - jd.y);
-      } else if (R == 3) {
-        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
-        const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
-        const double2 t2 = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
-        const double2 m = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
-        // s3 * i * t2
-        const double2 r2 = make_double2(-s3 * t2.y, s3 * t2.x);
-        y[0] = make_double2(v[0].x + t1.x, v[0].y + t1.y);
-        y[1] = make_double2(m.x + r2.x, m.y + r2.y);
-        y[2] = make_double2(m.x - r2.x, m.y - r2.y);
-      } else {  // R == 5
-        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-        const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
-        const double2 a1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
-        const double2 b1 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
-        const double2 a2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
-        const double2 b2 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
-        y[0] = make_double2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
-        const double2 m1 = make_double2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
-        const double2 m2 = make_double2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
-        // i * (s1 b1 + s2 b2) and i * (s2 b1 - s1 b2)
-        const double2 n1 = make_double2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
-        const double2 n2 = make_double2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
-        y[1] = make_double2(m1.x + n1.x, m1.y + n1.y);
-        y[4] = make_double2(m1.x - n1.x, m1.y - n1.y);
-        y[2] = make_double2(m2.x + n2.x, m2.y + n2.y);
-        y[3] = make_double2(m2.x - n2.x, m2.y - n2.y);
-      }
-      const int idxD = (jj / Ns) * Ns * R + kk;
-      for (int r = 0; r < R; r++) out[idxD + r * Ns] = y[r];
+    switch (R) {
+      case 4: fft_stage<4>(a, b, ncols, N, Ns, tw, sign); break;
+      case 2: fft_stage<2>(a, b, ncols, N, Ns, tw, sign); break;
+      case 3: fft_stage<3>(a, b, ncols, N, Ns, tw, sign); break;
+      default: fft_stage<5>(a, b, ncols, N, Ns, tw, sign); break;
     }
     __syncthreads();
     double2 * tmp = a; a = b; b = tmp;

@@ -719,7 +719,7 @@ static int ctx_create(domain_t * domain) {
   const size_t nline = (size_t)ctx->nh * nz;
   HIPCHK(hipMalloc(&ctx->d_spec, sizeof(double2) * nx * nline));
   HIPCHK(hipMalloc(&ctx->d_vbuf, sizeof(double) * nx * nline));
-  HIPCHK(hipMalloc(&ctx->d_delta, sizeof(double) * ncell));
+  HIPCHK(hipMalloc(&ctx->d_delta, sizeof(double) * ncell * (NDIMS + 1)));
   ctx->npart = 1024;
   HIPCHK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->npart * vofk::kDiagN));
   HIPCHK(hipHostMalloc(&ctx->h_scalar, sizeof(double) * 64));
@@ -1049,13 +1049,16 @@ static void swap_data(array_t * a, array_t * b) {
   b->data = tmp;
 }
 
-// fluid_predict_field (predict/main.c:62-79): the three sources of each
-// field, increments, implicit x solve and update (ux.c:306-388 etc.)
-extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep, const double dt,
-                                   fluid_t * fluid, const interface_t * interface) {
+// fluid_predict_field (predict/main.c:62-79): the sources of every field
+// from the old fields (compute_rhs, :31-40), then increments, the implicit x
+// solve and the update of every field (predict, :42-53; ux.c:306-388 etc.).
+// fused = 1: the source kernels write the increment directly (srcg is not
+// stored) and the line solver reads it.
+static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid,
+                        const interface_t * interface, int fused) {
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
-  // reset_srcs (predict/main.c:9-29): the a <-> b swap; a and g are fully
+  // reset_srcs (predict/main.c:9-29): the a <-> b swap; a (and g) are fully
   // rewritten by the source kernels
   if (rkstep != 0) {
     swap_data(&fluid->srcux[0], &fluid->srcux[1]);
@@ -1069,6 +1072,7 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
   const double pm = 0.5 * kRk[rkstep][2] * dt * fluid->m_dif;
   const double pt = 0.5 * kRk[rkstep][2] * dt * fluid->t_dif;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  const size_t ncell = (size_t)nx * ny * nz;
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
   const double * ifrcz = (const double *)interface->ifrcz.data;
@@ -1082,6 +1086,7 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
   base.p = (const double *)fluid->p.data;
   base.t = (const double *)fluid->t.data;
   base.add_buoyancy = ctx->prm[P_BUOY] != 0.;
+  base.ca = ca; base.cb = cb; base.cg = cg;
   struct FieldJob {
     const char * name;
     void (*kern)(Grid, vofk::RhsArgs);
@@ -1094,50 +1099,82 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
   };
   FieldJob jobs[4];
   int nj = 0;
-  jobs[nj++] = {"k_rhs_ux", vofk::k_rhs_ux, fluid->srcux, &fluid->ux, (const double *)interface->ifrcx.data,
-                ctx->d_lapx_ux, fluid->m_dif, 0};
-  jobs[nj++] = {"k_rhs_uy", vofk::k_rhs_uy, fluid->srcuy, &fluid->uy, (const double *)interface->ifrcy.data,
-                ctx->d_lapx_c, fluid->m_dif, 1};
+  jobs[nj++] = {"k_rhs_ux", vofk::k_rhs_ux, fluid->srcux, &fluid->ux,
+                (const double *)interface->ifrcx.data, ctx->d_lapx_ux, fluid->m_dif, 0};
+  jobs[nj++] = {"k_rhs_uy", vofk::k_rhs_uy, fluid->srcuy, &fluid->uy,
+                (const double *)interface->ifrcy.data, ctx->d_lapx_c, fluid->m_dif, 1};
 #if NDIMS == 3
-  jobs[nj++] = {"k_rhs_uz", vofk::k_rhs_uz, fluid->srcuz, &fluid->uz, ifrcz, ctx->d_lapx_c, fluid->m_dif, 2};
+  jobs[nj++] = {"k_rhs_uz", vofk::k_rhs_uz, fluid->srcuz, &fluid->uz, ifrcz, ctx->d_lapx_c,
+                fluid->m_dif, 2};
 #endif
-  jobs[nj++] = {"k_rhs_t", vofk::k_rhs_t, fluid->srct, &fluid->t, nullptr, ctx->d_lapx_c, fluid->t_dif, 3};
-  // compute_rhs (predict/main.c:31-40): every source from the old fields
+  jobs[nj++] = {"k_rhs_t", vofk::k_rhs_t, fluid->srct, &fluid->t, nullptr, ctx->d_lapx_c,
+                fluid->t_dif, 3};
+  vofk::LineJob lj{};
+  lj.nlines = ny * nz;
+  lj.ny = ny;
+  lj.n1 = NDIMS == 3 ? ny + 2 : 0;
+  lj.ca = ca; lj.cb = cb; lj.cg = cg;
   for (int f = 0; f < nj; f++) {
+    const int kind = jobs[f].kind;
     vofk::RhsArgs a = base;
     a.ifrc = jobs[f].ifrc;
     a.lapx = jobs[f].lapx;
     a.srca = (double *)jobs[f].src[0].data;
-    a.srcg = (double *)jobs[f].src[2].data;
     a.dif = jobs[f].dif;
-    const int ilo_n = jobs[f].kind == 0 ? nx - 1 : nx;
-    KLN(ctx, jobs[f].name, jobs[f].kern, grid3(ilo_n, ny, nz, kBlk), kBlk, ctx->grid, a);
-    LAUNCHCHK();
-  }
-  // predict (predict/main.c:42-53): increments, x solve, update, boundaries
-  for (int f = 0; f < nj; f++) {
-    const int kind = jobs[f].kind;
-    const double pref = kind == 3 ? pt : pm;
+    double * slab = ctx->d_delta + (size_t)f * ncell;
+    if (fused) {
+      a.delta = slab;
+      a.srcb = (const double *)jobs[f].src[1].data;
+      a.srcg = nullptr;
+    } else {
+      a.delta = nullptr;
+      a.srcb = nullptr;
+      a.srcg = (double *)jobs[f].src[2].data;
+    }
+    const int ni = kind == 0 ? nx - 1 : nx;
+    KLN(ctx, jobs[f].name, jobs[f].kern, grid3(ni, ny, nz, kBlk), kBlk, ctx->grid, a);
     const double * dtab = nullptr;
-    if (upload_tab(ctx, kind, (int)rkstep, pref, &dtab)) return 1;
-    const int n = kind == 0 ? nx - 1 : nx;
-    const int n0 = kind == 0 ? nx + 1 : nx + 2;
-    double * fld = (double *)jobs[f].field->data;
-    const int nlines = ny * nz;
-    const int nb = (nlines + vofk::kTLines - 1) / vofk::kTLines;
-    KLN(ctx, "k_thomas_lines", vofk::k_thomas_lines, dim3(nb), dim3(vofk::kTLines), dtab,
-                       (const double *)jobs[f].src[0].data, (const double *)jobs[f].src[1].data,
-                       (const double *)jobs[f].src[2].data, ca, cb, cg, ctx->d_delta, fld, n,
-                       nlines, 1, n0, NDIMS == 3 ? ny + 2 : 0, 1, ny);
+    if (upload_tab(ctx, kind, (int)rkstep, kind == 3 ? pt : pm, &dtab)) return 1;
+    vofk::LineField & F = lj.f[f];
+    F.tab = dtab;
+    F.srca = (const double *)jobs[f].src[0].data;
+    F.srcb = (const double *)jobs[f].src[1].data;
+    F.srcg = fused ? nullptr : (const double *)jobs[f].src[2].data;
+    F.q = slab;
+    F.field = (double *)jobs[f].field->data;
+    F.n = ni;
+    F.n0 = kind == 0 ? nx + 1 : nx + 2;
+    F.i0 = 1;
+  }
+  {
+    // lines per work-group so that the LDS tile stays within 32 KiB
+    int lpb = (int)(32768 / (sizeof(double) * (nx | 1)));
+    if (lpb > 32) lpb = 32;
+    if (lpb < 1) lpb = 1;
+    const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
+    const dim3 lg((unsigned)((lj.nlines + lpb - 1) / lpb), (unsigned)nj);
+    hipEvent_t e0_ = prof_begin(ctx);
+    hipLaunchKernelGGL(vofk::k_lines_lds, lg, dim3(vofk::kWThreads), shm, ctx->stream, lj, lpb);
     LAUNCHCHK();
+    if (e0_ && prof_end(ctx, "k_lines_lds", e0_)) return 1;
+  }
+  for (int f = 0; f < nj; f++) {
+    double * fld = (double *)jobs[f].field->data;
     int rc = 0;
-    if (kind == 0) rc = bc_ux(ctx, fld);
-    else if (kind == 1) rc = bc_uy(ctx, fld);
-    else if (kind == 2) rc = bc_uz(ctx, fld);
-    else rc = bc_t(ctx, fld);
+    switch (jobs[f].kind) {
+      case 0: rc = bc_ux(ctx, fld); break;
+      case 1: rc = bc_uy(ctx, fld); break;
+      case 2: rc = bc_uz(ctx, fld); break;
+      default: rc = bc_t(ctx, fld); break;
+    }
     if (rc) return 1;
   }
   return 0;
+}
+
+extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep, const double dt,
+                                   fluid_t * fluid, const interface_t * interface) {
+  return predict_impl(domain, rkstep, dt, fluid, interface, 0);
 }
 
 // Poisson: both reference solvers are the same discrete problem; the FFT
@@ -1359,7 +1396,7 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
     if (interface_compute_curvature_tensor(domain, interface)) return 1;
     if (interface_compute_force(domain, interface)) return 1;
     if (interface_update_vof(domain, rk, *dt, fluid, interface)) return 1;
-    if (fluid_predict_field(domain, rk, *dt, fluid, interface)) return 1;
+    if (predict_impl(domain, rk, *dt, fluid, interface, ctx->schedule == 1)) return 1;
     if (uniform) {
       if (fluid_compute_potential_dct(domain, rk, *dt, fluid)) return 1;
     } else {
@@ -1368,7 +1405,6 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
     if (fluid_correct_velocity(domain, rk, *dt, fluid)) return 1;
     if (fluid_update_pressure(domain, rk, *dt, fluid)) return 1;
   }
-  (void)ctx;
   return 0;
 }
 
