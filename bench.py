@@ -58,6 +58,15 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         "k_correct": c * (8 + 24 + 24),
         "k_pressure": c * (8 + 8 + 8),
         "k_dt_partial": c * 24,
+        # fused schedule: vof read; normal (3) and curvature written
+        "k_iface_fused": c * (8 + 24 + 8),
+        # velocity (3), vof, normal (3), srcb read; srca and the new vof written
+        "k_vof_fused": c * (24 + 8 + 24 + 8 + 8 + 8),
+        # ux uy uz p t, vof, curvature, srcb of 4 fields read; srca and the
+        # increment of 4 fields written
+        "k_rhs_fused": c * (40 + 8 + 8 + 32 + 32 + 32),
+        # psi, ux uy uz p read and written back
+        "k_corr_press": c * (8 + 32 + 32),
     }
     return table.get(kernel)
 
@@ -103,6 +112,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", type=int, default=None)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
+                    help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -119,6 +130,9 @@ def main(argv=None) -> int:
     print(f"[bench] IC {n}^3 generated in {t_ic:.1f} s", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
         g = sva.solver_from_ic(ic, d, schedule=args.schedule)
+    for t in args.tuning:
+        name, val = t.split("=")
+        g.set_tuning(name, int(val))
     # the library holds everything on the device from here on
     d0 = g.diagnostics()
     for _ in range(args.warmup):

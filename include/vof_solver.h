@@ -243,14 +243,31 @@ int fluid_finalise(const domain_t * domain, fluid_t * fluid);
 int domain_finalise(domain_t * domain);
 /* select the per-step kernel schedule of integrate():
  * 0 = one launch sequence per reference function (the per-function entry
- *     points), 1 = fused kernels (default) */
+ *     points), 1 = fused kernels (default).  Both give the same fields; the
+ *     fused schedule does not refresh the intermediate arrays dvof, flxx,
+ *     flxy, flxz, ifrcx, ifrcy, ifrcz and srcg. */
 int vof_set_schedule(domain_t * domain, int schedule);
+/* stages fused by schedule 1 (bit mask, default 63 = all): 1 curvature
+ * tensor, 2 VOF update, 4 sources with the surface force, 8 boundary images
+ * of the predictor, 16 boundary images of the potential, 32 correction with
+ * pressure */
+int vof_set_fusion(domain_t * domain, int mask);
 /* device diagnostics (logging/divergence.c:37-55, logging/interface.c:29-51,
- * logging/energy.c:43-65): out = {max|div u|, vof min, vof max,
- * vof volume fraction, kinetic energy x, y[, z], thermal energy} */
+ * logging/energy.c:43-65, logging/momentum.c:39-53): out = {max|div u|,
+ * vof min, vof max, vof volume fraction, kinetic energy x, y[, z], thermal
+ * energy, momentum x, y[, z], interface measure sum |grad vof| dV}
+ * (at most 12 values) */
 int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
                     const interface_t * interface, double * out);
 
+/* tile parameters (tuned by simplevofsolver_amd.tune): "fft_elems" = 1024 or
+ * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
+ * per work-group of the predictor's x solver (0 = automatic) */
+int vof_set_tuning(domain_t * domain, const char name[], int value);
+/* step.npy / time.npy of an IC or save directory (the reference's main.c
+ * reads and writes them with fileio.r_serial / w_serial) */
+int vof_read_step_time(const char dirname[], size_t * step, double * time);
+int vof_write_step_time(const char dirname[], size_t step, double time);
 /* per-kernel timing with HIP events recorded on the context's stream around
  * every launch (on = 1 starts, on = 0 stops); the report lists
  * "kernel launches total_ms" lines */

@@ -1808,3 +1808,60 @@ void orc_energy(const orc_t * o, double * out) {
 #undef CELLSIZE
   for (int d = 0; d <= NDIMS; d++) out[d] = q[d];
 }
+
+/* logging/momentum.c:39-53: x momentum on the x faces i = 2..nx (cell size
+ * DXC), y (and z) momentum on the cell centres (cell size DXF) */
+void orc_momentum(const orc_t * o, double * out) {
+  SIZES;
+  const double * dxf = o->dxf, * dxc = o->dxc;
+  const double dy = o->dy, dz = o->dz;
+  (void)dz;
+  const double * ux = o->ux, * uy = o->uy;
+  double q[3] = {0., 0., 0.};
+#if NDIMS == 2
+#define CELLSIZE(DX) ((DX) * dy)
+#else
+#define CELLSIZE(DX) ((DX) * dy * dz)
+#endif
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 2; i <= isize; i++) {
+    q[0] += UX(i, j, k) * CELLSIZE(DXC(i));
+  }
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+    q[1] += UY(i, j, k) * CELLSIZE(DXF(i));
+  }
+#if NDIMS == 3
+  const double * uz = o->uz;
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+    q[2] += UZ(i, j, k) * CELLSIZE(DXF(i));
+  }
+#endif
+#undef CELLSIZE
+  for (int d = 0; d < NDIMS; d++) out[d] = q[d];
+}
+
+/* interface measure sum |grad vof| dV (length in 2D, area in 3D) with
+ * central differences at the cell centres; the wall values of vof (0) are
+ * the outer neighbours at i = 1 and nx.  For a planar interface normal to
+ * x it returns the wall area exactly (the three non-zero differences
+ * telescope to 1/(2 dx) each side). */
+double orc_interface_area(const orc_t * o) {
+  SIZES;
+  const double * dxf = o->dxf, * dxc = o->dxc;
+  const double dy = o->dy, dz = o->dz;
+  (void)dz;
+  const double * vof = o->vof;
+  double s = 0.;
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+    const double gx = (VOF(i + 1, j, k) - VOF(i - 1, j, k)) / (DXC(i) + DXC(i + 1));
+    const double gy = (VOF(i, j + 1, k) - VOF(i, j - 1, k)) / (2. * dy);
+#if NDIMS == 2
+    const double g2 = gx * gx + gy * gy;
+    s += sqrt(g2) * (DXF(i) * dy);
+#else
+    const double gz = (VOF(i, j, k + 1) - VOF(i, j, k - 1)) / (2. * dz);
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    s += sqrt(g2) * (DXF(i) * dy * dz);
+#endif
+  }
+  return s;
+}

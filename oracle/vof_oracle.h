@@ -84,5 +84,7 @@ double orc_indicator(const double * normal_with_d, const double * x);
 double orc_max_divergence(const orc_t * o);
 void orc_vof_stats(const orc_t * o, double * vmin, double * vmax, double * volume);
 void orc_energy(const orc_t * o, double * out /* NDIMS + 1 */);
+void orc_momentum(const orc_t * o, double * out /* NDIMS */);
+double orc_interface_area(const orc_t * o);
 
 #endif

@@ -74,7 +74,8 @@ ABI_SYMBOLS_COMMON = [
     "vof_last_error", "vof_set_param", "vof_get_param", "vof_array_to_host",
     "vof_array_from_host", "vof_synchronize", "interface_finalise", "fluid_finalise",
     "domain_finalise", "vof_set_schedule", "vof_diagnostics", "vof_profile",
-    "vof_profile_reset", "vof_profile_report", "vof_copy_probe",
+    "vof_profile_reset", "vof_profile_report", "vof_copy_probe", "vof_set_fusion",
+    "vof_read_step_time", "vof_write_step_time", "vof_set_tuning",
 ]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
 
@@ -127,6 +128,10 @@ def load(ndims: int):
     lib.vof_profile_reset.argtypes = [vp]
     lib.vof_profile_report.argtypes = [vp, C.c_char_p, C.c_size_t]
     lib.vof_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, dp]
+    lib.vof_set_fusion.argtypes = [vp, C.c_int]
+    lib.vof_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
+    lib.vof_read_step_time.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), dp]
+    lib.vof_write_step_time.argtypes = [C.c_char_p, C.c_size_t, C.c_double]
     lib._structs = (Domain, Fluid, Interface)
     _LIBS[ndims] = lib
     return lib
@@ -252,18 +257,26 @@ class Solver:
         keys = ["max_div", "vof_min", "vof_max", "vof_volume", "ke_x", "ke_y"]
         if self.ndims == 3:
             keys.append("ke_z")
-        keys.append("thermal")
+        keys += ["thermal", "mom_x", "mom_y"]
+        if self.ndims == 3:
+            keys.append("mom_z")
+        keys.append("interface_area")
         return {k: out[i] for i, k in enumerate(keys)}
 
     def save(self, dirname: str, step: int = 0, time: float = 0.0):
         """save_entrypoint (src/main.c:17-32)."""
         os.makedirs(dirname, exist_ok=True)
-        np.save(os.path.join(dirname, "step.npy"), np.array(step, dtype=np.uint64))
-        np.save(os.path.join(dirname, "time.npy"), np.array(time, dtype=np.float64))
+        self._chk(self.lib.vof_write_step_time(dirname.encode(), step, time), "vof_write_step_time")
         self._chk(self.lib.domain_save(dirname.encode(), C.byref(self.domain)), "domain_save")
         self._chk(self.lib.fluid_save(dirname.encode(), C.byref(self.domain), C.byref(self.fluid)), "fluid_save")
         self._chk(self.lib.interface_save(dirname.encode(), C.byref(self.domain), C.byref(self.interface)),
                   "interface_save")
+
+    def set_fusion(self, mask: int):
+        self._chk(self.lib.vof_set_fusion(C.byref(self.domain), int(mask)), "vof_set_fusion")
+
+    def set_tuning(self, name: str, value: int):
+        self._chk(self.lib.vof_set_tuning(C.byref(self.domain), name.encode(), int(value)), "vof_set_tuning")
 
     def profile(self, on: bool = True):
         self._chk(self.lib.vof_profile(C.byref(self.domain), int(on)), "vof_profile")

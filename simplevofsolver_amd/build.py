@@ -47,8 +47,28 @@ def compile_cmd(ndims: int, out: str) -> list[str]:
     ]
 
 
+def driver_path(ndims: int) -> str:
+    return os.path.join(PKG, f"vofdrive{ndims}d")
+
+
+def build_driver(ndims: int, verbose: bool = False) -> str:
+    """The step-count driver (csrc/vof_driver.c), a plain C99 program linked
+    against the library of the same dimensionality."""
+    out = driver_path(ndims)
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", f"-DNDIMS={ndims}", "-D_POSIX_C_SOURCE=200809L",
+           "-I", INCLUDE, os.path.join(CSRC, "vof_driver.c"), "-L", PKG, f"-lvofhip{ndims}d",
+           f"-Wl,-rpath,$ORIGIN", "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"driver build failed for NDIMS={ndims}")
+    return out
+
+
 def build(ndims_list=(2, 3), verbose: bool = False) -> list[str]:
-    """Compile every library (always, also where one is already built)."""
+    """Compile every library and driver (always, also where built)."""
     outs = []
     for nd in ndims_list:
         out = lib_path(nd)
@@ -62,6 +82,7 @@ def build(ndims_list=(2, 3), verbose: bool = False) -> list[str]:
             raise RuntimeError(f"hipcc failed for NDIMS={nd}")
         os.replace(tmp, out)
         outs.append(out)
+        outs.append(build_driver(nd, verbose))
     return outs
 
 

@@ -346,9 +346,93 @@ __device__ inline double indicator(const double * n, const double * x) {
 #endif
 }
 
-// compute_flux_x (flxx.c:11-44), compute_flux_y (flxy.c:11-44) and the z
-// counterpart; thread (i, j, k) over 1..nx x 1..ny+1 x 1..nz+1 owns
-// FLXX(i), FLXY(j), FLXZ(k) where those exist
+// compute_flux_x (flxx.c:11-44): THINC flux through the x face i (2..nx)
+__device__ inline double flux_x_at(const double * __restrict__ ux, const double * __restrict__ vof,
+                                   const double * __restrict__ normal, int nx, int ny, int i, int j,
+                                   int k) {
+  constexpr int NC = NDIMS + 1;
+  const double vel = ux[I_UX(i, j, k)];
+  const int ii = vel < 0. ? i : i - 1;
+  const double x = vel < 0. ? -0.5 : +0.5;
+  const double lvof = vof[I_VOF(ii, j, k)];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
+  const double * n = normal + NC * I_NRM(ii, j, k);
+  double flux = 0.;
+#if NDIMS == 2
+  for (int jj = 0; jj < NGAUSS; jj++) {
+    const double pt[2] = {x, kGaussP[jj]};
+    flux += kGaussW[jj] * indicator(n, pt);
+  }
+#else
+  for (int jj = 0; jj < NGAUSS; jj++) {
+    double sz = 0.;
+    for (int kk = 0; kk < NGAUSS; kk++) {
+      const double pt[3] = {x, kGaussP[jj], kGaussP[kk]};
+      sz += kGaussW[kk] * indicator(n, pt);
+    }
+    flux += kGaussW[jj] * sz;
+  }
+#endif
+  return vel * flux;
+}
+
+// compute_flux_y (flxy.c:11-44): face j (1..ny+1)
+__device__ inline double flux_y_at(const double * __restrict__ uy, const double * __restrict__ vof,
+                                   const double * __restrict__ normal, int nx, int ny, int i, int j,
+                                   int k) {
+  constexpr int NC = NDIMS + 1;
+  const double vel = uy[I_C1(i, j, k)];
+  const int jj0 = vel < 0. ? j : j - 1;
+  const double y = vel < 0. ? -0.5 : +0.5;
+  const double lvof = vof[I_VOF(i, jj0, k)];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
+  const double * n = normal + NC * I_NRM(i, jj0, k);
+  double flux = 0.;
+#if NDIMS == 2
+  for (int ii = 0; ii < NGAUSS; ii++) {
+    const double pt[2] = {kGaussP[ii], y};
+    flux += kGaussW[ii] * indicator(n, pt);
+  }
+#else
+  for (int ii = 0; ii < NGAUSS; ii++) {
+    double sz = 0.;
+    for (int kk = 0; kk < NGAUSS; kk++) {
+      const double pt[3] = {kGaussP[ii], y, kGaussP[kk]};
+      sz += kGaussW[kk] * indicator(n, pt);
+    }
+    flux += kGaussW[ii] * sz;
+  }
+#endif
+  return vel * flux;
+}
+
+#if NDIMS == 3
+// the z counterpart: face k (1..nz+1)
+__device__ inline double flux_z_at(const double * __restrict__ uz, const double * __restrict__ vof,
+                                   const double * __restrict__ normal, int nx, int ny, int i, int j,
+                                   int k) {
+  constexpr int NC = NDIMS + 1;
+  const double vel = uz[I_C1(i, j, k)];
+  const int kk0 = vel < 0. ? k : k - 1;
+  const double z = vel < 0. ? -0.5 : +0.5;
+  const double lvof = vof[I_VOF(i, j, kk0)];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
+  const double * n = normal + NC * I_NRM(i, j, kk0);
+  double flux = 0.;
+  for (int ii = 0; ii < NGAUSS; ii++) {
+    double sy = 0.;
+    for (int jj = 0; jj < NGAUSS; jj++) {
+      const double pt[3] = {kGaussP[ii], kGaussP[jj], z};
+      sy += kGaussW[jj] * indicator(n, pt);
+    }
+    flux += kGaussW[ii] * sy;
+  }
+  return vel * flux;
+}
+#endif
+
+// thread (i, j, k) over 1..nx x 1..ny+1 x 1..nz+1 owns FLXX(i), FLXY(j),
+// FLXZ(k) where those exist
 __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                        const double * __restrict__ uz, const double * __restrict__ vof,
                        const double * __restrict__ normal, double * __restrict__ flxx,
@@ -364,91 +448,10 @@ __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __
   const bool kin = true;
   (void)uz; (void)flxz;
 #endif
-  constexpr int NC = NDIMS + 1;
-  if (i >= 2 && j <= ny && kin) {
-    const double vel = ux[I_UX(i, j, k)];
-    const int ii = vel < 0. ? i : i - 1;
-    const double x = vel < 0. ? -0.5 : +0.5;
-    const double lvof = vof[I_VOF(ii, j, k)];
-    double out;
-    if (lvof < kVofMin || 1. - kVofMin < lvof) {
-      out = vel * lvof;
-    } else {
-      const double * n = normal + NC * I_NRM(ii, j, k);
-      double flux = 0.;
-#if NDIMS == 2
-      for (int jj = 0; jj < NGAUSS; jj++) {
-        const double pt[2] = {x, kGaussP[jj]};
-        flux += kGaussW[jj] * indicator(n, pt);
-      }
-#else
-      for (int jj = 0; jj < NGAUSS; jj++) {
-        double sz = 0.;
-        for (int kk = 0; kk < NGAUSS; kk++) {
-          const double pt[3] = {x, kGaussP[jj], kGaussP[kk]};
-          sz += kGaussW[kk] * indicator(n, pt);
-        }
-        flux += kGaussW[jj] * sz;
-      }
-#endif
-      out = vel * flux;
-    }
-    flxx[I_FX(i, j, k)] = out;
-  }
-  if (kin) {
-    const double vel = uy[I_C1(i, j, k)];
-    const int jj0 = vel < 0. ? j : j - 1;
-    const double y = vel < 0. ? -0.5 : +0.5;
-    const double lvof = vof[I_VOF(i, jj0, k)];
-    double out;
-    if (lvof < kVofMin || 1. - kVofMin < lvof) {
-      out = vel * lvof;
-    } else {
-      const double * n = normal + NC * I_NRM(i, jj0, k);
-      double flux = 0.;
-#if NDIMS == 2
-      for (int ii = 0; ii < NGAUSS; ii++) {
-        const double pt[2] = {kGaussP[ii], y};
-        flux += kGaussW[ii] * indicator(n, pt);
-      }
-#else
-      for (int ii = 0; ii < NGAUSS; ii++) {
-        double sz = 0.;
-        for (int kk = 0; kk < NGAUSS; kk++) {
-          const double pt[3] = {kGaussP[ii], y, kGaussP[kk]};
-          sz += kGaussW[kk] * indicator(n, pt);
-        }
-        flux += kGaussW[ii] * sz;
-      }
-#endif
-      out = vel * flux;
-    }
-    flxy[I_FY(i, j, k)] = out;
-  }
+  if (i >= 2 && j <= ny && kin) flxx[I_FX(i, j, k)] = flux_x_at(ux, vof, normal, nx, ny, i, j, k);
+  if (kin) flxy[I_FY(i, j, k)] = flux_y_at(uy, vof, normal, nx, ny, i, j, k);
 #if NDIMS == 3
-  if (j <= ny) {
-    const double vel = uz[I_C1(i, j, k)];
-    const int kk0 = vel < 0. ? k : k - 1;
-    const double z = vel < 0. ? -0.5 : +0.5;
-    const double lvof = vof[I_VOF(i, j, kk0)];
-    double out;
-    if (lvof < kVofMin || 1. - kVofMin < lvof) {
-      out = vel * lvof;
-    } else {
-      const double * n = normal + NC * I_NRM(i, j, kk0);
-      double flux = 0.;
-      for (int ii = 0; ii < NGAUSS; ii++) {
-        double sy = 0.;
-        for (int jj = 0; jj < NGAUSS; jj++) {
-          const double pt[3] = {kGaussP[ii], kGaussP[jj], z};
-          sy += kGaussW[jj] * indicator(n, pt);
-        }
-        flux += kGaussW[ii] * sy;
-      }
-      out = vel * flux;
-    }
-    flxz[I_FZ(i, j, k)] = out;
-  }
+  if (j <= ny) flxz[I_FZ(i, j, k)] = flux_z_at(uz, vof, normal, nx, ny, i, j, k);
 #endif
 }
 
@@ -512,10 +515,10 @@ __device__ inline void rhs_store(const RhsArgs & a, size_t is, double sa, double
 }
 
 // compute_rhs_ux (ux.c:71-249), DOFs i = 2..nx
-__global__ void k_rhs_ux(Grid g, RhsArgs a) {
+__device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                               double ifrc, double & sa_out, double & sg_out) {
   const int nx = g.nx, ny = g.ny;
-  THREAD_IJK(2, 1, 1)
-  if (i > nx || j > ny) return;
+  (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double dxci = g.dxc[i - 1];
@@ -569,16 +572,25 @@ __global__ void k_rhs_ux(Grid g, RhsArgs a) {
 #endif
   sg -= 1. / dxci * (-a.p[I_C1(i - 1, j, k)] + a.p[I_C1(i, j, k)]);
   if (a.add_buoyancy) sa += +0.5 * a.t[I_C1(i - 1, j, k)] + 0.5 * a.t[I_C1(i, j, k)];
-  const size_t is = I_SX(i, j, k);
-  sa += a.ifrc[is];
-  rhs_store(a, is, sa, sg);
+  sa += ifrc;
+  sa_out = sa;
+  sg_out = sg;
+}
+
+__global__ void k_rhs_ux(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(2, 1, 1)
+  if (i > nx || j > ny) return;
+  double sa, sg;
+  rhs_ux_at(g, a, i, j, k, a.ifrc[I_SX(i, j, k)], sa, sg);
+  rhs_store(a, I_SX(i, j, k), sa, sg);
 }
 
 // compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx
-__global__ void k_rhs_uy(Grid g, RhsArgs a) {
+__device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                               double ifrc, double & sa_out, double & sg_out) {
   const int nx = g.nx, ny = g.ny;
-  THREAD_IJK(1, 1, 1)
-  if (i > nx || j > ny) return;
+  (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double dxfi = g.dxf[i - 1];
@@ -629,17 +641,26 @@ __global__ void k_rhs_uy(Grid g, RhsArgs a) {
   }
 #endif
   sg -= 1. / g.dy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
-  const size_t is = I_SC(i, j, k);
-  sa += a.ifrc[is];
-  rhs_store(a, is, sa, sg);
+  sa += ifrc;
+  sa_out = sa;
+  sg_out = sg;
+}
+
+__global__ void k_rhs_uy(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  double sa, sg;
+  rhs_uy_at(g, a, i, j, k, a.ifrc[I_SC(i, j, k)], sa, sg);
+  rhs_store(a, I_SC(i, j, k), sa, sg);
 }
 
 #if NDIMS == 3
 // compute_rhs_uz: the z counterpart of compute_rhs_uy, DOFs i = 1..nx
-__global__ void k_rhs_uz(Grid g, RhsArgs a) {
+__device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                               double ifrc, double & sa_out, double & sg_out) {
   const int nx = g.nx, ny = g.ny;
-  THREAD_IJK(1, 1, 1)
-  if (i > nx || j > ny) return;
+  (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double * uz = a.uz;
@@ -686,17 +707,26 @@ __global__ void k_rhs_uz(Grid g, RhsArgs a) {
     sa += a.dif * (lz * w_zm + cz * w_c + lz * w_zp);
   }
   sg -= 1. / g.dz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
-  const size_t is = I_SC(i, j, k);
-  sa += a.ifrc[is];
-  rhs_store(a, is, sa, sg);
+  sa += ifrc;
+  sa_out = sa;
+  sg_out = sg;
+}
+
+__global__ void k_rhs_uz(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  double sa, sg;
+  rhs_uz_at(g, a, i, j, k, a.ifrc[I_SC(i, j, k)], sa, sg);
+  rhs_store(a, I_SC(i, j, k), sa, sg);
 }
 #endif
 
 // compute_rhs_t (t.c:68-182), DOFs i = 1..nx
-__global__ void k_rhs_t(Grid g, RhsArgs a) {
+__device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                               double ifrc, double & sa_out, double & sg_out) {
   const int nx = g.nx, ny = g.ny;
-  THREAD_IJK(1, 1, 1)
-  if (i > nx || j > ny) return;
+  (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double * t = a.t;
@@ -741,8 +771,18 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
     sa += a.dif * (lz * t_zm + cz * t_c + lz * t_zp);
   }
 #endif
-  const size_t is = I_SC(i, j, k);
-  rhs_store(a, is, sa, sg);
+  (void)ifrc;
+  sa_out = sa;
+  sg_out = sg;
+}
+
+__global__ void k_rhs_t(Grid g, RhsArgs a) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  double sa, sg;
+  rhs_t_at(g, a, i, j, k, 0., sa, sg);
+  rhs_store(a, I_SC(i, j, k), sa, sg);
 }
 
 // ---------------------------------------------------------------------------
@@ -752,6 +792,72 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
 // The tri-diagonal matrix (I - prefactor Lx) is the same for every line, so
 // the elimination factors the reference recomputes per line are prepared
 // once on the host with the same arithmetic.
+// Store v at padded position (x, row r, plane p) and at every periodic image
+// of it inside the halo (h rows / planes on each side; r, p are storage
+// indices of an interior entry).  In 2D the plane index is ignored.
+__device__ inline void store_periodic(double * __restrict__ a, int x, int n0, int n1, int r, int p,
+                                      int ny, int nz, int h, double v) {
+  int rs[3];
+  int nr = 0;
+  rs[nr++] = r;
+  if (r < 2 * h) rs[nr++] = r + ny;
+  if (r >= ny) rs[nr++] = r - ny;
+#if NDIMS == 3
+  int ps[3];
+  int np = 0;
+  ps[np++] = p;
+  if (p < 2 * h) ps[np++] = p + nz;
+  if (p >= nz) ps[np++] = p - nz;
+  for (int b = 0; b < np; b++)
+    for (int c = 0; c < nr; c++) a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])] = v;
+#else
+  (void)p; (void)nz; (void)n1;
+  for (int c = 0; c < nr; c++) a[(size_t)x + (size_t)n0 * (size_t)rs[c]] = v;
+#endif
+}
+
+// the images of (x, r, p) only (not the entry itself)
+__device__ inline void store_images(double * __restrict__ a, int x, int n0, int n1, int r, int p,
+                                    int ny, int nz, int h, double v) {
+  int rs[3];
+  int nr = 0;
+  rs[nr++] = r;
+  if (r < 2 * h) rs[nr++] = r + ny;
+  if (r >= ny) rs[nr++] = r - ny;
+#if NDIMS == 3
+  int ps[3];
+  int np = 0;
+  ps[np++] = p;
+  if (p < 2 * h) ps[np++] = p + nz;
+  if (p >= nz) ps[np++] = p - nz;
+  for (int b = 0; b < np; b++)
+    for (int c = 0; c < nr; c++)
+      if (b != 0 || c != 0) a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])] = v;
+#else
+  (void)p; (void)nz; (void)n1;
+  for (int c = 1; c < nr; c++) a[(size_t)x + (size_t)n0 * (size_t)rs[c]] = v;
+#endif
+}
+
+// Neumann wall update of a cell array at interior row (r, p): the halo
+// images of the wall entries receive the wall values from before this update
+// (the reference copies halos before it sets the walls), then the walls take
+// the neighbouring interior values.
+__device__ inline void neumann_walls(double * __restrict__ a, int i, int nx, int n1, int r, int p,
+                                     int ny, int nz, double vnew) {
+  const int n0 = nx + 2;
+  if (i == 1) {
+    const size_t w = (size_t)n0 * ((size_t)r + (size_t)n1 * (size_t)p);
+    store_images(a, 0, n0, n1, r, p, ny, nz, 1, a[w]);
+    a[w] = vnew;
+  }
+  if (i == nx) {
+    const size_t w = (size_t)(nx + 1) + (size_t)n0 * ((size_t)r + (size_t)n1 * (size_t)p);
+    store_images(a, nx + 1, n0, n1, r, p, ny, nz, 1, a[w]);
+    a[w] = vnew;
+  }
+}
+
 // Multi-field line solver: one launch solves the x lines of up to four
 // fields (blockIdx.y selects the field).  256 lines per work-group, one line
 // per lane; x is streamed through LDS in chunks of CH values stored x-major
@@ -770,6 +876,7 @@ struct LineJob {
   LineField f[4];
   int nlines, ny, n1;        // lines per field, rows per plane, padded rows (0 in 2D)
   double ca, cb, cg;
+  int images, nz;            // also write periodic halo images (fused schedule)
 };
 
 constexpr int kLLines = 256;
@@ -960,20 +1067,22 @@ __global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
     }
   }
   __syncthreads();
-  // phase 3: field rows += solution
+  // phase 3: field rows += solution (and their halo images)
   {
     int e = tid, ll = tid / n, m = tid - (tid / n) * n;
     while (e < total) {
-      double * ptr[kWUnroll];
       double val[kWUnroll];
+      int xs[kWUnroll], rs[kWUnroll], ps[kWUnroll];
       int cnt = 0;
 #pragma unroll
       for (int u = 0; u < kWUnroll; u++) {
         if (e < total) {
           const int line = line0 + ll;
           const int jj = line % J.ny, kk = line / J.ny;
-          ptr[u] = F.field + (size_t)(F.i0 + m) + (size_t)F.n0 * ((size_t)(jj + 1) + (size_t)J.n1 * (size_t)(kk + 1));
-          val[u] = *ptr[u] + S[ll * ld + m];
+          xs[u] = F.i0 + m;
+          rs[u] = jj + 1;
+          ps[u] = kk + 1;
+          val[u] = F.field[(size_t)xs[u] + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])] + S[ll * ld + m];
           cnt = u + 1;
           e += kWThreads;
           m += kWThreads;
@@ -981,8 +1090,23 @@ __global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < kWUnroll; u++)
-        if (u < cnt) *ptr[u] = val[u];
+      for (int u = 0; u < kWUnroll; u++) {
+        if (u < cnt) {
+          if (J.images) {
+            store_periodic(F.field, xs[u], F.n0, J.n1, rs[u], ps[u], J.ny, J.nz, 1, val[u]);
+            // the Dirichlet wall entries of this row are constants the loop
+            // never changes; their halo images take them as the reference's
+            // halo copy would (overwriting values left from the initial load)
+            if (xs[u] == F.i0 || xs[u] == F.i0 + n - 1) {
+              const int xw = xs[u] == F.i0 ? F.i0 - 1 : F.i0 + n;
+              const double wv = F.field[(size_t)xw + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])];
+              store_images(F.field, xw, F.n0, J.n1, rs[u], ps[u], J.ny, J.nz, 1, wv);
+            }
+          } else {
+            F.field[(size_t)xs[u] + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])] = val[u];
+          }
+        }
+      }
     }
   }
 }
@@ -1118,14 +1242,15 @@ constexpr int kFftElems = 2048;   // complex elements per LDS buffer
 // assign_input (dft.c:315-345) fused with the real transform in y.  Two x
 // columns are packed into one complex column; the r2c halves are separated
 // after the transform.  Output spec[x][k][ky].
+template <int ELEMS>
 __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
     double2 * __restrict__ spec) {
-  __shared__ double2 sa[kFftElems], sb[kFftElems];
+  __shared__ double2 sa[ELEMS], sb[ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = kFftElems / N;            // complex columns per block
+  const int NC = ELEMS / N;                // complex columns per block
   const int x0 = blockIdx.x * 2 * NC;      // first interior x (0-based)
 #if NDIMS == 2
   const int k = 1;
@@ -1171,11 +1296,12 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
 
 #if NDIMS == 3
 // complex transform in z of spec[x][kz][ky] for a run of ky at one x
+template <int ELEMS>
 __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const double2 * __restrict__ tw,
                                                       double2 * __restrict__ spec, int nh, double sign) {
-  __shared__ double2 sa[kFftElems], sb[kFftElems];
+  __shared__ double2 sa[ELEMS], sb[ELEMS];
   const int N = g.nz;
-  const int NC = kFftElems / N;
+  const int NC = ELEMS / N;
   const int ky0 = blockIdx.x * NC;
   const int x = blockIdx.y;
   double2 * base = spec + (size_t)x * N * nh;
@@ -1196,8 +1322,12 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const 
 
 // solve_linear_systems (dft.c:364-395): per (ky, kz) line the complex
 // tri-diagonal system in x with c = -l - u + lambda and the Neumann fold,
-// eliminated as gtsv (tdm.c:17-50).  v goes to scratch vbuf[x][line].
-__global__ void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
+// eliminated as gtsv (tdm.c:17-50).  v goes to scratch vbuf[x][line].  One
+// lane per line; the rows of the next chunk of kTsU are loaded before the
+// current chunk is eliminated, so kTsU loads per lane stay in flight.
+constexpr int kTsU = 8;
+
+__global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, double2 * __restrict__ spec,
                               double * __restrict__ vbuf) {
@@ -1224,17 +1354,38 @@ __global__ void k_thomas_spec(int nx, int nlines, int nh, const double * __restr
   qprev.y = qprev.y / cc;
   q[0] = qprev;
   v[0] = vprev;
-  for (int i = 1; i < n - 1; i++) {
-    const double c = -pl[i] - pu[i] + eval;
-    const double val = 1. / (c - pl[i] * vprev);
-    const double vi = val * (pu[i]);
-    double2 qi = q[(size_t)i * st];
-    qi.x = val * (qi.x - pl[i] * qprev.x);
-    qi.y = val * (qi.y - pl[i] * qprev.y);
-    q[(size_t)i * st] = qi;
-    v[(size_t)i * st] = vi;
-    qprev = qi;
-    vprev = vi;
+  {
+    double2 cur[kTsU];
+#pragma unroll
+    for (int u = 0; u < kTsU; u++) {
+      if (1 + u < n - 1) cur[u] = q[(size_t)(1 + u) * st];
+    }
+    for (int i0 = 1; i0 < n - 1; i0 += kTsU) {
+      double2 nxt[kTsU];
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) {
+        const int i = i0 + kTsU + u;
+        if (i < n - 1) nxt[u] = q[(size_t)i * st];
+      }
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) {
+        const int i = i0 + u;
+        if (i < n - 1) {
+          const double c = -pl[i] - pu[i] + eval;
+          const double val = 1. / (c - pl[i] * vprev);
+          const double vi = val * (pu[i]);
+          double2 qi = cur[u];
+          qi.x = val * (qi.x - pl[i] * qprev.x);
+          qi.y = val * (qi.y - pl[i] * qprev.y);
+          q[(size_t)i * st] = qi;
+          v[(size_t)i * st] = vi;
+          qprev = qi;
+          vprev = vi;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) cur[u] = nxt[u];
+    }
   }
   {
     const int i = n - 1;
@@ -1253,25 +1404,58 @@ __global__ void k_thomas_spec(int nx, int nlines, int nh, const double * __restr
     q[(size_t)i * st] = qi;
     qprev = qi;
   }
-  for (int i = n - 2; i >= 0; i--) {
-    double2 qi = q[(size_t)i * st];
-    const double vi = v[(size_t)i * st];
-    qi.x -= vi * qprev.x;
-    qi.y -= vi * qprev.y;
-    q[(size_t)i * st] = qi;
-    qprev = qi;
+  {
+    double2 cq[kTsU];
+    double cv[kTsU];
+#pragma unroll
+    for (int u = 0; u < kTsU; u++) {
+      const int i = n - 2 - u;
+      if (i >= 0) {
+        cq[u] = q[(size_t)i * st];
+        cv[u] = v[(size_t)i * st];
+      }
+    }
+    for (int i0 = n - 2; i0 >= 0; i0 -= kTsU) {
+      double2 nq[kTsU];
+      double nv[kTsU];
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) {
+        const int i = i0 - kTsU - u;
+        if (i >= 0) {
+          nq[u] = q[(size_t)i * st];
+          nv[u] = v[(size_t)i * st];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) {
+        const int i = i0 - u;
+        if (i >= 0) {
+          double2 qi = cq[u];
+          qi.x -= cv[u] * qprev.x;
+          qi.y -= cv[u] * qprev.y;
+          q[(size_t)i * st] = qi;
+          qprev = qi;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kTsU; u++) {
+        cq[u] = nq[u];
+        cv[u] = nv[u];
+      }
+    }
   }
 }
 
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and
 // k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
+template <int ELEMS>
 __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
-    double * __restrict__ psi) {
-  __shared__ double2 sa[kFftElems], sb[kFftElems];
+    double * __restrict__ psi, int images) {
+  __shared__ double2 sa[ELEMS], sb[ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = kFftElems / N;
+  const int NC = ELEMS / N;
   const int x0 = blockIdx.x * 2 * NC;
   const int nh = N / 2 + 1;
 #if NDIMS == 2
@@ -1302,7 +1486,15 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
     const int i = x0 + xl + 1;
     if (i <= nx) {
       const double2 z = res[(xl >> 1) * N + j];
-      psi[I_C1(i, j + 1, k)] = (xl & 1) ? z.y : z.x;
+      const double v = (xl & 1) ? z.y : z.x;
+      if (images) {
+        // fluid_update_boundaries_psi folded in: halo images, Neumann walls
+        const int pl = NDIMS == 3 ? k : 0;
+        store_periodic(psi, i, nx + 2, ny + 2, j + 1, pl, ny, nz, 1, v);
+        neumann_walls(psi, i, nx, ny + 2, j + 1, pl, ny, nz, v);
+      } else {
+        psi[I_C1(i, j + 1, k)] = v;
+      }
     }
   }
 }
@@ -1404,7 +1596,10 @@ __global__ __launch_bounds__(kRedThreads) void k_min_final(const double * __rest
 // ---------------------------------------------------------------------------
 // diagnostics (logging/divergence.c, interface.c, energy.c): per-block
 // partials {max|div|, vof min, vof max, sum vof dV, sum dV, KE x, y, z, TE}
-constexpr int kDiagN = 9;
+// partial sums: 0 max|div|, 1 vof min, 2 vof max, 3 vof volume, 4 volume,
+// 5-7 kinetic energy x y z, 8 thermal energy, 9-11 momentum x y z,
+// 12 interface measure sum |grad vof| dV
+constexpr int kDiagN = 13;
 
 __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
@@ -1419,7 +1614,8 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
   const size_t ncell = (size_t)nx * ny * g.nz;
   const double dyz = g.dy * g.dz;
 #endif
-  double q[kDiagN] = {0., 1., 0., 0., 0., 0., 0., 0., 0.};
+  double q[kDiagN];
+  for (int m = 0; m < kDiagN; m++) q[m] = m == 1 ? 1. : 0.;
   for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < ncell;
        s += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(s % nx) + 1;
@@ -1437,6 +1633,7 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     div += (uz[I_C1(i, j, k + 1)] - uz[I_C1(i, j, k)]) / g.dz;
     const double w = uz[I_C1(i, j, k)];
     q[7] += 0.5 * (w * w) * (dx * dyz);
+    q[11] += w * (dx * dyz);
 #endif
     q[0] = fmax(q[0], fabs(div));
     const double lv = vof[I_VOF(i, j, k)];
@@ -1447,11 +1644,24 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     if (i >= 2) {
       const double u = ux[I_UX(i, j, k)];
       q[5] += 0.5 * (u * u) * (g.dxc[i - 1] * dyz);
+      q[9] += u * (g.dxc[i - 1] * dyz);
     }
     const double v = uy[I_C1(i, j, k)];
     q[6] += 0.5 * (v * v) * (dx * dyz);
+    q[10] += v * (dx * dyz);
     const double tt = t[I_C1(i, j, k)];
     q[8] += 0.5 * (tt * tt) * (dx * dyz);
+    {
+      const double gx = (vof[I_VOF(i + 1, j, k)] - vof[I_VOF(i - 1, j, k)]) / (g.dxc[i - 1] + g.dxc[i]);
+      const double gy = (vof[I_VOF(i, j + 1, k)] - vof[I_VOF(i, j - 1, k)]) / (2. * g.dy);
+#if NDIMS == 2
+      const double g2 = gx * gx + gy * gy;
+#else
+      const double gz = (vof[I_VOF(i, j, k + 1)] - vof[I_VOF(i, j, k - 1)]) / (2. * g.dz);
+      const double g2 = gx * gx + gy * gy + gz * gz;
+#endif
+      q[12] += sqrt(g2) * (dx * dyz);
+    }
   }
   __shared__ double red[kDiagN][kRedThreads];
   for (int m = 0; m < kDiagN; m++) red[m][threadIdx.x] = q[m];
@@ -1474,6 +1684,331 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
 __global__ void k_copy(const double * __restrict__ a, double * __restrict__ b, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
     b[t] = a[t];
+}
+
+
+// ===========================================================================
+// Fused schedule.  Each kernel below produces exactly the values of the
+// per-function kernels above (same expressions in the same order) and also
+// writes the periodic halo images and x walls the reference's boundary
+// update (src/halo.c:18-76 then fluid/boundary/*.c) would produce, so no
+// separate boundary launch is needed.
+
+#if NDIMS == 2
+#define PLANE1(k) 0
+#define PLANE2(k) 0
+#else
+#define PLANE1(k) (k)
+#define PLANE2(k) ((k) + 1)
+#endif
+
+// interface_compute_curvature_tensor fused: corner normals of a tile are
+// computed once into LDS (compute_gradient), then each cell forms its
+// curvature and, at interface cells, its normal and THINC intercept
+// (compute_normal / compute_curvature).  Interior cells only; the halo
+// cells get their periodic images.
+#if NDIMS == 2
+constexpr int kIX = 64, kIY = 8, kIZ = 1;
+#else
+constexpr int kIX = 32, kIY = 4, kIZ = 4;
+#endif
+constexpr int kIThreads = kIX * kIY * kIZ;
+
+__global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double * __restrict__ vof,
+                                                           double * __restrict__ normal,
+                                                           double * __restrict__ curv) {
+  constexpr int VX = kIX + 2, VY = kIY + 2;
+  constexpr int CX = kIX + 1, CY = kIY + 1;
+#if NDIMS == 2
+  constexpr int VZ = 1, CZ = 1;
+#else
+  constexpr int VZ = kIZ + 2, CZ = kIZ + 1;
+#endif
+  __shared__ double sv[VX * VY * VZ];
+  __shared__ double sc[NDIMS][CX * CY * CZ];
+  const int nx = g.nx, ny = g.ny;
+  const int i0 = 1 + blockIdx.x * kIX, j0 = 1 + blockIdx.y * kIY;
+#if NDIMS == 3
+  const int nz = g.nz;
+  const int k0 = 1 + blockIdx.z * kIZ;
+#else
+  const int nz = 1;
+  const int k0 = 1;
+#endif
+  const int tid = threadIdx.x;
+  // vof tile: cells (i0-1 .. i0+kIX, j0-1 .., k0-1 ..)
+  for (int e = tid; e < VX * VY * VZ; e += kIThreads) {
+    const int a = e % VX, r = e / VX;
+    const int b = r % VY;
+    const int c = r / VY;
+    const int i = i0 - 1 + a, j = j0 - 1 + b;
+#if NDIMS == 3
+    const int k = k0 - 1 + c;
+    const bool ok = i <= nx + 1 && j <= ny + 2 && k <= nz + 2;
+#else
+    const int k = 1;
+    (void)c;
+    const bool ok = i <= nx + 1 && j <= ny + 2;
+#endif
+    sv[e] = ok ? vof[I_VOF(i, j, k)] : 0.;
+  }
+  __syncthreads();
+  // corner normals (i0 .. i0+kIX, j0 .., k0 ..): compute_gradient
+  for (int e = tid; e < CX * CY * CZ; e += kIThreads) {
+    const int a = e % CX, r = e / CX;
+    const int b = r % CY;
+    const int c = r / CY;
+    const int i = i0 + a;
+    const double dx = g.dxc[min(i, nx + 1) - 1];
+#define SV(A, B, C) sv[(A) + VX * ((B) + VY * (C))]
+#if NDIMS == 2
+    (void)c;
+    const double va = SV(a, b, 0), vb = SV(a + 1, b, 0);
+    const double vc = SV(a, b + 1, 0), vd = SV(a + 1, b + 1, 0);
+    const double gx = 1. / dx * (-va + vb - vc + vd);
+    const double gy = 1. / g.dy * (-va - vb + vc + vd);
+    const double norm = sqrt(gx * gx + gy * gy);
+    const double ninv = 1. / fmax(norm, DBL_EPSILON);
+    sc[0][e] = gx * ninv;
+    sc[1][e] = gy * ninv;
+#else
+    const double a0 = SV(a, b, c), b0 = SV(a + 1, b, c), c0 = SV(a, b + 1, c), d0 = SV(a + 1, b + 1, c);
+    const double a1 = SV(a, b, c + 1), b1 = SV(a + 1, b, c + 1), c1 = SV(a, b + 1, c + 1), d1 = SV(a + 1, b + 1, c + 1);
+    const double sx0 = -a0 + b0 - c0 + d0, sx1 = -a1 + b1 - c1 + d1;
+    const double sy0 = -a0 - b0 + c0 + d0, sy1 = -a1 - b1 + c1 + d1;
+    const double sz0 = a0 + b0 + c0 + d0, sz1 = a1 + b1 + c1 + d1;
+    const double gx = 1. / dx * (sx0 + sx1);
+    const double gy = 1. / g.dy * (sy0 + sy1);
+    const double gz = 1. / g.dz * (-sz0 + sz1);
+    const double norm = sqrt(gx * gx + gy * gy + gz * gz);
+    const double ninv = 1. / fmax(norm, DBL_EPSILON);
+    sc[0][e] = gx * ninv;
+    sc[1][e] = gy * ninv;
+    sc[2][e] = gz * ninv;
+#endif
+#undef SV
+  }
+  __syncthreads();
+  const int a = tid % kIX, b = (tid / kIX) % kIY;
+  const int i = i0 + a, j = j0 + b;
+#if NDIMS == 3
+  const int c = tid / (kIX * kIY);
+  const int k = k0 + c;
+  if (i > nx || j > ny || k > nz) return;
+#else
+  const int c = 0;
+  const int k = 1;
+  if (i > nx || j > ny) return;
+#endif
+#define SC(D, A, B, C) sc[D][(A) + CX * ((B) + CY * (C))]
+  const double dx = g.dxf[i - 1];
+  double cv;
+#if NDIMS == 2
+  {
+    const double dnxdx = 1. / dx * (-SC(0, a, b, 0) + SC(0, a + 1, b, 0) - SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0));
+    const double dnydy = 1. / g.dy * (-SC(1, a, b, 0) - SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0));
+    cv = 0.5 * (-dnxdx - dnydy);
+  }
+#else
+  {
+    const double dnxdx = 1. / dx * ((-SC(0, a, b, c) + SC(0, a + 1, b, c) - SC(0, a, b + 1, c) + SC(0, a + 1, b + 1, c))
+                                    + (-SC(0, a, b, c + 1) + SC(0, a + 1, b, c + 1) - SC(0, a, b + 1, c + 1) + SC(0, a + 1, b + 1, c + 1)));
+    const double dnydy = 1. / g.dy * ((-SC(1, a, b, c) - SC(1, a + 1, b, c) + SC(1, a, b + 1, c) + SC(1, a + 1, b + 1, c))
+                                      + (-SC(1, a, b, c + 1) - SC(1, a + 1, b, c + 1) + SC(1, a, b + 1, c + 1) + SC(1, a + 1, b + 1, c + 1)));
+    const double dnzdz = 1. / g.dz * (-(SC(2, a, b, c) + SC(2, a + 1, b, c) + SC(2, a, b + 1, c) + SC(2, a + 1, b + 1, c))
+                                      + (SC(2, a, b, c + 1) + SC(2, a + 1, b, c + 1) + SC(2, a, b + 1, c + 1) + SC(2, a + 1, b + 1, c + 1)));
+    cv = 0.25 * (-dnxdx - dnydy - dnzdz);
+  }
+#endif
+  store_periodic(curv, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, cv);
+  const double lvof = sv[(a + 1) + VX * ((b + 1) + VY * (NDIMS == 3 ? c + 1 : 0))];
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return;
+  double nv[NDIMS + 1];
+#if NDIMS == 2
+  {
+    double nxv = SC(0, a, b, 0) + SC(0, a + 1, b, 0) + SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0);
+    double nyv = SC(1, a, b, 0) + SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0);
+    nxv /= dx;
+    nyv /= g.dy;
+    const double norm = sqrt(nxv * nxv + nyv * nyv);
+    const double ninv = 1. / fmax(norm, DBL_EPSILON);
+    nv[0] = nxv * ninv;
+    nv[1] = nyv * ninv;
+  }
+#else
+  for (int d = 0; d < 3; d++) {
+    nv[d] = (SC(d, a, b, c) + SC(d, a + 1, b, c) + SC(d, a, b + 1, c) + SC(d, a + 1, b + 1, c))
+          + (SC(d, a, b, c + 1) + SC(d, a + 1, b, c + 1) + SC(d, a, b + 1, c + 1) + SC(d, a + 1, b + 1, c + 1));
+  }
+  nv[0] /= dx;
+  nv[1] /= g.dy;
+  nv[2] /= g.dz;
+  {
+    const double norm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const double ninv = 1. / fmax(norm, DBL_EPSILON);
+    nv[0] *= ninv;
+    nv[1] *= ninv;
+    nv[2] *= ninv;
+  }
+#endif
+#undef SC
+  nv[NDIMS] = intercept(lvof, nv);
+  // NORMAL rows are 0..ny+1 like cell arrays but x runs 1..nx without padding
+  for (int d = 0; d <= NDIMS; d++) {
+    // component d of each image: the component array is interleaved, so
+    // store through a strided view
+    double * base = normal + d;
+    const int n0 = nx * (NDIMS + 1);
+    store_periodic(base, (i - 1) * (NDIMS + 1), n0, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
+  }
+}
+
+// interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
+// (compute_flux_*), the flux divergence and the RK update, written to the
+// other VOF buffer with its halo images (walls stay zero).
+__global__ void k_vof_fused(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                            const double * __restrict__ uz, const double * __restrict__ vof,
+                            const double * __restrict__ normal, double * __restrict__ srca,
+                            const double * __restrict__ srcb, double * __restrict__ vof_new,
+                            double dta, double dtb, int use_b) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+#if NDIMS == 3
+  const int nz = g.nz;
+#else
+  const int nz = 1;
+  (void)uz;
+#endif
+  const double fxm = i == 1 ? 0. : flux_x_at(ux, vof, normal, nx, ny, i, j, k);
+  const double fxp = i == nx ? 0. : flux_x_at(ux, vof, normal, nx, ny, i + 1, j, k);
+  const double fym = flux_y_at(uy, vof, normal, nx, ny, i, j, k);
+  const double fyp = flux_y_at(uy, vof, normal, nx, ny, i, j + 1, k);
+  double s = 0.;
+  s += 1. / g.dxf[i - 1] * (fxm - fxp);
+  s += 1. / g.dy * (fym - fyp);
+#if NDIMS == 3
+  const double fzm = flux_z_at(uz, vof, normal, nx, ny, i, j, k);
+  const double fzp = flux_z_at(uz, vof, normal, nx, ny, i, j, k + 1);
+  s += 1. / g.dz * (fzm - fzp);
+#endif
+  const size_t is = I_SC(i, j, k);
+  srca[is] = s;
+  double v = vof[I_VOF(i, j, k)];
+  v += dta * s;
+  if (use_b) v += dtb * srcb[is];
+  store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
+}
+
+// fluid_predict_field sources of all fields at once, with the CSF force
+// (force.c:11-66) formed in place from vof and curvature; writes srca and the
+// increment of every DOF
+struct FusedRhs {
+  RhsArgs base;
+  const double * vof;
+  const double * curv;
+  double tension;
+  const double * lapx[4];
+  double dif[4];
+  double * srca[4];
+  const double * srcb[4];
+  double * delta[4];
+};
+
+__global__ void k_rhs_fused(Grid g, FusedRhs f) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+  const double * vof = f.vof;
+  const double * curv = f.curv;
+  const double v = vof[I_VOF(i, j, k)];
+  const double cv = curv[I_C1(i, j, k)];
+  RhsArgs a = f.base;
+  double sa, sg;
+  const double ca = a.ca, cb = a.cb, cg = a.cg;
+  if (i >= 2) {
+    const double grad = 1. / g.dxc[i - 1] * (-vof[I_VOF(i - 1, j, k)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i - 1, j, k)] + cv);
+    a.lapx = f.lapx[0];
+    a.dif = f.dif[0];
+    rhs_ux_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
+    const size_t is = I_SX(i, j, k);
+    f.srca[0][is] = sa;
+    f.delta[0][is] = +ca * sa + cb * f.srcb[0][is] + cg * sg;
+  }
+  const size_t is = I_SC(i, j, k);
+  {
+    const double grad = 1. / g.dy * (-vof[I_VOF(i, j - 1, k)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i, j - 1, k)] + cv);
+    a.lapx = f.lapx[1];
+    a.dif = f.dif[1];
+    rhs_uy_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
+    f.srca[1][is] = sa;
+    f.delta[1][is] = +ca * sa + cb * f.srcb[1][is] + cg * sg;
+  }
+#if NDIMS == 3
+  {
+    const double grad = 1. / g.dz * (-vof[I_VOF(i, j, k - 1)] + v);
+    const double kappa = 0.5 * (curv[I_C1(i, j, k - 1)] + cv);
+    a.lapx = f.lapx[2];
+    a.dif = f.dif[2];
+    rhs_uz_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
+    f.srca[2][is] = sa;
+    f.delta[2][is] = +ca * sa + cb * f.srcb[2][is] + cg * sg;
+  }
+#endif
+  {
+    a.lapx = f.lapx[3];
+    a.dif = f.dif[3];
+    rhs_t_at(g, a, i, j, k, 0., sa, sg);
+    f.srca[3][is] = sa;
+    f.delta[3][is] = +ca * sa + cb * f.srcb[3][is] + cg * sg;
+  }
+}
+
+__device__ inline double prefactor_div(double a, double b) { return a / b; }
+
+// fluid_correct_velocity + fluid_update_pressure fused, with halo images
+// and the Neumann walls of p
+__global__ void k_corr_press(Grid g, double pref_c, double pref_p, const double * __restrict__ psi,
+                             double * __restrict__ ux, double * __restrict__ uy,
+                             double * __restrict__ uz, double * __restrict__ p) {
+  const int nx = g.nx, ny = g.ny;
+  THREAD_IJK(1, 1, 1)
+  if (i > nx || j > ny) return;
+#if NDIMS == 3
+  const int nz = g.nz;
+#else
+  const int nz = 1;
+  (void)uz;
+#endif
+  const double pc = psi[I_C1(i, j, k)];
+  const double pm = psi[I_C1(i - 1, j, k)];
+  if (i >= 2) {
+    const double dx = g.dxc[i - 1];
+    const double v = ux[I_UX(i, j, k)] - prefactor_div(pref_c, dx) * (+pc - pm);
+    store_periodic(ux, i - 1, nx + 1, ny + 2, j, PLANE1(k), ny, nz, 1, v);
+  }
+  {
+    const double v = uy[I_C1(i, j, k)] - prefactor_div(pref_c, g.dy) * (+pc - psi[I_C1(i, j - 1, k)]);
+    store_periodic(uy, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
+  }
+#if NDIMS == 3
+  {
+    const double v = uz[I_C1(i, j, k)] - prefactor_div(pref_c, g.dz) * (+pc - psi[I_C1(i, j, k - 1)]);
+    store_periodic(uz, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
+  }
+#endif
+  {
+    const double pp = psi[I_C1(i + 1, j, k)];
+    double v = p[I_C1(i, j, k)];
+    v += pc;
+    const double dpsidx_xm = (-pm + pc) / g.dxc[i - 1];
+    const double dpsidx_xp = (-pc + pp) / g.dxc[i];
+    v -= pref_p / g.dxf[i - 1] * (-dpsidx_xm + dpsidx_xp);
+    store_periodic(p, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
+    neumann_walls(p, i, nx, ny + 2, j, PLANE1(k), ny, nz, v);
+  }
 }
 
 }  // namespace vofk

@@ -43,6 +43,14 @@ static int fail(const char * fmt, ...) {
                                       hipGetErrorString(e_));                      \
   } while (0)
 
+// release paths: a failure to free is not reported (the handle is gone either way)
+static void dev_free(void * p) {
+  if (p) (void)hipFree(p);
+}
+static void host_free(void * p) {
+  if (p) (void)hipHostFree(p);
+}
+
 #define LAUNCHCHK()                                                                \
   do {                                                                             \
     hipError_t e_ = hipGetLastError();                                             \
@@ -201,6 +209,11 @@ struct vof_ctx {
   double prm[P_COUNT];
   bool prm_set[P_COUNT];
   int schedule;
+  int fuse;                              // F_* stages fused in schedule 1
+  int fft_elems;                         // LDS complex elements per FFT buffer (1024 / 2048)
+  int lines_lpb;                         // lines per work-group of the x solver (0 = auto)
+  double * d_vof_spare;                  // second VOF buffer of the fused update
+  size_t vof_bytes;
   Grid grid;
   double * d_dxf;
   double * d_dxc;
@@ -432,7 +445,7 @@ static int array_prepare(const domain_t * domain, const char * name, size_t elsi
 }
 
 static int array_free(array_t * a) {
-  if (a->data) hipFree(a->data);
+  if (a->data) dev_free(a->data);
   free(a->nadds);
   a->data = nullptr;
   a->nadds = nullptr;
@@ -613,6 +626,11 @@ static int ctx_create(domain_t * domain) {
 #endif
   ctx->dy = domain->dy;
   ctx->schedule = 1;
+  ctx->fuse = 63;
+  ctx->fft_elems = 2048;
+  ctx->lines_lpb = 0;
+  ctx->d_vof_spare = nullptr;
+  ctx->vof_bytes = 0;
   ctx->prof_on = false;
   ctx->ev_next = 0;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
@@ -789,17 +807,18 @@ extern "C" int domain_check_x_grid_is_uniform(const domain_t * domain, bool * x_
 extern "C" int domain_finalise(domain_t * domain) {
   vof_ctx * ctx = ctx_of(domain);
   if (ctx) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(ctx->d_dxf); hipFree(ctx->d_dxc); hipFree(ctx->d_lapx_ux); hipFree(ctx->d_lapx_c);
-    hipFree(ctx->d_tab); hipHostFree(ctx->h_tab);
-    for (int m = 0; m < 12; m++) hipEventDestroy(ctx->tab_done[m]);
-    hipFree(ctx->d_twy); if (ctx->d_twz) hipFree(ctx->d_twz);
-    hipFree(ctx->d_pl); hipFree(ctx->d_pu); hipFree(ctx->d_ey); hipFree(ctx->d_ez);
-    hipFree(ctx->d_spec); hipFree(ctx->d_vbuf); hipFree(ctx->d_delta); hipFree(ctx->d_part);
-    hipHostFree(ctx->h_scalar);
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx->d_dxf); dev_free(ctx->d_dxc); dev_free(ctx->d_lapx_ux); dev_free(ctx->d_lapx_c);
+    dev_free(ctx->d_tab); host_free(ctx->h_tab);
+    for (int m = 0; m < 12; m++) (void)hipEventDestroy(ctx->tab_done[m]);
+    dev_free(ctx->d_twy); if (ctx->d_twz) dev_free(ctx->d_twz);
+    dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez);
+    dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
+    if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
+    host_free(ctx->h_scalar);
     prof_flush(ctx);
-    for (auto & e : ctx->ev_pool) hipEventDestroy(e);
-    hipStreamDestroy(ctx->stream);
+    for (auto & e : ctx->ev_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     domain->info = nullptr;
   }
@@ -879,7 +898,7 @@ extern "C" int fluid_init(const char dirname_ic[], const domain_t * domain, flui
 }
 
 extern "C" int fluid_finalise(const domain_t * domain, fluid_t * fluid) {
-  if (ctx_of(domain)) hipStreamSynchronize(ctx_of(domain)->stream);
+  if (ctx_of(domain)) (void)hipStreamSynchronize(ctx_of(domain)->stream);
 #define FREE(n) array_free(&fluid->n);
   FOR_FLUID_ARRAYS(FREE)
 #undef FREE
@@ -1054,8 +1073,12 @@ static void swap_data(array_t * a, array_t * b) {
 // solve and the update of every field (predict, :42-53; ux.c:306-388 etc.).
 // fused = 1: the source kernels write the increment directly (srcg is not
 // stored) and the line solver reads it.
+// vof_old / curv non-null: the sources of all fields come from one fused
+// kernel that forms the CSF force from them (fused schedule); images: the
+// line solver writes the halo images itself instead of boundary launches.
 static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid,
-                        const interface_t * interface, int fused) {
+                        const interface_t * interface, int fused, const double * vof_old = nullptr,
+                        int images = 0) {
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
   // reset_srcs (predict/main.c:9-29): the a <-> b swap; a (and g) are fully
@@ -1114,6 +1137,24 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
   lj.ny = ny;
   lj.n1 = NDIMS == 3 ? ny + 2 : 0;
   lj.ca = ca; lj.cb = cb; lj.cg = cg;
+  lj.images = images;
+  lj.nz = nz;
+  if (vof_old) {
+    vofk::FusedRhs fr{};
+    fr.base = base;
+    fr.vof = vof_old;
+    fr.curv = (const double *)interface->curv.data;
+    fr.tension = interface->tension;
+    for (int f = 0; f < nj; f++) {
+      const int kind = jobs[f].kind;
+      fr.lapx[kind] = jobs[f].lapx;
+      fr.dif[kind] = jobs[f].dif;
+      fr.srca[kind] = (double *)jobs[f].src[0].data;
+      fr.srcb[kind] = (const double *)jobs[f].src[1].data;
+      fr.delta[kind] = ctx->d_delta + (size_t)f * ncell;
+    }
+    KLN(ctx, "k_rhs_fused", vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid, fr);
+  }
   for (int f = 0; f < nj; f++) {
     const int kind = jobs[f].kind;
     vofk::RhsArgs a = base;
@@ -1122,7 +1163,9 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     a.srca = (double *)jobs[f].src[0].data;
     a.dif = jobs[f].dif;
     double * slab = ctx->d_delta + (size_t)f * ncell;
-    if (fused) {
+    if (vof_old) {
+      // sources already formed by k_rhs_fused
+    } else if (fused) {
       a.delta = slab;
       a.srcb = (const double *)jobs[f].src[1].data;
       a.srcg = nullptr;
@@ -1132,14 +1175,14 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
       a.srcg = (double *)jobs[f].src[2].data;
     }
     const int ni = kind == 0 ? nx - 1 : nx;
-    KLN(ctx, jobs[f].name, jobs[f].kern, grid3(ni, ny, nz, kBlk), kBlk, ctx->grid, a);
+    if (!vof_old) KLN(ctx, jobs[f].name, jobs[f].kern, grid3(ni, ny, nz, kBlk), kBlk, ctx->grid, a);
     const double * dtab = nullptr;
     if (upload_tab(ctx, kind, (int)rkstep, kind == 3 ? pt : pm, &dtab)) return 1;
     vofk::LineField & F = lj.f[f];
     F.tab = dtab;
     F.srca = (const double *)jobs[f].src[0].data;
     F.srcb = (const double *)jobs[f].src[1].data;
-    F.srcg = fused ? nullptr : (const double *)jobs[f].src[2].data;
+    F.srcg = (fused || vof_old) ? nullptr : (const double *)jobs[f].src[2].data;
     F.q = slab;
     F.field = (double *)jobs[f].field->data;
     F.n = ni;
@@ -1148,9 +1191,10 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
   }
   {
     // lines per work-group so that the LDS tile stays within 32 KiB
-    int lpb = (int)(32768 / (sizeof(double) * (nx | 1)));
-    if (lpb > 32) lpb = 32;
+    int lpb = ctx->lines_lpb > 0 ? ctx->lines_lpb : (int)(32768 / (sizeof(double) * (nx | 1)));
+    if (lpb > 64) lpb = 64;
     if (lpb < 1) lpb = 1;
+    if ((size_t)lpb * (nx | 1) * sizeof(double) > 65536) lpb = (int)(65536 / (sizeof(double) * (nx | 1)));
     const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
     const dim3 lg((unsigned)((lj.nlines + lpb - 1) / lpb), (unsigned)nj);
     hipEvent_t e0_ = prof_begin(ctx);
@@ -1158,6 +1202,7 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     LAUNCHCHK();
     if (e0_ && prof_end(ctx, "k_lines_lds", e0_)) return 1;
   }
+  if (images) return 0;
   for (int f = 0; f < nj; f++) {
     double * fld = (double *)jobs[f].field->data;
     int rc = 0;
@@ -1179,54 +1224,72 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 
 // Poisson: both reference solvers are the same discrete problem; the FFT
 // normalisation 1/(ny nz) is folded into the right-hand side as dft.c:329-330
-static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid) {
+static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid, int images) {
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
   const double norm = (double)ny * (NDIMS == 3 ? (double)nz : 1.);
   const double prefactor = 1. / (kRk[rkstep][2] * dt) / norm;
-  const int NC = vofk::kFftElems / ny;
+  const int elems = (ctx->fft_elems == 1024 && ny <= 1024 && (NDIMS == 2 || nz <= 1024)) ? 1024 : 2048;
+  const int NC = elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
 #else
   const double * uz = nullptr;
 #endif
-  KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
-                     (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
-  LAUNCHCHK();
+  if (elems == 1024) {
+    KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y<1024>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+        ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
+        (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
+  } else {
+    KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+        ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
+        (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
+  }
 #if NDIMS == 3
-  const int NCz = vofk::kFftElems / nz;
+  const int NCz = elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
-  KLN(ctx, "k_fft_z", vofk::k_fft_z, gz, dim3(vofk::kFftThreads), ctx->grid,
-                     ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, -1.);
-  LAUNCHCHK();
+  if (elems == 1024) {
+    KLN(ctx, "k_fft_z", vofk::k_fft_z<1024>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+        ctx->d_spec, ctx->nh, -1.);
+  } else {
+    KLN(ctx, "k_fft_z", vofk::k_fft_z<2048>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+        ctx->d_spec, ctx->nh, -1.);
+  }
 #endif
   const int nlines = ctx->nh * nz;
-  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), nx,
-                     nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec,
-                     ctx->d_vbuf);
-  LAUNCHCHK();
+  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), nx, nlines, ctx->nh,
+      ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
 #if NDIMS == 3
-  KLN(ctx, "k_fft_z", vofk::k_fft_z, gz, dim3(vofk::kFftThreads), ctx->grid,
-                     ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, +1.);
-  LAUNCHCHK();
+  if (elems == 1024) {
+    KLN(ctx, "k_fft_z", vofk::k_fft_z<1024>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+        ctx->d_spec, ctx->nh, +1.);
+  } else {
+    KLN(ctx, "k_fft_z", vofk::k_fft_z<2048>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+        ctx->d_spec, ctx->nh, +1.);
+  }
 #endif
-  KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
-                     ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec,
-                     (double *)fluid->psi.data);
-  LAUNCHCHK();
-  return bc_neumann(ctx, (double *)fluid->psi.data);
+  if (elems == 1024) {
+    KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi<1024>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
+        images);
+  } else {
+    KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
+        images);
+  }
+  return images ? 0 : bc_neumann(ctx, (double *)fluid->psi.data);
 }
 
 extern "C" int fluid_compute_potential_dft(const domain_t * domain, const size_t rkstep,
                                            const double dt, fluid_t * fluid) {
-  return potential(domain, rkstep, dt, fluid);
+  return potential(domain, rkstep, dt, fluid, 0);
 }
 
 extern "C" int fluid_compute_potential_dct(const domain_t * domain, const size_t rkstep,
                                            const double dt, fluid_t * fluid) {
-  return potential(domain, rkstep, dt, fluid);
+  return potential(domain, rkstep, dt, fluid, 0);
 }
 
 // correct_velocity/{main,ux,uy}.c
@@ -1295,7 +1358,7 @@ extern "C" int interface_init(const char dirname_ic[], const domain_t * domain, 
 }
 
 extern "C" int interface_finalise(const domain_t * domain, interface_t * interface) {
-  if (ctx_of(domain)) hipStreamSynchronize(ctx_of(domain)->stream);
+  if (ctx_of(domain)) (void)hipStreamSynchronize(ctx_of(domain)->stream);
   array_free(&interface->vof);
   array_free(&interface->ifrcx);
   array_free(&interface->ifrcy);
@@ -1387,8 +1450,79 @@ extern "C" int interface_update_vof(const domain_t * domain, const size_t rkstep
 // ---------------------------------------------------------------------------
 // integrate (src/integrate.c:11-68)
 
+enum { F_IFACE = 1, F_VOF = 2, F_RHS = 4, F_LINES = 8, F_PSI = 16, F_CORR = 32 };
+
+static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
+  vof_ctx * ctx = ctx_of(domain);
+  const int fz = ctx->fuse;
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  if ((fz & F_VOF) && !ctx->d_vof_spare) {
+    ctx->vof_bytes = interface->vof.datasize;
+    HIPCHK(hipMalloc(&ctx->d_vof_spare, ctx->vof_bytes));
+    HIPCHK(hipMemsetAsync(ctx->d_vof_spare, 0, ctx->vof_bytes, ctx->stream));
+  }
+  if (fluid_decide_dt(domain, fluid, dt)) return 1;
+  const double dtv = *dt;
+#if NDIMS == 3
+  double * uz = (double *)fluid->uz.data;
+#else
+  double * uz = nullptr;
+#endif
+  for (size_t rk = 0; rk < 3; rk++) {
+    // curvature tensor (curvature_tensor.c:178-186)
+    if (fz & F_IFACE) {
+      const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
+                    (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
+      KLN(ctx, "k_iface_fused", vofk::k_iface_fused, gi, dim3(vofk::kIThreads), ctx->grid,
+          (const double *)interface->vof.data, (double *)interface->normal.data,
+          (double *)interface->curv.data);
+    } else {
+      if (interface_compute_curvature_tensor(domain, interface)) return 1;
+    }
+    const bool rhs_fused = (fz & F_RHS) && (fz & F_VOF);
+    if (!rhs_fused && interface_compute_force(domain, interface)) return 1;
+    // VOF update (update/main.c:112-126)
+    const double * vof_old = nullptr;
+    if (fz & F_VOF) {
+      if (rk != 0) swap_data(&interface->src[0], &interface->src[1]);
+      const double dta = dtv * kRk[rk][0];
+      const double dtb = dtv * kRk[rk][1];
+      KLN(ctx, "k_vof_fused", vofk::k_vof_fused, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid,
+          (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uz,
+          (const double *)interface->vof.data, (const double *)interface->normal.data,
+          (double *)interface->src[0].data, (const double *)interface->src[1].data,
+          ctx->d_vof_spare, dta, dtb, (int)(rk != 0));
+      void * tmp = interface->vof.data;
+      interface->vof.data = ctx->d_vof_spare;
+      ctx->d_vof_spare = (double *)tmp;
+      vof_old = ctx->d_vof_spare;
+    } else {
+      if (interface_update_vof(domain, rk, dtv, fluid, interface)) return 1;
+    }
+    // predictor (predict/main.c:62-79)
+    if (predict_impl(domain, rk, dtv, fluid, interface, 1, rhs_fused ? vof_old : nullptr,
+                     (fz & F_LINES) ? 1 : 0))
+      return 1;
+    // Poisson (both reference solvers are the same discrete problem)
+    if (potential(domain, rk, dtv, fluid, (fz & F_PSI) ? 1 : 0)) return 1;
+    // correction and pressure (correct_velocity/main.c:15-27, update_pressure.c:87-109)
+    if (fz & F_CORR) {
+      const double pref_c = kRk[rk][2] * dtv;
+      const double pref_p = 0.5 * kRk[rk][2] * dtv * fluid->m_dif;
+      KLN(ctx, "k_corr_press", vofk::k_corr_press, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid, pref_c,
+          pref_p, (const double *)fluid->psi.data, (double *)fluid->ux.data, (double *)fluid->uy.data,
+          uz, (double *)fluid->p.data);
+    } else {
+      if (fluid_correct_velocity(domain, rk, dtv, fluid)) return 1;
+      if (fluid_update_pressure(domain, rk, dtv, fluid)) return 1;
+    }
+  }
+  return 0;
+}
+
 extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
   CTX_OR_FAIL(domain);
+  if (ctx->schedule == 1) return integrate_fused(domain, fluid, interface, dt);
   bool uniform = false;
   domain_check_x_grid_is_uniform(domain, &uniform);
   if (fluid_decide_dt(domain, fluid, dt)) return 1;
@@ -1396,7 +1530,7 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
     if (interface_compute_curvature_tensor(domain, interface)) return 1;
     if (interface_compute_force(domain, interface)) return 1;
     if (interface_update_vof(domain, rk, *dt, fluid, interface)) return 1;
-    if (predict_impl(domain, rk, *dt, fluid, interface, ctx->schedule == 1)) return 1;
+    if (fluid_predict_field(domain, rk, *dt, fluid, interface)) return 1;
     if (uniform) {
       if (fluid_compute_potential_dct(domain, rk, *dt, fluid)) return 1;
     } else {
@@ -1405,6 +1539,32 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
     if (fluid_correct_velocity(domain, rk, *dt, fluid)) return 1;
     if (fluid_update_pressure(domain, rk, *dt, fluid)) return 1;
   }
+  return 0;
+}
+
+// tile parameters: "fft_elems" (1024 or 2048 complex per LDS buffer) and
+// "lines_lpb" (lines per work-group of the x solver, 0 = automatic)
+extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
+  CTX_OR_FAIL(domain);
+  if (!strcmp(name, "fft_elems")) {
+    if (value != 1024 && value != 2048) return fail("fft_elems must be 1024 or 2048");
+    if (value < ctx->ny || (NDIMS == 3 && value < ctx->nz)) return fail("fft_elems below the FFT length");
+    ctx->fft_elems = value;
+    return 0;
+  }
+  if (!strcmp(name, "lines_lpb")) {
+    if (value < 0 || value > 64) return fail("lines_lpb must be in 0..64");
+    ctx->lines_lpb = value;
+    return 0;
+  }
+  return fail("vof_set_tuning: unknown parameter %s", name);
+}
+
+// stages of the fused schedule (bit mask; the default is all of them)
+extern "C" int vof_set_fusion(domain_t * domain, int mask) {
+  CTX_OR_FAIL(domain);
+  if (mask < 0 || mask > 63) return fail("vof_set_fusion: mask %d out of range", mask);
+  ctx->fuse = mask;
   return 0;
 }
 
@@ -1427,7 +1587,8 @@ extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
   std::vector<double> h((size_t)nb * vofk::kDiagN);
   HIPCHK(hipMemcpyAsync(h.data(), ctx->d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  double q[vofk::kDiagN] = {0., 1., 0., 0., 0., 0., 0., 0., 0.};
+  double q[vofk::kDiagN];
+  for (int m = 0; m < vofk::kDiagN; m++) q[m] = m == 1 ? 1. : 0.;
   for (int b = 0; b < nb; b++) {
     const double * r = &h[(size_t)b * vofk::kDiagN];
     q[0] = fmax(q[0], r[0]);
@@ -1446,6 +1607,27 @@ extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
   out[o++] = q[7];
 #endif
   out[o++] = q[8];
+  out[o++] = q[9];
+  out[o++] = q[10];
+#if NDIMS == 3
+  out[o++] = q[11];
+#endif
+  out[o++] = q[12];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// step / time scalars of an IC or save directory (src/main.c:64-71, 26-27)
+
+extern "C" int vof_read_step_time(const char dirname[], size_t * step, double * time) {
+  if (npy::read(dirname, "step", kNpySizeT, {}, step, sizeof(size_t))) return 1;
+  if (npy::read(dirname, "time", kNpyDouble, {}, time, sizeof(double))) return 1;
+  return 0;
+}
+
+extern "C" int vof_write_step_time(const char dirname[], size_t step, double time) {
+  if (npy::write(dirname, "step", kNpySizeT, {}, &step, sizeof(size_t))) return 1;
+  if (npy::write(dirname, "time", kNpyDouble, {}, &time, sizeof(double))) return 1;
   return 0;
 }
 
@@ -1514,10 +1696,10 @@ extern "C" int vof_copy_probe(domain_t * domain, size_t n, int reps, double * gb
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *gbps = 2. * sizeof(double) * (double)n * reps / (ms * 1e-3) / 1e9;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(a);
+  dev_free(b);
   return 0;
 }
 
@@ -1652,7 +1834,7 @@ static int tdm_solve(const tdm_info_t * info, void * data) {
 static int tdm_destruct(tdm_info_t * info) {
   if (!info) return fail("ERROR(destruct): info is NULL");
   free(info->l); free(info->c); free(info->u);
-  hipFree(info->d_luc); hipFree(info->d_v); hipFree(info->d_q1);
+  dev_free(info->d_luc); dev_free(info->d_v); dev_free(info->d_q1);
   delete info;
   return 0;
 }
@@ -1694,7 +1876,7 @@ extern "C" int linear_system_init(const void * info, const bool implicit[NDIMS],
 
 extern "C" int linear_system_finalise(linear_system_t * ls) {
   if (!ls || !ls->is_initialised) return 1;
-  hipFree(ls->x1pncl);
+  dev_free(ls->x1pncl);
   if (ls->implicit[0]) tdm_destruct(ls->tdm_x);
   ls->is_initialised = false;
   return 0;

@@ -77,6 +77,9 @@ def load(ndims: int):
     lib.orc_indicator.argtypes = [_dp, _dp]
     lib.orc_vof_stats.argtypes = [vp, _dp, _dp, _dp]
     lib.orc_energy.argtypes = [vp, _dp]
+    lib.orc_momentum.argtypes = [vp, _dp]
+    lib.orc_interface_area.argtypes = [vp]
+    lib.orc_interface_area.restype = C.c_double
     _LIBS[ndims] = lib
     return lib
 
@@ -203,6 +206,14 @@ class Oracle:
         out = np.zeros(self.ndims + 1)
         self.lib.orc_energy(self.h, _ptr(out))
         return out
+
+    def momentum(self):
+        out = np.zeros(self.ndims)
+        self.lib.orc_momentum(self.h, _ptr(out))
+        return out
+
+    def interface_area(self):
+        return self.lib.orc_interface_area(self.h)
 
 
 # free-standing numerics -----------------------------------------------------

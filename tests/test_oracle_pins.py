@@ -475,3 +475,37 @@ def test_3d_z_uniform_reproduces_2d_step():
         for kk in range(hy, nz + hy):
             assert np.max(np.abs(a3[kk].ravel() - a2)) < 1e-13 * max(1e-300, np.max(np.abs(a2))), name
     assert np.max(np.abs(o3.get("uz"))) < 1e-13
+
+
+# --- diagnostics: momentum and interface measure ---------------------------------
+
+@pytest.mark.parametrize("uniformx", [True, False])
+def test_momentum_of_uniform_velocities(uniformx):
+    """x momentum lives on the faces i = 2..nx (width DXC), y momentum on the
+    cell centres (width DXF): a stretched grid tells the two apart."""
+    nx, ny, lx, ly = 16, 8, 1.0, 2.0
+    ux = np.zeros((ny, nx + 1))
+    ux[:, 1:-1] = 0.7
+    uy = np.full((ny, nx + 2), -0.3)
+    o = Oracle(_case(nx, ny, {"ux": ux, "uy": uy}, lx=lx, ly=ly, uniformx=uniformx))
+    _, xc = make_xgrid(lx, nx, uniformx)
+    m = o.momentum()
+    assert abs(m[0] - 0.7 * (xc[nx] - xc[1]) * ly) < 1e-14
+    assert abs(m[1] - (-0.3) * lx * ly) < 1e-14
+
+
+def test_interface_measure_of_planar_and_circular_interfaces():
+    """Central differences telescope across a monotone jump, so a slab with
+    two planar interfaces measures exactly 2 ly; a disc of radius R measures
+    its perimeter 2 pi R to discretisation accuracy."""
+    nx, ny, ly = 16, 8, 1.5
+    v = np.zeros((ny, nx + 2))
+    v[:, 3] = 0.6
+    v[:, 4:9] = 1.0
+    v[:, 9] = 0.3
+    o = Oracle(_case(nx, ny, {"vof": v}, ly=ly))
+    assert abs(o.interface_area() - 2 * ly) < 1e-13
+    n, R = 128, 0.25
+    xf, _ = make_xgrid(1.0, n)
+    o2 = Oracle(_case(n, n, {"vof": _pad_cells(vof_discs(xf, n, 1.0, [(0.5, 0.5, R)]))}))
+    assert abs(o2.interface_area() - 2 * np.pi * R) < 0.03 * 2 * np.pi * R
