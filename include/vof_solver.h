@@ -260,6 +260,26 @@ int vof_set_fusion(domain_t * domain, int mask);
 int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
                     const interface_t * interface, double * out);
 
+/* Nusselt numbers (logging/nusselt/main.c:18-50): out = {wall heat flux,
+ * kinetic energy injection, kinetic energy dissipation, thermal
+ * dissipation}; 3D adds the z terms and normalises by t_dif * ly * lz */
+int vof_nusselt(const domain_t * domain, const fluid_t * fluid, double out[4]);
+
+/* ---- statistics (replaces include/statistics.h:7-30) ---------------------
+ * init reads stat_rate and stat_after from the environment; collect adds
+ * one sample of ux, ux^2, uy, uy^2, t, t^2, ux t (3D: uz, uz^2) on the
+ * device; output writes output/stat/step%010zu with the y (and z) sums as
+ * 1D NPY profiles, num, m_dif, t_dif and the domain */
+typedef struct {
+  int (* const init)(const domain_t * domain, const double time);
+  int (* const collect)(const domain_t * domain, const fluid_t * fluid);
+  int (* const output)(const domain_t * domain, const size_t step);
+  double (* const get_next_time)(void);
+} statistics_t;
+extern const statistics_t statistics;
+/* statistics.output into any directory */
+int vof_statistics_output_to(const domain_t * domain, const char dirname[]);
+
 /* tile parameters (tuned by simplevofsolver_amd.tune): "fft_elems" = 1024 or
  * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
  * per work-group of the predictor's x solver (0 = automatic) */

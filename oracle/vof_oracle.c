@@ -1865,3 +1865,263 @@ double orc_interface_area(const orc_t * o) {
   }
   return s;
 }
+
+/* ---- logging/nusselt (heat_flux.c, kinetic_energy_*.c, thermal_*.c): four Nusselt numbers ---------------------------
+ * 2D as the reference; 3D adds the z terms of each integral and the
+ * reference flux t_dif * ly * lz.  out = {heat flux on the walls, kinetic
+ * energy injection, kinetic energy dissipation, thermal dissipation}. */
+void orc_nusselt(const orc_t * o, double * out) {
+  SIZES;
+  const double * dxf = o->dxf, * dxc = o->dxc;
+  const double dy = o->dy, dz = o->dz;
+  (void)dz;
+  const double * ux = o->ux, * uy = o->uy, * t = o->t;
+#if NDIMS == 3
+  const double * uz = o->uz;
+  const double dyz = dy * dz;
+  const double ref = o->t_dif * o->lengths[1] * o->lengths[2];
+#else
+  const double dyz = dy;
+  const double ref = o->t_dif * o->lengths[1];
+#endif
+  const double md = o->m_dif, td = o->t_dif;
+  /* heat_flux.c:20-45 */
+  double hf = 0.;
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) {
+    const double dtdx_xm = (+T(1, j, k) - T(0, j, k)) / DXC(1);
+    const double dtdx_xp = (+T(isize + 1, j, k) - T(isize, j, k)) / DXC(isize + 1);
+    hf -= 0.5 * td * (dtdx_xm + dtdx_xp) * dyz;
+  }
+  out[0] = hf / ref;
+  /* kinetic_energy_injection.c:14-45 */
+  double inj = 0.;
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 2; i <= isize; i++) {
+    const double t_ = +0.5 * T(i - 1, j, k) + 0.5 * T(i, j, k);
+    inj += UX(i, j, k) * t_ * (DXC(i) * dyz);
+  }
+  out[1] = inj / ref + 1.;
+  /* kinetic_energy_dissipation.c:14-160, one accumulator per term */
+  double kd = 0.;
+  {
+    double s = 0.;   /* ux, x */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize + 1; i++) {
+      const double cs = DXC(i) * dyz;
+      if (1 != i) {
+        const double d = 1. / DXF(i - 1) * (UX(i, j, k) - UX(i - 1, j, k));
+        s += md * 0.5 * (DXF(i - 1) / DXC(i)) * (d * d) * cs;
+      }
+      if (isize + 1 != i) {
+        const double d = 1. / DXF(i) * (UX(i + 1, j, k) - UX(i, j, k));
+        s += md * 0.5 * (DXF(i) / DXC(i)) * (d * d) * cs;
+      }
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* ux, y */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize + 1; i++) {
+      const double cs = DXC(i) * dyz;
+      const double dm = 1. / dy * (UX(i, j, k) - UX(i, j - 1, k));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dy * (UX(i, j + 1, k) - UX(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uy, x */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / DXC(i) * (UY(i, j, k) - UY(i - 1, j, k));
+      s += md * 0.5 * (DXC(i) / DXF(i)) * (1 == i ? 2. : 1.) * (dm * dm) * cs;
+      const double dp = 1. / DXC(i + 1) * (UY(i + 1, j, k) - UY(i, j, k));
+      s += md * 0.5 * (DXC(i + 1) / DXF(i)) * (isize == i ? 2. : 1.) * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uy, y */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dy * (UY(i, j, k) - UY(i, j - 1, k));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dy * (UY(i, j + 1, k) - UY(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+#if NDIMS == 3
+  {
+    double s = 0.;   /* ux, z */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize + 1; i++) {
+      const double cs = DXC(i) * dyz;
+      const double dm = 1. / dz * (UX(i, j, k) - UX(i, j, k - 1));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dz * (UX(i, j, k + 1) - UX(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uy, z */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dz * (UY(i, j, k) - UY(i, j, k - 1));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dz * (UY(i, j, k + 1) - UY(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uz, x */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / DXC(i) * (UZ(i, j, k) - UZ(i - 1, j, k));
+      s += md * 0.5 * (DXC(i) / DXF(i)) * (1 == i ? 2. : 1.) * (dm * dm) * cs;
+      const double dp = 1. / DXC(i + 1) * (UZ(i + 1, j, k) - UZ(i, j, k));
+      s += md * 0.5 * (DXC(i + 1) / DXF(i)) * (isize == i ? 2. : 1.) * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uz, y */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dy * (UZ(i, j, k) - UZ(i, j - 1, k));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dy * (UZ(i, j + 1, k) - UZ(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+  {
+    double s = 0.;   /* uz, z */
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dz * (UZ(i, j, k) - UZ(i, j, k - 1));
+      s += md * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dz * (UZ(i, j, k + 1) - UZ(i, j, k));
+      s += md * 0.5 * (dp * dp) * cs;
+    }
+    kd += s;
+  }
+#endif
+  out[2] = kd / ref + 1.;
+  /* thermal_energy_dissipation.c:14-90 */
+  double th = 0.;
+  {
+    double s = 0.;
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / DXC(i) * (T(i, j, k) - T(i - 1, j, k));
+      s += td * 0.5 * (DXC(i) / DXF(i)) * (1 == i ? 2. : 1.) * (dm * dm) * cs;
+      const double dp = 1. / DXC(i + 1) * (T(i + 1, j, k) - T(i, j, k));
+      s += td * 0.5 * (DXC(i + 1) / DXF(i)) * (isize == i ? 2. : 1.) * (dp * dp) * cs;
+    }
+    th += s;
+  }
+  {
+    double s = 0.;
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dy * (T(i, j, k) - T(i, j - 1, k));
+      s += td * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dy * (T(i, j + 1, k) - T(i, j, k));
+      s += td * 0.5 * (dp * dp) * cs;
+    }
+    th += s;
+  }
+#if NDIMS == 3
+  {
+    double s = 0.;
+    KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) for (int i = 1; i <= isize; i++) {
+      const double cs = DXF(i) * dyz;
+      const double dm = 1. / dz * (T(i, j, k) - T(i, j, k - 1));
+      s += td * 0.5 * (dm * dm) * cs;
+      const double dp = 1. / dz * (T(i, j, k + 1) - T(i, j, k));
+      s += td * 0.5 * (dp * dp) * cs;
+    }
+    th += s;
+  }
+#endif
+  out[3] = th / ref;
+}
+
+/* ---- statistics.c:117-223: time-averaging accumulators -------------------
+ * ux1 ux2 uxt on the x faces i = 1..nx+1, uy1 uy2 (uz1 uz2) t1 t2 on the
+ * cells i = 0..nx+1, all interior rows; reduce sums over y (and z) as
+ * statistics.c:232-275 */
+struct orc_stats {
+  size_t num;
+  int wf, wc;        /* row widths: nx + 1 faces, nx + 2 cells */
+  size_t rows;       /* ny (* nz) */
+  double * a[9];     /* ux1 ux2 uy1 uy2 t1 t2 uxt uz1 uz2 */
+};
+
+static const char * const kStatNames[9] = {"ux1", "ux2", "uy1", "uy2", "t1", "t2", "uxt", "uz1", "uz2"};
+
+orc_stats_t * orc_stats_create(const orc_t * o) {
+  orc_stats_t * s = calloc(1, sizeof(orc_stats_t));
+  s->wf = o->isize + 1;
+  s->wc = o->isize + 2;
+  s->rows = (size_t)o->jsize * (size_t)o->ksize;
+  for (int n = 0; n < 9; n++) {
+    const int w = (n == 0 || n == 1 || n == 6) ? s->wf : s->wc;
+    s->a[n] = calloc((size_t)w * s->rows, sizeof(double));
+  }
+  return s;
+}
+
+void orc_stats_destroy(orc_stats_t * s) {
+  if (!s) return;
+  for (int n = 0; n < 9; n++) free(s->a[n]);
+  free(s);
+}
+
+void orc_stats_collect(orc_stats_t * s, const orc_t * o) {
+  SIZES;
+  const double * ux = o->ux, * uy = o->uy, * t = o->t;
+#if NDIMS == 3
+  const double * uz = o->uz;
+#endif
+  KLOOP(1, ksize) for (int j = 1; j <= jsize; j++) {
+    const size_t r = (size_t)(j - 1) + (size_t)jsize * (size_t)(k - 1);
+    for (int i = 1; i <= isize + 1; i++) {
+      const double u = UX(i, j, k);
+      s->a[0][(size_t)(i - 1) + s->wf * r] += u;
+      s->a[1][(size_t)(i - 1) + s->wf * r] += u * u;
+      const double t_ = +0.5 * T(i - 1, j, k) + 0.5 * T(i, j, k);
+      s->a[6][(size_t)(i - 1) + s->wf * r] += u * t_;
+    }
+    for (int i = 0; i <= isize + 1; i++) {
+      const double v = UY(i, j, k);
+      s->a[2][(size_t)i + s->wc * r] += v;
+      s->a[3][(size_t)i + s->wc * r] += v * v;
+      const double tt = T(i, j, k);
+      s->a[4][(size_t)i + s->wc * r] += tt;
+      s->a[5][(size_t)i + s->wc * r] += tt * tt;
+#if NDIMS == 3
+      const double w = UZ(i, j, k);
+      s->a[7][(size_t)i + s->wc * r] += w;
+      s->a[8][(size_t)i + s->wc * r] += w * w;
+#endif
+    }
+  }
+  s->num += 1;
+}
+
+size_t orc_stats_num(const orc_stats_t * s) { return s->num; }
+
+/* sum over the rows in row order; returns the profile length or 0 */
+int orc_stats_reduce(const orc_stats_t * s, const char * name, double * out) {
+  for (int n = 0; n < 9; n++) {
+    if (strcmp(name, kStatNames[n])) continue;
+    const int w = (n == 0 || n == 1 || n == 6) ? s->wf : s->wc;
+    for (int i = 0; i < w; i++) out[i] = 0.;
+    for (size_t r = 0; r < s->rows; r++)
+      for (int i = 0; i < w; i++) out[i] += s->a[n][(size_t)i + (size_t)w * r];
+    return w;
+  }
+  return 0;
+}

@@ -86,5 +86,14 @@ void orc_vof_stats(const orc_t * o, double * vmin, double * vmax, double * volum
 void orc_energy(const orc_t * o, double * out /* NDIMS + 1 */);
 void orc_momentum(const orc_t * o, double * out /* NDIMS */);
 double orc_interface_area(const orc_t * o);
+void orc_nusselt(const orc_t * o, double * out /* 4 */);
+
+/* statistics accumulators (statistics.c) */
+typedef struct orc_stats orc_stats_t;
+orc_stats_t * orc_stats_create(const orc_t * o);
+void orc_stats_destroy(orc_stats_t * s);
+void orc_stats_collect(orc_stats_t * s, const orc_t * o);
+size_t orc_stats_num(const orc_stats_t * s);
+int orc_stats_reduce(const orc_stats_t * s, const char * name, double * out);
 
 #endif

@@ -75,7 +75,8 @@ ABI_SYMBOLS_COMMON = [
     "vof_array_from_host", "vof_synchronize", "interface_finalise", "fluid_finalise",
     "domain_finalise", "vof_set_schedule", "vof_diagnostics", "vof_profile",
     "vof_profile_reset", "vof_profile_report", "vof_copy_probe", "vof_set_fusion",
-    "vof_read_step_time", "vof_write_step_time", "vof_set_tuning",
+    "vof_read_step_time", "vof_write_step_time", "vof_set_tuning", "vof_nusselt",
+    "statistics", "vof_statistics_output_to",
 ]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
 
@@ -130,6 +131,8 @@ def load(ndims: int):
     lib.vof_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, dp]
     lib.vof_set_fusion.argtypes = [vp, C.c_int]
     lib.vof_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
+    lib.vof_nusselt.argtypes = [vp, vp, dp]
+    lib.vof_statistics_output_to.argtypes = [vp, C.c_char_p]
     lib.vof_read_step_time.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), dp]
     lib.vof_write_step_time.argtypes = [C.c_char_p, C.c_size_t, C.c_double]
     lib._structs = (Domain, Fluid, Interface)
@@ -143,6 +146,16 @@ class VofError(RuntimeError):
 
 PARAM_NAMES = ("Ra", "Pr", "We", "coef_dt_adv", "coef_dt_dif", "add_buoyancy", "uy_xm",
                "uy_xp", "uz_xm", "uz_xp", "t_xm", "t_xp")
+
+
+class _StatisticsVT(C.Structure):
+    """statistics_t (include/statistics.h:7-30): four function pointers."""
+    _fields_ = [
+        ("init", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)),
+        ("collect", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)),
+        ("output", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t)),
+        ("get_next_time", C.CFUNCTYPE(C.c_double)),
+    ]
 
 
 class Solver:
@@ -274,6 +287,29 @@ class Solver:
 
     def set_fusion(self, mask: int):
         self._chk(self.lib.vof_set_fusion(C.byref(self.domain), int(mask)), "vof_set_fusion")
+
+    def nusselt(self) -> dict:
+        """logging/nusselt/main.c:18-50 on the device."""
+        out = (C.c_double * 4)()
+        self._chk(self.lib.vof_nusselt(C.byref(self.domain), C.byref(self.fluid), out), "vof_nusselt")
+        return dict(zip(["heat_flux", "ke_injection", "ke_dissipation", "thermal_dissipation"], out))
+
+    def statistics_init(self, time: float = 0.0):
+        """statistics.init (statistics.c:59-100); reads stat_rate and
+        stat_after from the environment."""
+        vt = _StatisticsVT.in_dll(self.lib, "statistics")
+        self._chk(vt.init(C.byref(self.domain), time), "statistics.init")
+
+    def statistics_collect(self):
+        vt = _StatisticsVT.in_dll(self.lib, "statistics")
+        self._chk(vt.collect(C.byref(self.domain), C.byref(self.fluid)), "statistics.collect")
+
+    def statistics_next_time(self) -> float:
+        return _StatisticsVT.in_dll(self.lib, "statistics").get_next_time()
+
+    def statistics_output(self, dirname: str):
+        self._chk(self.lib.vof_statistics_output_to(C.byref(self.domain), dirname.encode()),
+                  "vof_statistics_output_to")
 
     def set_tuning(self, name: str, value: int):
         self._chk(self.lib.vof_set_tuning(C.byref(self.domain), name.encode(), int(value)), "vof_set_tuning")

@@ -56,7 +56,7 @@ def build_driver(ndims: int, verbose: bool = False) -> str:
     against the library of the same dimensionality."""
     out = driver_path(ndims)
     cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", f"-DNDIMS={ndims}", "-D_POSIX_C_SOURCE=200809L",
-           "-I", INCLUDE, os.path.join(CSRC, "vof_driver.c"), "-L", PKG, f"-lvofhip{ndims}d",
+           "-I", INCLUDE, os.path.join(CSRC, "vof_driver.c"), "-L", PKG, f"-lvofhip{ndims}d", "-lm",
            f"-Wl,-rpath,$ORIGIN", "-o", out]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -1680,6 +1680,159 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     for (int m = 0; m < kDiagN; m++) part[(size_t)blockIdx.x * kDiagN + m] = red[m][0];
 }
 
+// logging/nusselt: partial sums of {wall heat flux, kinetic energy
+// injection, kinetic energy dissipation, thermal dissipation} over the x
+// faces i = 1..nx+1 of every interior row (cell terms use i <= nx); 3D adds
+// the z terms of each integral
+constexpr int kNuN = 4;
+
+__device__ inline double sq(double a) { return a * a; }
+
+__global__ __launch_bounds__(kRedThreads) void k_nusselt_partial(
+    Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+    const double * __restrict__ uz, const double * __restrict__ t, double md, double td,
+    double * __restrict__ part) {
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 2
+  const size_t nrow = (size_t)ny;
+  const double dyz = g.dy;
+  (void)uz;
+#else
+  const size_t nrow = (size_t)ny * g.nz;
+  const double dyz = g.dy * g.dz;
+  const double dz = g.dz;
+#endif
+  const double dy = g.dy;
+  const double * dxf = g.dxf;
+  const double * dxc = g.dxc;
+  double q[kNuN] = {0., 0., 0., 0.};
+  const size_t n = (size_t)(nx + 1) * nrow;
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(s % (size_t)(nx + 1)) + 1;
+    const size_t r = s / (size_t)(nx + 1);
+    const int j = (int)(r % ny) + 1;
+#if NDIMS == 2
+    const int k = 1;
+#else
+    const int k = (int)(r / ny) + 1;
+#endif
+    // DXC(i) = dxc[i - 1], DXF(i) = dxf[i - 1]
+    if (i == 1) {
+      const double dtdx_xm = (+t[I_C1(1, j, k)] - t[I_C1(0, j, k)]) / dxc[0];
+      const double dtdx_xp = (+t[I_C1(nx + 1, j, k)] - t[I_C1(nx, j, k)]) / dxc[nx];
+      q[0] -= 0.5 * td * (dtdx_xm + dtdx_xp) * dyz;
+    }
+    if (i >= 2 && i <= nx) {
+      const double t_ = +0.5 * t[I_C1(i - 1, j, k)] + 0.5 * t[I_C1(i, j, k)];
+      q[1] += ux[I_UX(i, j, k)] * t_ * (dxc[i - 1] * dyz);
+    }
+    {
+      // ux terms on the faces i = 1..nx+1
+      const double cs = dxc[i - 1] * dyz;
+      const double uc = ux[I_UX(i, j, k)];
+      if (i != 1) q[2] += md * 0.5 * (dxf[i - 2] / dxc[i - 1]) * sq(1. / dxf[i - 2] * (uc - ux[I_UX(i - 1, j, k)])) * cs;
+      if (i != nx + 1) q[2] += md * 0.5 * (dxf[i - 1] / dxc[i - 1]) * sq(1. / dxf[i - 1] * (ux[I_UX(i + 1, j, k)] - uc)) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (uc - ux[I_UX(i, j - 1, k)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (ux[I_UX(i, j + 1, k)] - uc)) * cs;
+#if NDIMS == 3
+      q[2] += md * 0.5 * sq(1. / dz * (uc - ux[I_UX(i, j, k - 1)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dz * (ux[I_UX(i, j, k + 1)] - uc)) * cs;
+#endif
+    }
+    if (i <= nx) {
+      const double cs = dxf[i - 1] * dyz;
+      const double w1m = 1 == i ? 2. : 1.;
+      const double w1p = nx == i ? 2. : 1.;
+      const double wm = dxc[i - 1] / dxf[i - 1];
+      const double wp = dxc[i] / dxf[i - 1];
+      // uy (and uz) terms on the cell centres
+      const double vc = uy[I_C1(i, j, k)];
+      q[2] += md * 0.5 * wm * w1m * sq(1. / dxc[i - 1] * (vc - uy[I_C1(i - 1, j, k)])) * cs;
+      q[2] += md * 0.5 * wp * w1p * sq(1. / dxc[i] * (uy[I_C1(i + 1, j, k)] - vc)) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (vc - uy[I_C1(i, j - 1, k)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (uy[I_C1(i, j + 1, k)] - vc)) * cs;
+#if NDIMS == 3
+      q[2] += md * 0.5 * sq(1. / dz * (vc - uy[I_C1(i, j, k - 1)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dz * (uy[I_C1(i, j, k + 1)] - vc)) * cs;
+      const double wc = uz[I_C1(i, j, k)];
+      q[2] += md * 0.5 * wm * w1m * sq(1. / dxc[i - 1] * (wc - uz[I_C1(i - 1, j, k)])) * cs;
+      q[2] += md * 0.5 * wp * w1p * sq(1. / dxc[i] * (uz[I_C1(i + 1, j, k)] - wc)) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (wc - uz[I_C1(i, j - 1, k)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dy * (uz[I_C1(i, j + 1, k)] - wc)) * cs;
+      q[2] += md * 0.5 * sq(1. / dz * (wc - uz[I_C1(i, j, k - 1)])) * cs;
+      q[2] += md * 0.5 * sq(1. / dz * (uz[I_C1(i, j, k + 1)] - wc)) * cs;
+#endif
+      const double tc = t[I_C1(i, j, k)];
+      q[3] += td * 0.5 * wm * w1m * sq(1. / dxc[i - 1] * (tc - t[I_C1(i - 1, j, k)])) * cs;
+      q[3] += td * 0.5 * wp * w1p * sq(1. / dxc[i] * (t[I_C1(i + 1, j, k)] - tc)) * cs;
+      q[3] += td * 0.5 * sq(1. / dy * (tc - t[I_C1(i, j - 1, k)])) * cs;
+      q[3] += td * 0.5 * sq(1. / dy * (t[I_C1(i, j + 1, k)] - tc)) * cs;
+#if NDIMS == 3
+      q[3] += td * 0.5 * sq(1. / dz * (tc - t[I_C1(i, j, k - 1)])) * cs;
+      q[3] += td * 0.5 * sq(1. / dz * (t[I_C1(i, j, k + 1)] - tc)) * cs;
+#endif
+    }
+  }
+  __shared__ double red[kNuN][kRedThreads];
+  for (int m = 0; m < kNuN; m++) red[m][threadIdx.x] = q[m];
+  __syncthreads();
+  for (int st = kRedThreads / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int m = 0; m < kNuN; m++) red[m][threadIdx.x] += red[m][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int m = 0; m < kNuN; m++) part[(size_t)blockIdx.x * kNuN + m] = red[m][0];
+}
+
+// statistics.c:117-197: add one sample to the accumulators; faces
+// i = 1..nx+1 (ux1 ux2 uxt, row width nx+1), cells i = 0..nx+1 (uy1 uy2 t1
+// t2 [uz1 uz2], row width nx+2); rows are (j, k) in y-fastest order
+struct StatAcc {
+  double * a[9];   // ux1 ux2 uy1 uy2 t1 t2 uxt uz1 uz2
+};
+
+__global__ void k_stats_collect(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                                const double * __restrict__ uz, const double * __restrict__ t, StatAcc acc) {
+  const int nx = g.nx, ny = g.ny;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);   // 0..nx+1
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;
+  const int k = (int)blockIdx.z + 1;
+  if (i > nx + 1 || j > ny) return;
+  const size_t r = (size_t)(j - 1) + (size_t)ny * (size_t)(k - 1);
+  if (i >= 1) {
+    const size_t e = (size_t)(i - 1) + (size_t)(nx + 1) * r;
+    const double u = ux[I_UX(i, j, k)];
+    acc.a[0][e] += u;
+    acc.a[1][e] += u * u;
+    const double t_ = +0.5 * t[I_C1(i - 1, j, k)] + 0.5 * t[I_C1(i, j, k)];
+    acc.a[6][e] += u * t_;
+  }
+  const size_t e = (size_t)i + (size_t)(nx + 2) * r;
+  const double v = uy[I_C1(i, j, k)];
+  acc.a[2][e] += v;
+  acc.a[3][e] += v * v;
+  const double tt = t[I_C1(i, j, k)];
+  acc.a[4][e] += tt;
+  acc.a[5][e] += tt * tt;
+#if NDIMS == 3
+  const double w = uz[I_C1(i, j, k)];
+  acc.a[7][e] += w;
+  acc.a[8][e] += w * w;
+#else
+  (void)uz;
+#endif
+}
+
+// statistics.c:232-275: sum over the rows in row order, one lane per x entry
+__global__ void k_stats_reduce(const double * __restrict__ a, int w, size_t rows, double * __restrict__ out) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= w) return;
+  double s = 0.;
+  for (size_t r = 0; r < rows; r++) s += a[(size_t)i + (size_t)w * r];
+  out[i] = s;
+}
+
 // plain copy, the calibration kernel for hardware byte counters
 __global__ void k_copy(const double * __restrict__ a, double * __restrict__ b, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)

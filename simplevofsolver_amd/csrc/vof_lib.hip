@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <errno.h>
+#include <sys/stat.h>
 
 #include <string>
 #include <vector>
@@ -241,6 +242,10 @@ struct vof_ctx {
   double * d_part;
   int npart;
   double * h_scalar;                     // pinned
+  // statistics accumulators (statistics.c), allocated by statistics.init
+  double * d_stat[9];
+  size_t stat_num;
+  double stat_rate, stat_next, stat_m_dif, stat_t_dif;
   // per-kernel timing with HIP events on this context's stream
   bool prof_on;
   std::vector<hipEvent_t> ev_pool;
@@ -815,6 +820,7 @@ extern "C" int domain_finalise(domain_t * domain) {
     dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez);
     dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
     if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
+    for (int m = 0; m < 9; m++) dev_free(ctx->d_stat[m]);
     host_free(ctx->h_scalar);
     prof_flush(ctx);
     for (auto & e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1615,6 +1621,153 @@ extern "C" int vof_diagnostics(const domain_t * domain, const fluid_t * fluid,
   out[o++] = q[12];
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Nusselt numbers (logging/nusselt/main.c:18-50): {wall heat flux, kinetic
+// energy injection, kinetic energy dissipation, thermal dissipation}, each
+// normalised by the conductive flux t_dif * ly (* lz)
+
+extern "C" int vof_nusselt(const domain_t * domain, const fluid_t * fluid, double out[4]) {
+  CTX_OR_FAIL(domain);
+  const int nb = 256;
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+  const double ref = fluid->t_dif * domain->lengths[1] * domain->lengths[2];
+#else
+  const double * uz = nullptr;
+  const double ref = fluid->t_dif * domain->lengths[1];
+#endif
+  KLN(ctx, "k_nusselt_partial", vofk::k_nusselt_partial, dim3(nb), dim3(vofk::kRedThreads), ctx->grid,
+      (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz, (const double *)fluid->t.data,
+      fluid->m_dif, fluid->t_dif, ctx->d_part);
+  std::vector<double> h((size_t)nb * vofk::kNuN);
+  HIPCHK(hipMemcpyAsync(h.data(), ctx->d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double q[vofk::kNuN] = {0., 0., 0., 0.};
+  for (int b = 0; b < nb; b++)
+    for (int m = 0; m < vofk::kNuN; m++) q[m] += h[(size_t)b * vofk::kNuN + m];
+  out[0] = q[0] / ref;
+  out[1] = q[1] / ref + 1.;
+  out[2] = q[2] / ref + 1.;
+  out[3] = q[3] / ref;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// statistics (statistics.c): device accumulators of ux, ux^2, uy, uy^2, t,
+// t^2, ux t (and uz, uz^2 in 3D), reduced over y (and z) on output.  Like the
+// reference module this is one collector per process: get_next_time reports
+// the domain that ran init last.
+
+static const char * const kStatNames[9] = {"ux1", "ux2", "uy1", "uy2", "t1", "t2", "uxt", "uz1", "uz2"};
+static double g_stat_next = 0.;
+
+static int stat_width(const vof_ctx * ctx, int m) {
+  return (m == 0 || m == 1 || m == 6) ? ctx->nx + 1 : ctx->nx + 2;
+}
+
+static int stats_init(const domain_t * domain, const double time) {
+  CTX_OR_FAIL(domain);
+  // config.get_double("stat_rate" / "stat_after") (statistics.c:63-70)
+  double rate = 0., after = 0.;
+  for (int m = 0; m < 2; m++) {
+    const char * name = m == 0 ? "stat_rate" : "stat_after";
+    const char * sv = getenv(name);
+    if (!sv) return fail("%s not found", name);
+    errno = 0;
+    const double v = strtod(sv, nullptr);
+    if (errno != 0) return fail("%s: invalid value as double", name);
+    (m == 0 ? rate : after) = v;
+  }
+  if (!(rate > 0.)) return fail("stat_rate must be positive");
+  ctx->stat_rate = rate;
+  ctx->stat_next = rate * ceil(fmax(DBL_EPSILON, fmax(time, after)) / rate);
+  g_stat_next = ctx->stat_next;
+  const size_t rows = (size_t)ctx->ny * (size_t)ctx->nz;
+  for (int m = 0; m < 9; m++) {
+    if (NDIMS == 2 && m >= 7) continue;
+    const size_t bytes = sizeof(double) * (size_t)stat_width(ctx, m) * rows;
+    if (!ctx->d_stat[m]) HIPCHK(hipMalloc(&ctx->d_stat[m], bytes));
+    HIPCHK(hipMemsetAsync(ctx->d_stat[m], 0, bytes, ctx->stream));
+  }
+  ctx->stat_num = 0;
+  return 0;
+}
+
+static int stats_collect(const domain_t * domain, const fluid_t * fluid) {
+  CTX_OR_FAIL(domain);
+  if (!ctx->d_stat[0]) return fail("statistics.collect before statistics.init");
+  vofk::StatAcc acc;
+  for (int m = 0; m < 9; m++) acc.a[m] = ctx->d_stat[m];
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+#else
+  const double * uz = nullptr;
+#endif
+  KLN(ctx, "k_stats_collect", vofk::k_stats_collect, grid3(ctx->nx + 2, ctx->ny, ctx->nz, kBlk), kBlk,
+      ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz,
+      (const double *)fluid->t.data, acc);
+  ctx->stat_m_dif = fluid->m_dif;
+  ctx->stat_t_dif = fluid->t_dif;
+  ctx->stat_num += 1;
+  ctx->stat_next += ctx->stat_rate;
+  g_stat_next = ctx->stat_next;
+  return 0;
+}
+
+static int mkdir_p(const char * dir) {
+  std::string p(dir);
+  for (size_t pos = 1; pos <= p.size(); pos++) {
+    if (pos == p.size() || p[pos] == '/') {
+      const std::string sub = p.substr(0, pos);
+      if (mkdir(sub.c_str(), 0777) != 0 && errno != EEXIST) return fail("mkdir %s failed", sub.c_str());
+    }
+  }
+  return 0;
+}
+
+// write the collected statistics into dirname (statistics.c:285-330)
+extern "C" int vof_statistics_output_to(const domain_t * domain, const char dirname[]) {
+  CTX_OR_FAIL(domain);
+  if (ctx->stat_num == 0) return 0;
+  if (mkdir_p(dirname)) return 1;
+  if (npy::write(dirname, "num", kNpySizeT, {}, &ctx->stat_num, sizeof(size_t))) return 1;
+  if (npy::write(dirname, "m_dif", kNpyDouble, {}, &ctx->stat_m_dif, sizeof(double))) return 1;
+  if (npy::write(dirname, "t_dif", kNpyDouble, {}, &ctx->stat_t_dif, sizeof(double))) return 1;
+  if (domain_save(dirname, domain)) return 1;
+  const size_t rows = (size_t)ctx->ny * (size_t)ctx->nz;
+  double * d_vec = ctx->d_part;   // nx + 2 <= npart * kDiagN entries
+  if ((size_t)(ctx->nx + 2) > (size_t)ctx->npart * vofk::kDiagN) return fail("statistics: profile too long");
+  std::vector<double> h((size_t)ctx->nx + 2);
+  for (int m = 0; m < 9; m++) {
+    if (NDIMS == 2 && m >= 7) continue;
+    const int w = stat_width(ctx, m);
+    KLN(ctx, "k_stats_reduce", vofk::k_stats_reduce, dim3((w + 63) / 64), dim3(64), ctx->d_stat[m], w, rows, d_vec);
+    HIPCHK(hipMemcpyAsync(h.data(), d_vec, sizeof(double) * (size_t)w, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (npy::write(dirname, kStatNames[m], kNpyDouble, {(size_t)w}, h.data(), sizeof(double))) return 1;
+  }
+  return 0;
+}
+
+static int stats_output(const domain_t * domain, const size_t step) {
+  // statistics.c:291-299: output/stat/step%010zu
+  char dir[64];
+  snprintf(dir, sizeof dir, "output/stat/step%010zu", step);
+  return vof_statistics_output_to(domain, dir);
+}
+
+static double stats_get_next_time(void) { return g_stat_next; }
+
+// host-only object: the device pass must not see a table of host functions
+#if !defined(__HIP_DEVICE_COMPILE__)
+extern "C" const statistics_t statistics = {
+    stats_init,
+    stats_collect,
+    stats_output,
+    stats_get_next_time,
+};
+#endif
 
 // ---------------------------------------------------------------------------
 // step / time scalars of an IC or save directory (src/main.c:64-71, 26-27)

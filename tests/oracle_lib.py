@@ -80,6 +80,14 @@ def load(ndims: int):
     lib.orc_momentum.argtypes = [vp, _dp]
     lib.orc_interface_area.argtypes = [vp]
     lib.orc_interface_area.restype = C.c_double
+    lib.orc_nusselt.argtypes = [vp, _dp]
+    lib.orc_stats_create.argtypes = [vp]
+    lib.orc_stats_create.restype = vp
+    lib.orc_stats_destroy.argtypes = [vp]
+    lib.orc_stats_collect.argtypes = [vp, vp]
+    lib.orc_stats_num.argtypes = [vp]
+    lib.orc_stats_num.restype = C.c_size_t
+    lib.orc_stats_reduce.argtypes = [vp, C.c_char_p, _dp]
     _LIBS[ndims] = lib
     return lib
 
@@ -214,6 +222,38 @@ class Oracle:
 
     def interface_area(self):
         return self.lib.orc_interface_area(self.h)
+
+    def nusselt(self):
+        out = np.zeros(4)
+        self.lib.orc_nusselt(self.h, _ptr(out))
+        return out
+
+
+class OracleStats:
+    """statistics.c accumulators on the CPU."""
+
+    def __init__(self, o: Oracle):
+        self.o = o
+        self.h = o.lib.orc_stats_create(o.h)
+
+    def collect(self):
+        self.o.lib.orc_stats_collect(self.h, self.o.h)
+
+    @property
+    def num(self):
+        return self.o.lib.orc_stats_num(self.h)
+
+    def reduce(self, name):
+        nx = self.o.glsizes[0]
+        out = np.zeros(nx + 2)
+        w = self.o.lib.orc_stats_reduce(self.h, name.encode(), _ptr(out))
+        assert w > 0, name
+        return out[:w]
+
+    def close(self):
+        if self.h:
+            self.o.lib.orc_stats_destroy(self.h)
+            self.h = None
 
 
 # free-standing numerics -----------------------------------------------------

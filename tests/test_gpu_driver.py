@@ -124,3 +124,81 @@ def test_diagnostics_match_oracle(gl, tmp_path):
         assert rel(d["interface_area"], o.interface_area()) < 1e-12
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("gl", [(32, 24), (16, 12, 20)])
+def test_nusselt_matches_oracle(gl, tmp_path):
+    ic = random_fields(gl, seed=29)
+    o, g = make_pair(ic, tmp_path)
+    try:
+        for _ in range(2):
+            o.integrate()
+            g.integrate()
+        d = g.nusselt()
+        ref = o.nusselt()
+        got = np.array([d["heat_flux"], d["ke_injection"], d["ke_dissipation"], d["thermal_dissipation"]])
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)), (got, ref)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("gl", [(24, 16), (12, 10, 8)])
+def test_statistics_match_oracle(gl, tmp_path, monkeypatch):
+    """Accumulation is one add per element per sample and the reduction sums
+    rows in order, as the oracle does: the profiles agree bit for bit."""
+    import oracle_lib as OL
+    monkeypatch.setenv("stat_rate", "0.5")
+    monkeypatch.setenv("stat_after", "0.0")
+    ic = random_fields(gl, seed=31)
+    o, g = make_pair(ic, tmp_path)
+    st = OL.OracleStats(o)
+    try:
+        g.statistics_init(0.0)
+        assert g.statistics_next_time() == 0.5
+        for _ in range(3):
+            st.collect()
+            g.statistics_collect()
+            o.integrate()
+            g.integrate()
+        assert g.statistics_next_time() == 2.0
+        out = tmp_path / "stat"
+        g.statistics_output(str(out))
+        assert np.load(out / "num.npy") == 3
+        names = ["ux1", "ux2", "uy1", "uy2", "t1", "t2", "uxt"] + (["uz1", "uz2"] if len(gl) == 3 else [])
+        for n in names:
+            assert np.array_equal(np.load(out / f"{n}.npy"), st.reduce(n)), n
+        assert (out / "xf.npy").exists() and (out / "m_dif.npy").exists()
+    finally:
+        st.close()
+        g.close()
+
+
+def test_driver_logging_and_statistics(tmp_path):
+    """With log_rate / stat_rate set the driver writes the reference's log
+    files (logging/main.c:82-104) and a statistics directory."""
+    gl = (16, 20)
+    ic = random_fields(gl, seed=37)
+    sva.solver_from_ic(ic, str(tmp_path / "ic")).close()
+    env = dict(os.environ)
+    for k, v in ic.params.items():
+        env[k] = repr(float(v))
+    env.update(log_rate="1e-9", stat_rate="1e-9", stat_after="0")
+    res = subprocess.run([driver_path(2), str(tmp_path / "ic"), str(tmp_path / "run"), "4"], env=env,
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    log = tmp_path / "run" / "log"
+    for name, ncol in (("divergence.dat", 2), ("momentum.dat", 3), ("energy.dat", 4), ("nusselt.dat", 5),
+                       ("vof.dat", 4)):
+        rows = [ln.split() for ln in (log / name).read_text().splitlines()]
+        assert len(rows) == 4 and all(len(r) == ncol for r in rows), name
+        float(rows[-1][-1])
+    st = tmp_path / "run" / "stat" / "step0000000004"
+    assert np.load(st / "num.npy") == 4
+    assert np.load(st / "ux1.npy").shape == (gl[0] + 1,)
+    final = sva.Solver(str(tmp_path / "run" / "step0000000004"), 2, dict(ic.params))
+    try:
+        d = final.diagnostics()
+        vol = float((log / "vof.dat").read_text().splitlines()[-1].split()[-1])
+        assert abs(vol - d["vof_volume"]) < 1e-14
+    finally:
+        final.close()

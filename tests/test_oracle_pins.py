@@ -509,3 +509,125 @@ def test_interface_measure_of_planar_and_circular_interfaces():
     xf, _ = make_xgrid(1.0, n)
     o2 = Oracle(_case(n, n, {"vof": _pad_cells(vof_discs(xf, n, 1.0, [(0.5, 0.5, R)]))}))
     assert abs(o2.interface_area() - 2 * np.pi * R) < 0.03 * 2 * np.pi * R
+
+
+# --- logging/nusselt and statistics ----------------------------------------------
+
+def _linear_t_case(nx, ny, uniformx, lx=1.0, ly=1.0, params=None, fields=None):
+    xf, xc = make_xgrid(lx, nx, uniformx)
+    t = np.tile(0.5 - xc / lx, (ny, 1))
+    f = {"t": t}
+    if fields:
+        f.update(fields)
+    prm = dict(t_xm=0.5, t_xp=-0.5)
+    if params:
+        prm.update(params)
+    return _case(nx, ny, f, lx=lx, ly=ly, uniformx=uniformx, params=prm)
+
+
+@pytest.mark.parametrize("uniformx", [True, False])
+def test_nusselt_of_the_conductive_state(uniformx):
+    """Linear temperature between the walls and no flow: every definition
+    gives Nu = 1 (the wall fluxes and the dissipation both telescope, the
+    doubled half-cell weights at the walls included)."""
+    o = Oracle(_linear_t_case(16, 8, uniformx))
+    nu = o.nusselt()
+    assert np.max(np.abs(nu - 1.0)) < 1e-13, nu
+
+
+@pytest.mark.parametrize("uniformx", [True, False])
+def test_nusselt_of_a_linear_shear_and_a_uniform_injection(uniformx):
+    """uy = c x (walls uy_xm = 0, uy_xp = c lx): the kinetic dissipation is
+    m_dif c^2 lx ly exactly; a uniform ux against uniform T injects
+    u0 T0 (xc[nx] - xc[1]) ly."""
+    nx, ny, lx, ly, c = 16, 8, 1.0, 2.0, 0.3
+    xf, xc = make_xgrid(lx, nx, uniformx)
+    uy = np.tile(c * xc, (ny, 1))
+    o = Oracle(_case(nx, ny, {"uy": uy}, lx=lx, ly=ly, uniformx=uniformx,
+                     params=dict(uy_xm=0.0, uy_xp=c * lx)))
+    md, td = o.lib.orc_m_dif(o.h), o.lib.orc_t_dif(o.h)
+    nu = o.nusselt()
+    assert abs(nu[2] - (md * c * c * lx * ly / (td * ly) + 1.0)) < 1e-12
+    u0, t0 = 0.4, 0.25
+    ux = np.zeros((ny, nx + 1))
+    ux[:, 1:-1] = u0
+    o2 = Oracle(_case(nx, ny, {"ux": ux, "t": np.full((ny, nx + 2), t0)}, lx=lx, ly=ly, uniformx=uniformx,
+                      params=dict(t_xm=t0, t_xp=t0)))
+    nu2 = o2.nusselt()
+    assert abs(nu2[1] - (u0 * t0 * (xc[nx] - xc[1]) * ly / (td * ly) + 1.0)) < 1e-12
+    assert abs(nu2[0]) < 1e-13 and abs(nu2[3]) < 1e-13
+
+
+def test_nusselt_3d_reduces_to_2d_and_is_symmetric_in_y_and_z():
+    """A z-uniform 3D field gives the 2D numbers (the reference flux gains
+    the factor lz); a field varying in y only equals its y<->z transpose."""
+    nx, ny = 12, 10
+    ic2 = random_fields((nx, ny), seed=5)
+    # *_init applies the x walls after the halo copy (halo.c, boundary/ux.c),
+    # so the halo rows keep the IC's wall entries until the first step;
+    # zero them so that the y and z halos agree
+    ic2.fields["ux"][:, [0, -1]] = 0.0
+    o2 = Oracle(ic2)
+    nz = 6
+    f3 = {k: np.repeat(v[None], nz, axis=0) for k, v in ic2.fields.items()}
+    f3["uz"] = np.zeros_like(f3["uy"])
+    ic3 = InitialCondition((ic2.lengths[0], ic2.lengths[1], 0.5), (nx, ny, nz), ic2.xf, ic2.xc, f3,
+                           dict(ic2.params))
+    o3 = Oracle(ic3)
+    assert np.max(np.abs(o3.nusselt() - o2.nusselt())) < 1e-12 * np.max(np.abs(o2.nusselt()))
+    # y-only variation vs the same variation in z
+    rng = np.random.default_rng(9)
+    n = 8
+    prof = rng.uniform(-1, 1, size=n)
+    base = random_fields((nx, n, n), seed=6)
+    fy = {k: np.zeros_like(v) for k, v in base.fields.items()}
+    fz = {k: np.zeros_like(v) for k, v in base.fields.items()}
+    fy["t"][:] = prof[None, :, None]
+    fz["t"][:] = prof[:, None, None]
+    fy["ux"][:, :, 1:-1] = prof[None, :, None]
+    fz["ux"][:, :, 1:-1] = prof[:, None, None]
+    fy["uz"][:] = prof[None, :, None]       # uz(y) mirrors uy(z)
+    fz["uy"][:] = prof[:, None, None]
+    prm = dict(base.params, t_xm=0.0, t_xp=0.0)
+    a = Oracle(InitialCondition(base.lengths, base.glsizes, base.xf, base.xc, fy, prm)).nusselt()
+    b = Oracle(InitialCondition(base.lengths, base.glsizes, base.xf, base.xc, fz, prm)).nusselt()
+    assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("gl", [(12, 10), (8, 6, 4)])
+def test_statistics_profiles(gl):
+    """The reduced profiles are the row sums of the accumulated samples on
+    the face (ux1 ux2 uxt: i = 1..nx+1) and cell (i = 0..nx+1) ranges."""
+    ic = random_fields(gl, seed=8)
+    o = Oracle(ic)
+    st = OL.OracleStats(o)
+    nd = len(gl)
+    samples = []
+    try:
+        for _ in range(3):
+            st.collect()
+            samples.append({n: o.get(n) for n in ["ux", "uy", "t"] + (["uz"] if nd == 3 else [])})
+            o.integrate()
+        assert st.num == 3
+        nx, ny = gl[0], gl[1]
+
+        def interior_rows(a):
+            # padded arrays: y (and z) halos of one row
+            if nd == 2:
+                return a.reshape(ny + 2, -1)[1:-1]
+            return a.reshape(gl[2] + 2, ny + 2, -1)[1:-1, 1:-1].reshape(-1, a.size // ((gl[2] + 2) * (ny + 2)))
+
+        exp = {k: 0.0 for k in ["ux1", "ux2", "uy1", "uy2", "t1", "t2", "uxt"]}
+        for sm in samples:
+            ux = interior_rows(sm["ux"])          # (rows, nx+1): faces 1..nx+1
+            uy = interior_rows(sm["uy"])          # (rows, nx+2): cells 0..nx+1
+            t = interior_rows(sm["t"])
+            tf = 0.5 * t[:, :-1] + 0.5 * t[:, 1:]
+            for k, v in (("ux1", ux), ("ux2", ux * ux), ("uy1", uy), ("uy2", uy * uy), ("t1", t),
+                         ("t2", t * t), ("uxt", ux * tf)):
+                exp[k] = exp[k] + v
+        for k, v in exp.items():
+            assert np.allclose(st.reduce(k), v.sum(axis=0), rtol=1e-13, atol=1e-13), k
+        assert st.reduce("ux1").shape == (nx + 1,) and st.reduce("t1").shape == (nx + 2,)
+    finally:
+        st.close()
