@@ -112,6 +112,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", type=int, default=None)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--poisson-mixed", type=int, default=-1, metavar="K",
+                    help="time the mixed-precision form (FP32 transforms + K FP64 refinements) "
+                         "instead of the plain FP64 step")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
                     help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb)")
     args = ap.parse_args(argv)
@@ -133,6 +136,8 @@ def main(argv=None) -> int:
     for t in args.tuning:
         name, val = t.split("=")
         g.set_tuning(name, int(val))
+    if args.poisson_mixed >= 0:
+        g.set_poisson_mixed(args.poisson_mixed)
     # the library holds everything on the device from here on
     d0 = g.diagnostics()
     for _ in range(args.warmup):
@@ -188,7 +193,7 @@ def main(argv=None) -> int:
             "workload": f"plain 3D {n}x{n}x{n} FP64 droplets (8 spheres r in [0.06,0.12], "
                         f"16-mode divergence-free velocity rms 0.1, Re=500, We=20), x walls, y/z periodic",
             "cells": cells,
-            "form": "plain",
+            "form": "plain" if args.poisson_mixed < 0 else f"mixed-precision Poisson, {args.poisson_mixed} refinement(s)",
         },
         "roofline": {
             "bound": "hbm",

@@ -280,6 +280,11 @@ extern const statistics_t statistics;
 /* statistics.output into any directory */
 int vof_statistics_output_to(const domain_t * domain, const char dirname[]);
 
+/* Poisson precision: refinements < 0 (default) is the plain FP64 solver;
+ * refinements >= 0 transforms in FP32 (FP32 spectrum, FP64 x eliminations)
+ * and then applies that many FP64 residual corrections psi += M(r - A psi) */
+int vof_set_poisson_mixed(domain_t * domain, int refinements);
+
 /* tile parameters (tuned by simplevofsolver_amd.tune): "fft_elems" = 1024 or
  * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
  * per work-group of the predictor's x solver (0 = automatic) */

@@ -76,7 +76,7 @@ ABI_SYMBOLS_COMMON = [
     "domain_finalise", "vof_set_schedule", "vof_diagnostics", "vof_profile",
     "vof_profile_reset", "vof_profile_report", "vof_copy_probe", "vof_set_fusion",
     "vof_read_step_time", "vof_write_step_time", "vof_set_tuning", "vof_nusselt",
-    "statistics", "vof_statistics_output_to",
+    "statistics", "vof_statistics_output_to", "vof_set_poisson_mixed",
 ]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
 
@@ -132,6 +132,7 @@ def load(ndims: int):
     lib.vof_set_fusion.argtypes = [vp, C.c_int]
     lib.vof_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     lib.vof_nusselt.argtypes = [vp, vp, dp]
+    lib.vof_set_poisson_mixed.argtypes = [vp, C.c_int]
     lib.vof_statistics_output_to.argtypes = [vp, C.c_char_p]
     lib.vof_read_step_time.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), dp]
     lib.vof_write_step_time.argtypes = [C.c_char_p, C.c_size_t, C.c_double]
@@ -310,6 +311,11 @@ class Solver:
     def statistics_output(self, dirname: str):
         self._chk(self.lib.vof_statistics_output_to(C.byref(self.domain), dirname.encode()),
                   "vof_statistics_output_to")
+
+    def set_poisson_mixed(self, refinements: int):
+        """refinements < 0: plain FP64 Poisson solve; >= 0: FP32 transforms
+        plus that many FP64 residual corrections."""
+        self._chk(self.lib.vof_set_poisson_mixed(C.byref(self.domain), int(refinements)), "vof_set_poisson_mixed")
 
     def set_tuning(self, name: str, value: int):
         self._chk(self.lib.vof_set_tuning(C.byref(self.domain), name.encode(), int(value)), "vof_set_tuning")

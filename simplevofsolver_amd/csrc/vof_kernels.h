@@ -1127,87 +1127,94 @@ struct FftPlan {
   int rad[16];
 };
 
+// complex element types of the transforms: FP64 (double2) for the plain
+// solver, FP32 (float2) for the mixed-precision solver
+template <class T2> struct Cx;
+template <> struct Cx<double2> {
+  using S = double;
+  __device__ static double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Cx<float2> {
+  using S = float;
+  __device__ static float2 mk(float a, float b) { return make_float2(a, b); }
+};
+
 // radix-R DFT of v in place: y_k = sum_j v_j exp(sign 2 pi i j k / R)
-template <int R>
-__device__ inline void butterfly(double2 * v, double sign);
-
-template <>
-__device__ inline void butterfly<2>(double2 * v, double) {
-  const double2 a = v[0], b = v[1];
-  v[0] = make_double2(a.x + b.x, a.y + b.y);
-  v[1] = make_double2(a.x - b.x, a.y - b.y);
-}
-
-template <>
-__device__ inline void butterfly<4>(double2 * v, double sign) {
-  const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-  const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-  const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-  const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
-  const double2 jd = make_double2(-sign * d13.y, sign * d13.x);   // (sign i) d13
-  v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-  v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-  v[1] = make_double2(d02.x + jd.x, d02.y + jd.y);
-  v[3] = make_double2(d02.x - jd.x, d02.y - jd.y);
-}
-
-template <>
-__device__ inline void butterfly<3>(double2 * v, double sign) {
-  const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
-  const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
-  const double2 t2 = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
-  const double2 m = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
-  const double2 r2 = make_double2(-s3 * t2.y, s3 * t2.x);         // s3 i t2
-  v[0] = make_double2(v[0].x + t1.x, v[0].y + t1.y);
-  v[1] = make_double2(m.x + r2.x, m.y + r2.y);
-  v[2] = make_double2(m.x - r2.x, m.y - r2.y);
-}
-
-template <>
-__device__ inline void butterfly<5>(double2 * v, double sign) {
-  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-  const double s1 = sign * 0.95105651629515357212, s2 = sign * 0.58778525229247312917;
-  const double2 a1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
-  const double2 b1 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
-  const double2 a2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
-  const double2 b2 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
-  const double2 x0 = v[0];
-  v[0] = make_double2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
-  const double2 m1 = make_double2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
-  const double2 m2 = make_double2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
-  const double2 n1 = make_double2(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
-  const double2 n2 = make_double2(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
-  v[1] = make_double2(m1.x + n1.x, m1.y + n1.y);
-  v[4] = make_double2(m1.x - n1.x, m1.y - n1.y);
-  v[2] = make_double2(m2.x + n2.x, m2.y + n2.y);
-  v[3] = make_double2(m2.x - n2.x, m2.y - n2.y);
+template <int R, class T2>
+__device__ inline void butterfly(T2 * v, typename Cx<T2>::S sign) {
+  using C = Cx<T2>;
+  using S = typename C::S;
+  if constexpr (R == 2) {
+    const T2 a = v[0], b = v[1];
+    v[0] = C::mk(a.x + b.x, a.y + b.y);
+    v[1] = C::mk(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const T2 s02 = C::mk(v[0].x + v[2].x, v[0].y + v[2].y);
+    const T2 d02 = C::mk(v[0].x - v[2].x, v[0].y - v[2].y);
+    const T2 s13 = C::mk(v[1].x + v[3].x, v[1].y + v[3].y);
+    const T2 d13 = C::mk(v[1].x - v[3].x, v[1].y - v[3].y);
+    const T2 jd = C::mk(-sign * d13.y, sign * d13.x);   // (sign i) d13
+    v[0] = C::mk(s02.x + s13.x, s02.y + s13.y);
+    v[2] = C::mk(s02.x - s13.x, s02.y - s13.y);
+    v[1] = C::mk(d02.x + jd.x, d02.y + jd.y);
+    v[3] = C::mk(d02.x - jd.x, d02.y - jd.y);
+  } else if constexpr (R == 3) {
+    const S c3 = S(-0.5), s3 = sign * S(0.86602540378443864676);
+    const T2 t1 = C::mk(v[1].x + v[2].x, v[1].y + v[2].y);
+    const T2 t2 = C::mk(v[1].x - v[2].x, v[1].y - v[2].y);
+    const T2 m = C::mk(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+    const T2 r2 = C::mk(-s3 * t2.y, s3 * t2.x);         // s3 i t2
+    v[0] = C::mk(v[0].x + t1.x, v[0].y + t1.y);
+    v[1] = C::mk(m.x + r2.x, m.y + r2.y);
+    v[2] = C::mk(m.x - r2.x, m.y - r2.y);
+  } else {
+    static_assert(R == 5, "radix");
+    const S c1 = S(0.30901699437494742410), c2 = S(-0.80901699437494742410);
+    const S s1 = sign * S(0.95105651629515357212), s2 = sign * S(0.58778525229247312917);
+    const T2 a1 = C::mk(v[1].x + v[4].x, v[1].y + v[4].y);
+    const T2 b1 = C::mk(v[1].x - v[4].x, v[1].y - v[4].y);
+    const T2 a2 = C::mk(v[2].x + v[3].x, v[2].y + v[3].y);
+    const T2 b2 = C::mk(v[2].x - v[3].x, v[2].y - v[3].y);
+    const T2 x0 = v[0];
+    v[0] = C::mk(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    const T2 m1 = C::mk(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const T2 m2 = C::mk(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    const T2 n1 = C::mk(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+    const T2 n2 = C::mk(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+    v[1] = C::mk(m1.x + n1.x, m1.y + n1.y);
+    v[4] = C::mk(m1.x - n1.x, m1.y - n1.y);
+    v[2] = C::mk(m2.x + n2.x, m2.y + n2.y);
+    v[3] = C::mk(m2.x - n2.x, m2.y - n2.y);
+  }
 }
 
 // one Stockham stage: combine R sub-transforms of length Ns read at stride
 // N/R (Govindaraju et al., SC'08 formulation)
-template <int R>
-__device__ inline void fft_stage(const double2 * a, double2 * b, int ncols, int N, int Ns,
-                                 const double2 * __restrict__ tw, double sign) {
+template <int R, class T2>
+__device__ inline void fft_stage(const T2 * a, T2 * b, int ncols, int N, int Ns,
+                                 const T2 * __restrict__ tw, typename Cx<T2>::S sign) {
+  using C = Cx<T2>;
+  using S = typename C::S;
   const int nb = N / R;
   const int tstride = N / (Ns * R);
   for (int t = threadIdx.x; t < ncols * nb; t += blockDim.x) {
     const int col = t / nb, jj = t - col * nb;
-    const double2 * in = a + (size_t)col * N;
-    double2 * out = b + (size_t)col * N;
+    const T2 * in = a + (size_t)col * N;
+    T2 * out = b + (size_t)col * N;
     const int kk = jj % Ns;
-    double2 v[R];
+    T2 v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const double2 x = in[jj + r * nb];
+      const T2 x = in[jj + r * nb];
       if (r == 0 || kk == 0) {
         v[r] = x;
       } else {
-        const double2 w = tw[kk * r * tstride];
-        const double ws = sign * w.y;
-        v[r] = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
+        const T2 w = tw[kk * r * tstride];
+        const S ws = sign * w.y;
+        v[r] = C::mk(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
       }
     }
-    butterfly<R>(v, sign);
+    butterfly<R, T2>(v, sign);
     const int idxD = (jj / Ns) * Ns * R + kk;
 #pragma unroll
     for (int r = 0; r < R; r++) out[idxD + r * Ns] = v[r];
@@ -1217,20 +1224,21 @@ __device__ inline void fft_stage(const double2 * a, double2 * b, int ncols, int 
 // Stockham autosort FFT of `ncols` LDS columns of length P.n between buffers
 // a and b; returns the buffer holding the result.
 // out[k] = sum_j in[j] exp(sign 2 pi i j k / n); tw[m] = (cos, sin)(2 pi m / n)
-__device__ double2 * lds_fft(double2 * a, double2 * b, int ncols, const FftPlan & P,
-                             const double2 * __restrict__ tw, double sign) {
+template <class T2>
+__device__ T2 * lds_fft(T2 * a, T2 * b, int ncols, const FftPlan & P, const T2 * __restrict__ tw,
+                        typename Cx<T2>::S sign) {
   const int N = P.n;
   int Ns = 1;
   for (int s = 0; s < P.nst; s++) {
     const int R = P.rad[s];
     switch (R) {
-      case 4: fft_stage<4>(a, b, ncols, N, Ns, tw, sign); break;
-      case 2: fft_stage<2>(a, b, ncols, N, Ns, tw, sign); break;
-      case 3: fft_stage<3>(a, b, ncols, N, Ns, tw, sign); break;
-      default: fft_stage<5>(a, b, ncols, N, Ns, tw, sign); break;
+      case 4: fft_stage<4, T2>(a, b, ncols, N, Ns, tw, sign); break;
+      case 2: fft_stage<2, T2>(a, b, ncols, N, Ns, tw, sign); break;
+      case 3: fft_stage<3, T2>(a, b, ncols, N, Ns, tw, sign); break;
+      default: fft_stage<5, T2>(a, b, ncols, N, Ns, tw, sign); break;
     }
     __syncthreads();
-    double2 * tmp = a; a = b; b = tmp;
+    T2 * tmp = a; a = b; b = tmp;
     Ns *= R;
   }
   return a;
@@ -1279,7 +1287,7 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
     slot[xl & 1] = r;
   }
   __syncthreads();
-  double2 * res = lds_fft(sa, sb, NC, P, tw, -1.);
+  double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, -1.);
   const int nh = N / 2 + 1;
   for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
     const int c = e / nh, ky = e % nh;
@@ -1296,22 +1304,22 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
 
 #if NDIMS == 3
 // complex transform in z of spec[x][kz][ky] for a run of ky at one x
-template <int ELEMS>
-__global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const double2 * __restrict__ tw,
-                                                      double2 * __restrict__ spec, int nh, double sign) {
-  __shared__ double2 sa[ELEMS], sb[ELEMS];
+template <int ELEMS, class T2>
+__global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const T2 * __restrict__ tw,
+                                                      T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
+  __shared__ T2 sa[ELEMS], sb[ELEMS];
   const int N = g.nz;
   const int NC = ELEMS / N;
   const int ky0 = blockIdx.x * NC;
   const int x = blockIdx.y;
-  double2 * base = spec + (size_t)x * N * nh;
+  T2 * base = spec + (size_t)x * N * nh;
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
     const int c = e % NC, kz = e / NC;
     const int ky = ky0 + c;
-    sa[c * N + kz] = ky < nh ? base[(size_t)kz * nh + ky] : make_double2(0., 0.);
+    sa[c * N + kz] = ky < nh ? base[(size_t)kz * nh + ky] : Cx<T2>::mk(0, 0);
   }
   __syncthreads();
-  double2 * res = lds_fft(sa, sb, NC, P, tw, sign);
+  T2 * res = lds_fft<T2>(sa, sb, NC, P, tw, sign);
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
     const int c = e % NC, kz = e / NC;
     const int ky = ky0 + c;
@@ -1327,9 +1335,18 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const 
 // current chunk is eliminated, so kTsU loads per lane stay in flight.
 constexpr int kTsU = 8;
 
+__device__ inline double2 to_d2(double2 a) { return a; }
+__device__ inline double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <class T2> __device__ inline T2 from_d2(double2 a);
+template <> __device__ inline double2 from_d2<double2>(double2 a) { return a; }
+template <> __device__ inline float2 from_d2<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+
+// T2 is the storage type of the spectrum (double2, or float2 in the
+// mixed-precision solver); the elimination itself is FP64 either way
+template <class T2>
 __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
-                              const double * __restrict__ ez, double2 * __restrict__ spec,
+                              const double * __restrict__ ez, T2 * __restrict__ spec,
                               double * __restrict__ vbuf) {
   const int line = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (line >= nlines) return;
@@ -1342,30 +1359,30 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
   const double eval = ey[ky] + ez[kz];
 #endif
   const size_t st = (size_t)nlines;
-  double2 * q = spec + line;
+  T2 * q = spec + line;
   double * v = vbuf + line;
   const int n = nx;
   // row 0 (with the Neumann fold c[0] += l[0])
   double cc = -pl[0] - pu[0] + eval;
   cc += pl[0];
   double vprev = pu[0] / cc;
-  double2 qprev = q[0];
+  double2 qprev = to_d2(q[0]);
   qprev.x = qprev.x / cc;
   qprev.y = qprev.y / cc;
-  q[0] = qprev;
+  q[0] = from_d2<T2>(qprev);
   v[0] = vprev;
   {
     double2 cur[kTsU];
 #pragma unroll
     for (int u = 0; u < kTsU; u++) {
-      if (1 + u < n - 1) cur[u] = q[(size_t)(1 + u) * st];
+      if (1 + u < n - 1) cur[u] = to_d2(q[(size_t)(1 + u) * st]);
     }
     for (int i0 = 1; i0 < n - 1; i0 += kTsU) {
       double2 nxt[kTsU];
 #pragma unroll
       for (int u = 0; u < kTsU; u++) {
         const int i = i0 + kTsU + u;
-        if (i < n - 1) nxt[u] = q[(size_t)i * st];
+        if (i < n - 1) nxt[u] = to_d2(q[(size_t)i * st]);
       }
 #pragma unroll
       for (int u = 0; u < kTsU; u++) {
@@ -1377,7 +1394,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
           double2 qi = cur[u];
           qi.x = val * (qi.x - pl[i] * qprev.x);
           qi.y = val * (qi.y - pl[i] * qprev.y);
-          q[(size_t)i * st] = qi;
+          q[(size_t)i * st] = from_d2<T2>(qi);
           v[(size_t)i * st] = vi;
           qprev = qi;
           vprev = vi;
@@ -1392,8 +1409,13 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
     double c = -pl[i] - pu[i] + eval;
     c += pu[i];
     const double val = c - pl[i] * vprev;
-    double2 qi = q[(size_t)i * st];
-    if (fabs(val) > DBL_EPSILON) {
+    double2 qi = to_d2(q[(size_t)i * st]);
+    // the (ky, kz) = (0, 0) line is singular (Neumann in x).  Its right-hand
+    // side is compatible to FP64 rounding, and then the guard below picks the
+    // gauge; an FP32 spectrum is compatible only to FP32 rounding, which
+    // 1 / val would blow up, so that line always takes the gauge branch
+    const bool gauge = sizeof(T2) == sizeof(float2) && eval == 0.;
+    if (!gauge && fabs(val) > DBL_EPSILON) {
       const double r = 1. / val;
       qi.x = r * (qi.x - pl[i] * qprev.x);
       qi.y = r * (qi.y - pl[i] * qprev.y);
@@ -1401,7 +1423,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
       qi.x = 0.;
       qi.y = 0.;
     }
-    q[(size_t)i * st] = qi;
+    q[(size_t)i * st] = from_d2<T2>(qi);
     qprev = qi;
   }
   {
@@ -1411,7 +1433,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
     for (int u = 0; u < kTsU; u++) {
       const int i = n - 2 - u;
       if (i >= 0) {
-        cq[u] = q[(size_t)i * st];
+        cq[u] = to_d2(q[(size_t)i * st]);
         cv[u] = v[(size_t)i * st];
       }
     }
@@ -1422,7 +1444,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
       for (int u = 0; u < kTsU; u++) {
         const int i = i0 - kTsU - u;
         if (i >= 0) {
-          nq[u] = q[(size_t)i * st];
+          nq[u] = to_d2(q[(size_t)i * st]);
           nv[u] = v[(size_t)i * st];
         }
       }
@@ -1433,7 +1455,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
           double2 qi = cq[u];
           qi.x -= cv[u] * qprev.x;
           qi.y -= cv[u] * qprev.y;
-          q[(size_t)i * st] = qi;
+          q[(size_t)i * st] = from_d2<T2>(qi);
           qprev = qi;
         }
       }
@@ -1480,7 +1502,7 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
     sa[c * N + ky] = make_double2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  double2 * res = lds_fft(sa, sb, NC, P, tw, +1.);
+  double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, +1.);
   for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
     const int xl = e % (2 * NC), j = e / (2 * NC);
     const int i = x0 + xl + 1;
@@ -1495,6 +1517,136 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
       } else {
         psi[I_C1(i, j + 1, k)] = v;
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision Poisson solve: the transforms run in FP32 on an FP32
+// spectrum, the x eliminations in FP64 (k_thomas_spec<float2>), and FP64
+// residual corrections refine psi:
+//   psi = M(r0);  repeat: psi += M(r0 - A psi)
+// with r0 = div(u*) / (gamma dt) and A the discrete Laplacian the FP64
+// solver inverts (second differences in y and z, the l/u coefficients and
+// Neumann walls in x).  M includes the 1/(ny nz) of the unnormalised
+// transforms.
+
+// mode 0: r = prefactor * div u, stored in rhs; mode 1: r = rhs - A psi.
+// Then (float)(r * scale) is transformed in y as k_div_rfft_y does.
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
+    Grid g, FftPlan P, const float2 * __restrict__ tw, const double * __restrict__ ux,
+    const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
+    double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
+    const double * __restrict__ pu, int mode, float2 * __restrict__ spec) {
+  __shared__ float2 sa[ELEMS], sb[ELEMS];
+  const int nx = g.nx, ny = g.ny;
+  const int N = ny;
+  const int NC = ELEMS / N;
+  const int x0 = blockIdx.x * 2 * NC;
+#if NDIMS == 2
+  const int k = 1;
+  const int nz = 1;
+  (void)uz;
+#else
+  const int k = (int)blockIdx.y + 1;
+  const int nz = g.nz;
+#endif
+  const double rdy2 = 1. / (g.dy * g.dy);
+#if NDIMS == 3
+  const double rdz2 = 1. / (g.dz * g.dz);
+#endif
+  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
+    const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+    const int i = x0 + xl + 1;
+    float r32 = 0.f;
+    if (i <= nx) {
+      const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+      double r;
+      if (mode == 0) {
+        const double dx = g.dxf[i - 1];
+        double div = (ux[I_UX(i + 1, j, k)] - ux[I_UX(i, j, k)]) / dx +
+                     (uy[I_C1(i, j + 1, k)] - uy[I_C1(i, j, k)]) / g.dy;
+#if NDIMS == 3
+        div += (uz[I_C1(i, j, k + 1)] - uz[I_C1(i, j, k)]) / g.dz;
+#endif
+        r = prefactor * div;
+        rhs[ir] = r;
+      } else {
+        const double pc = psi[I_C1(i, j, k)];
+        // psi(0) = psi(1) and psi(nx+1) = psi(nx) (Neumann images) fold the
+        // wall rows of the x operator
+        double ap = pl[i - 1] * (psi[I_C1(i - 1, j, k)] - pc) + pu[i - 1] * (psi[I_C1(i + 1, j, k)] - pc);
+        ap += rdy2 * (psi[I_C1(i, j - 1, k)] - 2. * pc + psi[I_C1(i, j + 1, k)]);
+#if NDIMS == 3
+        ap += rdz2 * (psi[I_C1(i, j, k - 1)] - 2. * pc + psi[I_C1(i, j, k + 1)]);
+#endif
+        r = rhs[ir] - ap;
+      }
+      r32 = (float)(r * scale);
+    }
+    float * slot = reinterpret_cast<float *>(&sa[(xl >> 1) * N + (j - 1)]);
+    slot[xl & 1] = r32;
+  }
+  __syncthreads();
+  float2 * res = lds_fft<float2>(sa, sb, NC, P, tw, -1.f);
+  const int nh = N / 2 + 1;
+  for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
+    const int c = e / nh, ky = e % nh;
+    const float2 z = res[c * N + ky];
+    const float2 zc = res[c * N + (N - ky) % N];
+    const float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
+    const float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
+    const int xa = x0 + 2 * c, xb = xa + 1;
+    if (xa < nx) spec[((size_t)xa * nz + (k - 1)) * nh + ky] = A;
+    if (xb < nx) spec[((size_t)xb * nz + (k - 1)) * nh + ky] = B;
+  }
+}
+
+// inverse real transform in y in FP32; psi = delta (accumulate 0) or
+// psi += delta (accumulate 1) in FP64, with halo images and Neumann walls
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_irfft32_y_psi(
+    Grid g, FftPlan P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
+    double * __restrict__ psi, int accumulate) {
+  __shared__ float2 sa[ELEMS], sb[ELEMS];
+  const int nx = g.nx, ny = g.ny;
+  const int N = ny;
+  const int NC = ELEMS / N;
+  const int x0 = blockIdx.x * 2 * NC;
+  const int nh = N / 2 + 1;
+#if NDIMS == 2
+  const int k = 1;
+  const int nz = 1;
+#else
+  const int k = (int)blockIdx.y + 1;
+  const int nz = g.nz;
+#endif
+  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
+    const int c = e / N, ky = e % N;
+    const int xa = x0 + 2 * c, xb = xa + 1;
+    const int kk = ky <= N / 2 ? ky : N - ky;
+    const float cs = ky <= N / 2 ? 1.f : -1.f;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (xa < nx) A = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
+    if (xb < nx) B = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
+    if (kk == 0 || 2 * kk == N) { A.y = 0.f; B.y = 0.f; }
+    A.y *= cs;
+    B.y *= cs;
+    sa[c * N + ky] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  float2 * res = lds_fft<float2>(sa, sb, NC, P, tw, +1.f);
+  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
+    const int xl = e % (2 * NC), j = e / (2 * NC);
+    const int i = x0 + xl + 1;
+    if (i <= nx) {
+      const float2 z = res[(xl >> 1) * N + j];
+      double v = (double)((xl & 1) ? z.y : z.x);
+      if (accumulate) v = psi[I_C1(i, j + 1, k)] + v;
+      const int pln = NDIMS == 3 ? k : 0;
+      store_periodic(psi, i, nx + 2, ny + 2, j + 1, pln, ny, nz, 1, v);
+      neumann_walls(psi, i, nx, ny + 2, j + 1, pln, ny, nz, v);
     }
   }
 }

@@ -236,6 +236,11 @@ struct vof_ctx {
   double * d_ey;                         // eigenvalues in y, z (dft.c:263-291)
   double * d_ez;
   double2 * d_spec;
+  // mixed-precision Poisson (refinements < 0: plain FP64)
+  int poisson_refine;
+  float2 * d_spec32;
+  float2 * d_twy32;
+  float2 * d_twz32;
   double * d_vbuf;
   // scratch
   double * d_delta;                      // x1 pencil of increments
@@ -632,7 +637,8 @@ static int ctx_create(domain_t * domain) {
   ctx->dy = domain->dy;
   ctx->schedule = 1;
   ctx->fuse = 63;
-  ctx->fft_elems = 2048;
+  ctx->fft_elems = 1024;
+  ctx->poisson_refine = -1;
   ctx->lines_lpb = 0;
   ctx->d_vof_spare = nullptr;
   ctx->vof_bytes = 0;
@@ -818,6 +824,7 @@ extern "C" int domain_finalise(domain_t * domain) {
     for (int m = 0; m < 12; m++) (void)hipEventDestroy(ctx->tab_done[m]);
     dev_free(ctx->d_twy); if (ctx->d_twz) dev_free(ctx->d_twz);
     dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez);
+    dev_free(ctx->d_spec32); dev_free(ctx->d_twy32); dev_free(ctx->d_twz32);
     dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
     if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
     for (int m = 0; m < 9; m++) dev_free(ctx->d_stat[m]);
@@ -1230,9 +1237,51 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 
 // Poisson: both reference solvers are the same discrete problem; the FFT
 // normalisation 1/(ny nz) is folded into the right-hand side as dft.c:329-330
+// mixed-precision solve: FP32 transforms, FP64 eliminations, and
+// `poisson_refine` FP64 residual corrections; psi always carries its halo
+// images and Neumann walls (the residual reads them)
+static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fluid) {
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  const double prefactor = 1. / (kRk[rkstep][2] * dt);
+  const double scale = 1. / ((double)ny * (NDIMS == 3 ? (double)nz : 1.));
+  const int elems = 2048;
+  const int NC = elems / ny;
+  const int nbx = (nx + 2 * NC - 1) / (2 * NC);
+#if NDIMS == 3
+  const double * uz = (const double *)fluid->uz.data;
+  const int NCz = elems / nz;
+  const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
+#else
+  const double * uz = nullptr;
+#endif
+  double * psi = (double *)fluid->psi.data;
+  const int nlines = ctx->nh * nz;
+  for (int pass = 0; pass <= ctx->poisson_refine; pass++) {
+    KLN(ctx, "k_res_rfft32_y", vofk::k_res_rfft32_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy32, (const double *)fluid->ux.data,
+        (const double *)fluid->uy.data, uz, prefactor, scale, ctx->d_delta, (const double *)psi, ctx->d_pl,
+        ctx->d_pu, pass == 0 ? 0 : 1, ctx->d_spec32);
+#if NDIMS == 3
+    KLN(ctx, "k_fft32_z", (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z,
+        ctx->d_twz32, ctx->d_spec32, ctx->nh, -1.f);
+#endif
+    KLN(ctx, "k_thomas_spec32", vofk::k_thomas_spec<float2>, dim3((nlines + 63) / 64), dim3(64), nx, nlines,
+        ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec32, ctx->d_vbuf);
+#if NDIMS == 3
+    KLN(ctx, "k_fft32_z", (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z,
+        ctx->d_twz32, ctx->d_spec32, ctx->nh, +1.f);
+#endif
+    KLN(ctx, "k_irfft32_y_psi", vofk::k_irfft32_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi,
+        pass == 0 ? 0 : 1);
+  }
+  return 0;
+}
+
 static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid, int images) {
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
+  if (ctx->poisson_refine >= 0) return potential_mixed(ctx, rkstep, dt, fluid);
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
   const double norm = (double)ny * (NDIMS == 3 ? (double)nz : 1.);
   const double prefactor = 1. / (kRk[rkstep][2] * dt) / norm;
@@ -1257,22 +1306,22 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const int NCz = elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
   if (elems == 1024) {
-    KLN(ctx, "k_fft_z", vofk::k_fft_z<1024>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLN(ctx, "k_fft_z", (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, -1.);
   } else {
-    KLN(ctx, "k_fft_z", vofk::k_fft_z<2048>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLN(ctx, "k_fft_z", (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, -1.);
   }
 #endif
   const int nlines = ctx->nh * nz;
-  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec, dim3((nlines + 63) / 64), dim3(64), nx, nlines, ctx->nh,
-      ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64), dim3(64), nx, nlines,
+      ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
 #if NDIMS == 3
   if (elems == 1024) {
-    KLN(ctx, "k_fft_z", vofk::k_fft_z<1024>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLN(ctx, "k_fft_z", (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, +1.);
   } else {
-    KLN(ctx, "k_fft_z", vofk::k_fft_z<2048>, gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLN(ctx, "k_fft_z", (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, +1.);
   }
 #endif
@@ -1564,6 +1613,29 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     return 0;
   }
   return fail("vof_set_tuning: unknown parameter %s", name);
+}
+
+// Poisson precision: refinements < 0 selects the plain FP64 solver; >= 0 the
+// FP32-transform solver followed by that many FP64 residual corrections
+extern "C" int vof_set_poisson_mixed(domain_t * domain, int refinements) {
+  CTX_OR_FAIL(domain);
+  if (refinements > 16) return fail("vof_set_poisson_mixed: at most 16 refinements");
+  if (refinements >= 0 && !ctx->d_spec32) {
+    if (ctx->ny > 2048 || (NDIMS == 3 && ctx->nz > 2048)) return fail("mixed Poisson: FFT length above 2048");
+    const size_t nline = (size_t)ctx->nh * ctx->nz;
+    HIPCHK(hipMalloc(&ctx->d_spec32, sizeof(float2) * ctx->nx * nline));
+    for (int d = 0; d < (NDIMS == 3 ? 2 : 1); d++) {
+      const int n = d == 0 ? ctx->ny : ctx->nz;
+      std::vector<double2> tw = make_twiddles(n);
+      std::vector<float2> tw32(n);
+      for (int m = 0; m < n; m++) tw32[m] = make_float2((float)tw[m].x, (float)tw[m].y);
+      float2 ** dst = d == 0 ? &ctx->d_twy32 : &ctx->d_twz32;
+      HIPCHK(hipMalloc(dst, sizeof(float2) * n));
+      HIPCHK(hipMemcpy(*dst, tw32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    }
+  }
+  ctx->poisson_refine = refinements < 0 ? -1 : refinements;
+  return 0;
 }
 
 // stages of the fused schedule (bit mask; the default is all of them)

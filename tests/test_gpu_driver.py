@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import simplevofsolver_amd as sva
-from parity_util import interior, make_pair, rel
+from parity_util import copy_state, interior, make_pair, rel
 from simplevofsolver_amd.build import driver_path
 from simplevofsolver_amd.ic import random_fields
 
@@ -145,7 +145,8 @@ def test_nusselt_matches_oracle(gl, tmp_path):
 @pytest.mark.parametrize("gl", [(24, 16), (12, 10, 8)])
 def test_statistics_match_oracle(gl, tmp_path, monkeypatch):
     """Accumulation is one add per element per sample and the reduction sums
-    rows in order, as the oracle does: the profiles agree bit for bit."""
+    rows in order, as the oracle does: on identical samples (the oracle's
+    state copied in before each collect) the profiles agree bit for bit."""
     import oracle_lib as OL
     monkeypatch.setenv("stat_rate", "0.5")
     monkeypatch.setenv("stat_after", "0.0")
@@ -157,9 +158,9 @@ def test_statistics_match_oracle(gl, tmp_path, monkeypatch):
         assert g.statistics_next_time() == 0.5
         for _ in range(3):
             st.collect()
+            copy_state(o, g)
             g.statistics_collect()
             o.integrate()
-            g.integrate()
         assert g.statistics_next_time() == 2.0
         out = tmp_path / "stat"
         g.statistics_output(str(out))
