@@ -102,6 +102,76 @@ def cpu_baseline(ic, steps: int):
                       f"on one host core ({el:.1f} s)"}
 
 
+def mixed_form(g, cells, steps, plain_value):
+    """The same solver with the FP32-transform Poisson solve and one FP64
+    refinement, timed over `steps` steps."""
+    g.set_poisson_mixed(1)
+    g.integrate()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.integrate()
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.set_poisson_mixed(-1)
+    v = cells * steps / el / 1e6
+    print(f"[bench] mixed-precision form {v:.1f} Mcell-steps/s ({el / steps * 1e3:.3f} ms/step)",
+          file=sys.stderr, flush=True)
+    return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+            "vs_plain": round(v / plain_value, 4),
+            "workload": "same 3D 256^3 case, FP32 transforms + FP64 x eliminations, 1 FP64 refinement"}
+
+
+def batched_form(sva, ncases, steps):
+    """`ncases` independent 2D cases of uneven sizes (nx in {64..256},
+    ny = 2 nx) advanced together, against the same cases stepped one after
+    another with integrate()."""
+    from simplevofsolver_amd.ic import batch_2d_cases
+    t0 = time.perf_counter()
+    cases = batch_2d_cases(ncases, seed=4)
+    print(f"[bench] {ncases} batch cases generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    cells = sum(c.glsizes[0] * c.glsizes[1] for c in cases)
+    solvers = []
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        for c, ic in enumerate(cases):
+            solvers.append(sva.solver_from_ic(ic, os.path.join(d, f"c{c}")))
+    try:
+        # one after another (each integrate() waits for its own dt)
+        for s in solvers:
+            s.integrate()
+        t0 = time.perf_counter()
+        nseq = max(1, steps // 5)
+        for _ in range(nseq):
+            for s in solvers:
+                s.integrate()
+        for s in solvers:
+            s.synchronize()
+        seq = cells * nseq / (time.perf_counter() - t0) / 1e6
+        b = sva.Batch(solvers)
+        try:
+            b.integrate()
+            b.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.integrate()
+            b.synchronize()
+            el = time.perf_counter() - t0
+        finally:
+            b.close()
+    finally:
+        for s in solvers:
+            s.close()
+    v = cells * steps / el / 1e6
+    print(f"[bench] batched form {v:.1f} Mcell-steps/s ({el / steps * 1e3:.3f} ms/step), "
+          f"one by one {seq:.1f}", file=sys.stderr, flush=True)
+    return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+            "sequential_value": round(seq, 3), "vs_sequential": round(v / seq, 3), "cases": ncases,
+            "cells": cells, "steps": steps,
+            "workload": f"{ncases} 2D cases, nx in {{64,96,128,192,256}}, ny = 2 nx, 1-4 droplets, "
+                        "Re in [50,500], We in [5,50], FP64"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,7 +186,11 @@ def main(argv=None) -> int:
                     help="time the mixed-precision form (FP32 transforms + K FP64 refinements) "
                          "instead of the plain FP64 step")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
-                    help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb)")
+                    help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb, zchunk, xcd_remap)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the mixed-precision and batched forms")
+    ap.add_argument("--batch-cases", type=int, default=96)
+    ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -166,6 +240,12 @@ def main(argv=None) -> int:
     achieved = kbytes / kavg_s / 1e9 if kbytes else None
     traffic = read_traffic(kname, f"3d{n}")
     copy_gbs = g.copy_probe(1 << 27, 10)
+    forms = {}
+    if not args.no_forms and args.poisson_mixed < 0:
+        try:
+            forms["mixed_precision"] = mixed_form(g, cells, args.steps, value)
+        except Exception as e:  # reported, never fatal for the plain line
+            forms["mixed_precision"] = {"error": repr(e)}
     if args.kernels:
         print(f"[bench] kernels (timed region {el * 1e3:.2f} ms, kernel sum {kern_total:.2f} ms):",
               file=sys.stderr)
@@ -176,6 +256,11 @@ def main(argv=None) -> int:
                   file=sys.stderr)
         print(f"[bench] copy probe {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     g.close()
+    if not args.no_forms:
+        try:
+            forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps)
+        except Exception as e:
+            forms["batched"] = {"error": repr(e)}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -217,6 +302,7 @@ def main(argv=None) -> int:
             "steps": args.warmup + args.steps,
         },
     }
+    out["forms"] = forms or None
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)

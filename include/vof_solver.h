@@ -285,9 +285,45 @@ int vof_statistics_output_to(const domain_t * domain, const char dirname[]);
  * and then applies that many FP64 residual corrections psi += M(r - A psi) */
 int vof_set_poisson_mixed(domain_t * domain, int refinements);
 
-/* tile parameters (tuned by simplevofsolver_amd.tune): "fft_elems" = 1024 or
+/* ---- batches: many independent cases advanced together -----------------
+ * The cases are initialised one by one with the entry points above (own
+ * sizes, parameters and grids; fused schedule) and then stepped together:
+ * each kernel of the step runs as one launch covering every case's
+ * work-groups, with results bit-identical to integrate() on each case.  The
+ * step is asynchronous on the batch's stream: vof_batch_synchronize before
+ * touching a case through any other entry point.  No counterpart in the
+ * reference (one case per process there). */
+typedef struct vof_batch vof_batch_t;
+int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
+                     interface_t * const interfaces[], vof_batch_t ** batch);
+/* one time step of every case; dts[c] receives case c's dt (may be NULL) */
+int vof_batch_integrate(vof_batch_t * batch, double dts[]);
+int vof_batch_synchronize(vof_batch_t * batch);
+int vof_batch_destroy(vof_batch_t * batch);
+
+/* one time step with a given dt instead of fluid_decide_dt (fused schedule):
+ * the frozen dt sequence of the adjoint and its finite-difference checks */
+int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, interface_t * interface, double dt);
+
+#if NDIMS == 2
+/* ---- adjoint (2D): gradient of a loss after nsteps w.r.t. the initial
+ * velocity.  Loss = droplet centroid x (sum xc vof dV / sum vof dV) +
+ * interface length (sum |grad vof| dV, central differences).  The forward
+ * runs dts[0..nsteps-1] from the current state, keeping a checkpoint every
+ * `every` steps; the reverse sweep recomputes each segment.  grad_ux /
+ * grad_uy are host arrays of the padded ux / uy shapes (values on the
+ * velocity DOFs, zero elsewhere).  The solver is left at its initial state.
+ * No counterpart in the reference. */
+int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, interface_t * interface, int nsteps,
+                         const double * dts, int every, double * loss, double * grad_ux, double * grad_uy);
+/* the same loss of the current state */
+int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, double * loss);
+#endif
+
+/* tile parameters (results do not depend on them): "fft_elems" = 1024 or
  * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
- * per work-group of the predictor's x solver (0 = automatic) */
+ * per work-group of the predictor's x solver (0 = automatic);
+ * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e */
 int vof_set_tuning(domain_t * domain, const char name[], int value);
 /* step.npy / time.npy of an IC or save directory (the reference's main.c
  * reads and writes them with fileio.r_serial / w_serial) */

@@ -77,7 +77,10 @@ ABI_SYMBOLS_COMMON = [
     "vof_profile_reset", "vof_profile_report", "vof_copy_probe", "vof_set_fusion",
     "vof_read_step_time", "vof_write_step_time", "vof_set_tuning", "vof_nusselt",
     "statistics", "vof_statistics_output_to", "vof_set_poisson_mixed",
+    "vof_batch_create", "vof_batch_integrate", "vof_batch_synchronize", "vof_batch_destroy",
+    "vof_integrate_fixed_dt",
 ]
+ABI_SYMBOLS_2D = ["vof_adjoint_gradient", "vof_adjoint_loss"]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
 
 
@@ -133,6 +136,14 @@ def load(ndims: int):
     lib.vof_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
     lib.vof_nusselt.argtypes = [vp, vp, dp]
     lib.vof_set_poisson_mixed.argtypes = [vp, C.c_int]
+    lib.vof_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    lib.vof_batch_integrate.argtypes = [vp, dp]
+    lib.vof_batch_synchronize.argtypes = [vp]
+    lib.vof_batch_destroy.argtypes = [vp]
+    lib.vof_integrate_fixed_dt.argtypes = [vp, vp, vp, C.c_double]
+    if ndims == 2:
+        lib.vof_adjoint_gradient.argtypes = [vp, vp, vp, C.c_int, dp, C.c_int, dp, dp, dp]
+        lib.vof_adjoint_loss.argtypes = [vp, vp, dp]
     lib.vof_statistics_output_to.argtypes = [vp, C.c_char_p]
     lib.vof_read_step_time.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), dp]
     lib.vof_write_step_time.argtypes = [C.c_char_p, C.c_size_t, C.c_double]
@@ -312,6 +323,30 @@ class Solver:
         self._chk(self.lib.vof_statistics_output_to(C.byref(self.domain), dirname.encode()),
                   "vof_statistics_output_to")
 
+    def integrate_fixed_dt(self, dt: float):
+        """One step with the given dt (no fluid_decide_dt)."""
+        self._chk(self.lib.vof_integrate_fixed_dt(C.byref(self.domain), C.byref(self.fluid),
+                                                  C.byref(self.interface), float(dt)), "vof_integrate_fixed_dt")
+
+    def adjoint_loss(self) -> float:
+        out = C.c_double()
+        self._chk(self.lib.vof_adjoint_loss(C.byref(self.domain), C.byref(self.interface), C.byref(out)),
+                  "vof_adjoint_loss")
+        return out.value
+
+    def adjoint_gradient(self, dts, every: int = 10):
+        """(loss, d loss / d ux0, d loss / d uy0) for len(dts) steps with the
+        given dt sequence (2D); the solver is left at its initial state."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        gux = np.zeros(self._array("ux").datasize // 8)
+        guy = np.zeros(self._array("uy").datasize // 8)
+        loss = C.c_double()
+        self._chk(self.lib.vof_adjoint_gradient(C.byref(self.domain), C.byref(self.fluid), C.byref(self.interface),
+                                                len(dts), dts.ctypes.data_as(C.POINTER(C.c_double)), int(every),
+                                                C.byref(loss), gux.ctypes.data_as(C.POINTER(C.c_double)),
+                                                guy.ctypes.data_as(C.POINTER(C.c_double))), "vof_adjoint_gradient")
+        return loss.value, gux, guy
+
     def set_poisson_mixed(self, refinements: int):
         """refinements < 0: plain FP64 Poisson solve; >= 0: FP32 transforms
         plus that many FP64 residual corrections."""
@@ -359,6 +394,45 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """vof_batch_*: many independent cases (Solver objects of one
+    dimensionality, any sizes) advanced together, one launch per kernel of
+    the step for all of them."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("empty batch")
+        nd = self.solvers[0].ndims
+        if any(s.ndims != nd for s in self.solvers):
+            raise ValueError("a batch holds cases of one dimensionality")
+        self.lib = self.solvers[0].lib
+        n = len(self.solvers)
+        arr = C.c_void_p * n
+        doms = arr(*[C.addressof(s.domain) for s in self.solvers])
+        flus = arr(*[C.addressof(s.fluid) for s in self.solvers])
+        ifs = arr(*[C.addressof(s.interface) for s in self.solvers])
+        self.h = C.c_void_p()
+        if self.lib.vof_batch_create(n, doms, flus, ifs, C.byref(self.h)) != 0:
+            raise VofError("vof_batch_create: " + self.lib.vof_last_error().decode())
+        self._dts = (C.c_double * n)()
+
+    def integrate(self):
+        """One step of every case; returns the list of dt."""
+        if self.lib.vof_batch_integrate(self.h, self._dts) != 0:
+            raise VofError("vof_batch_integrate: " + self.lib.vof_last_error().decode())
+        return list(self._dts)
+
+    def synchronize(self):
+        if self.lib.vof_batch_synchronize(self.h) != 0:
+            raise VofError("vof_batch_synchronize: " + self.lib.vof_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.lib.vof_batch_destroy(self.h)
+            self.h = None
 
 
 def solver_from_ic(ic_obj, dirname: str, schedule: int | None = None, params: dict | None = None) -> "Solver":

@@ -49,17 +49,70 @@ struct Grid {
   const double * __restrict__ dxc;  // [nx+1] DXC(i) = dxc[i-1]
 };
 
+// Work-groups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Tile kernels renumber their work-groups so that every XCD processes
+// one contiguous run of tiles (x fastest): tiles that share cache lines or
+// stencil planes then meet in the same L2.  The map is a bijection of
+// [0, total), so each tile is still processed exactly once.
+constexpr unsigned kXcds = 8;
+
+// Every kernel body takes its work-group coordinates and grid shape as a
+// VBlock: the hardware's for a plain launch, or a case's own tile for a
+// batched launch that covers many cases (one work-group = one tile of one
+// case).
+struct VBlock {
+  unsigned x, y, z;
+  unsigned gx, gy, gz;
+};
+
+__device__ inline VBlock hw_block() {
+  VBlock b;
+  b.x = blockIdx.x;
+  b.y = blockIdx.y;
+  b.z = blockIdx.z;
+  b.gx = gridDim.x;
+  b.gy = gridDim.y;
+  b.gz = gridDim.z;
+  return b;
+}
+
+struct Tile3 {
+  unsigned x, y, z;
+};
+
+__device__ inline Tile3 xcd_tile(const VBlock & vb) {
+  const unsigned gx = vb.gx, gy = vb.gy;
+  const unsigned total = gx * gy * vb.gz;
+  const unsigned w = vb.x + gx * (vb.y + gy * vb.z);
+  const unsigned q = total / kXcds, r = total % kXcds;
+  const unsigned xcd = w % kXcds, slot = w / kXcds;
+  const unsigned l = xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+  Tile3 t;
+  t.x = l % gx;
+  t.y = (l / gx) % gy;
+  t.z = l / (gx * gy);
+  return t;
+}
+
+__device__ inline Tile3 plain_tile(const VBlock & vb) {
+  Tile3 t;
+  t.x = vb.x;
+  t.y = vb.y;
+  t.z = vb.z;
+  return t;
+}
+
 #if NDIMS == 2
 #define KIDX 1
 #define THREAD_IJK(ilo, jlo, klo)                               \
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (ilo); \
-  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (jlo); \
+  const int i = (int)(vb.x * blockDim.x + threadIdx.x) + (ilo); \
+  const int j = (int)(vb.y * blockDim.y + threadIdx.y) + (jlo); \
   const int k = 1; (void)k;
 #else
 #define THREAD_IJK(ilo, jlo, klo)                               \
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (ilo); \
-  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (jlo); \
-  const int k = (int)blockIdx.z + (klo);
+  const int i = (int)(vb.x * blockDim.x + threadIdx.x) + (ilo); \
+  const int j = (int)(vb.y * blockDim.y + threadIdx.y) + (jlo); \
+  const int k = (int)vb.z + (klo);
 #endif
 
 // ---------------------------------------------------------------------------
@@ -72,8 +125,8 @@ struct Grid {
 // once from its fully wrapped interior source, which this kernel never
 // writes, so the copy is race free and equals the reference's y-then-z
 // exchange.
-__global__ void k_halo(double * __restrict__ a, int n0, int ny, int nz, int h) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_halo_body(const VBlock & vb, double * __restrict__ a, int n0, int ny, int nz, int h) {
+  const size_t t = (size_t)vb.x * blockDim.x + threadIdx.x;
   const int n1 = ny + 2 * h;
 #if NDIMS == 2
   const int np = 1;
@@ -113,12 +166,16 @@ __global__ void k_halo(double * __restrict__ a, int n0, int ny, int nz, int h) {
   a[x + (size_t)n0 * (r + (size_t)n1 * p)] = a[x + (size_t)n0 * (rs + (size_t)n1 * ps)];
 }
 
+__global__ void k_halo(double * __restrict__ a, int n0, int ny, int nz, int h) {
+  k_halo_body(hw_block(), a, n0, ny, nz, h);
+}
+
 // x walls on rows [r0, r0+nr) of planes [p0, p0+np): mode 0 sets Dirichlet
 // values (vlo at column xlo, vhi at xhi), mode 1 copies the neighbour
 // (Neumann, p.c:15-16)
-__global__ void k_walls(double * __restrict__ a, int n0, int n1, int r0, int nr, int p0,
+__device__ __forceinline__ void k_walls_body(const VBlock & vb, double * __restrict__ a, int n0, int n1, int r0, int nr, int p0,
                         int np, int xlo, int xhi, int mode, double vlo, double vhi) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int t = (int)(vb.x * blockDim.x + threadIdx.x);
   if (t >= nr * np) return;
   const int r = r0 + t % nr;
   const int p = p0 + t / nr;
@@ -132,11 +189,16 @@ __global__ void k_walls(double * __restrict__ a, int n0, int n1, int r0, int nr,
   }
 }
 
+__global__ void k_walls(double * __restrict__ a, int n0, int n1, int r0, int nr, int p0,
+                        int np, int xlo, int xhi, int mode, double vlo, double vhi) {
+  k_walls_body(hw_block(), a, n0, n1, r0, nr, p0, np, xlo, xhi, mode, vlo, vhi);
+}
+
 // ---------------------------------------------------------------------------
 // interface_compute_curvature_tensor: src/interface/curvature_tensor.c
 
 // compute_gradient (:14-48): corners i = 1..nx+1, j = 0..ny+2 (k = 0..nz+2)
-__global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __restrict__ dvof) {
+__device__ __forceinline__ void k_gradient_body(const VBlock & vb, Grid g, const double * __restrict__ vof, double * __restrict__ dvof) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 0, 0)
   if (i > nx + 1 || j > ny + 2) return;
@@ -172,6 +234,10 @@ __global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __r
   o[1] = gy * ninv;
   o[2] = gz * ninv;
 #endif
+}
+
+__global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __restrict__ dvof) {
+  k_gradient_body(hw_block(), g, vof, dvof);
 }
 
 // compute_intercept (:50-92): Newton on D = exp(-2 beta d), at most 8
@@ -229,7 +295,7 @@ __device__ inline double intercept(const double vof, const double * n) {
 
 // compute_normal (:94-145) + compute_curvature (:147-176), cells
 // i = 1..nx, j = 0..ny+1 (k = 0..nz+1); both read the same corner normals
-__global__ void k_normal_curv(Grid g, const double * __restrict__ vof,
+__device__ __forceinline__ void k_normal_curv_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
                               const double * __restrict__ dvof, double * __restrict__ normal,
                               double * __restrict__ curv) {
   const int nx = g.nx, ny = g.ny;
@@ -302,9 +368,15 @@ __global__ void k_normal_curv(Grid g, const double * __restrict__ vof,
 #endif
 }
 
+__global__ void k_normal_curv(Grid g, const double * __restrict__ vof,
+                              const double * __restrict__ dvof, double * __restrict__ normal,
+                              double * __restrict__ curv) {
+  k_normal_curv_body(hw_block(), g, vof, dvof, normal, curv);
+}
+
 // ---------------------------------------------------------------------------
 // interface_compute_force: src/interface/force.c:11-76 (cells 1..nx x 1..ny)
-__global__ void k_force(Grid g, double tension, const double * __restrict__ vof,
+__device__ __forceinline__ void k_force_body(const VBlock & vb, Grid g, double tension, const double * __restrict__ vof,
                         const double * __restrict__ curv, double * __restrict__ ifrcx,
                         double * __restrict__ ifrcy, double * __restrict__ ifrcz) {
   const int nx = g.nx, ny = g.ny;
@@ -332,6 +404,12 @@ __global__ void k_force(Grid g, double tension, const double * __restrict__ vof,
 #else
   (void)ifrcz;
 #endif
+}
+
+__global__ void k_force(Grid g, double tension, const double * __restrict__ vof,
+                        const double * __restrict__ curv, double * __restrict__ ifrcx,
+                        double * __restrict__ ifrcy, double * __restrict__ ifrcz) {
+  k_force_body(hw_block(), g, tension, vof, curv, ifrcx, ifrcy, ifrcz);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,7 +511,7 @@ __device__ inline double flux_z_at(const double * __restrict__ uz, const double 
 
 // thread (i, j, k) over 1..nx x 1..ny+1 x 1..nz+1 owns FLXX(i), FLXY(j),
 // FLXZ(k) where those exist
-__global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+__device__ __forceinline__ void k_flux_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                        const double * __restrict__ uz, const double * __restrict__ vof,
                        const double * __restrict__ normal, double * __restrict__ flxx,
                        double * __restrict__ flxy, double * __restrict__ flxz) {
@@ -455,9 +533,16 @@ __global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __
 #endif
 }
 
+__global__ void k_flux(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                       const double * __restrict__ uz, const double * __restrict__ vof,
+                       const double * __restrict__ normal, double * __restrict__ flxx,
+                       double * __restrict__ flxy, double * __restrict__ flxz) {
+  k_flux_body(hw_block(), g, ux, uy, uz, vof, normal, flxx, flxy, flxz);
+}
+
 // interface_compute_rhs (update/main.c:47-78) + interface_advect_vof
 // (:80-110): src_a starts from zero (reset_srcs :28-45)
-__global__ void k_vof_update(Grid g, const double * __restrict__ flxx,
+__device__ __forceinline__ void k_vof_update_body(const VBlock & vb, Grid g, const double * __restrict__ flxx,
                              const double * __restrict__ flxy, const double * __restrict__ flxz,
                              double * __restrict__ srca, const double * __restrict__ srcb,
                              double * __restrict__ vof, double dta, double dtb, int use_b) {
@@ -478,6 +563,13 @@ __global__ void k_vof_update(Grid g, const double * __restrict__ flxx,
   v += dta * s;
   if (use_b) v += dtb * srcb[is];
   vof[I_VOF(i, j, k)] = v;
+}
+
+__global__ void k_vof_update(Grid g, const double * __restrict__ flxx,
+                             const double * __restrict__ flxy, const double * __restrict__ flxz,
+                             double * __restrict__ srca, const double * __restrict__ srcb,
+                             double * __restrict__ vof, double dta, double dtb, int use_b) {
+  k_vof_update_body(hw_block(), g, flxx, flxy, flxz, srca, srcb, vof, dta, dtb, use_b);
 }
 
 // ---------------------------------------------------------------------------
@@ -577,13 +669,17 @@ __device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j
   sg_out = sg;
 }
 
-__global__ void k_rhs_ux(Grid g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs_ux_body(const VBlock & vb, Grid g, RhsArgs a) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(2, 1, 1)
   if (i > nx || j > ny) return;
   double sa, sg;
   rhs_ux_at(g, a, i, j, k, a.ifrc[I_SX(i, j, k)], sa, sg);
   rhs_store(a, I_SX(i, j, k), sa, sg);
+}
+
+__global__ void k_rhs_ux(Grid g, RhsArgs a) {
+  k_rhs_ux_body(hw_block(), g, a);
 }
 
 // compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx
@@ -646,13 +742,17 @@ __device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j
   sg_out = sg;
 }
 
-__global__ void k_rhs_uy(Grid g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs_uy_body(const VBlock & vb, Grid g, RhsArgs a) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
   double sa, sg;
   rhs_uy_at(g, a, i, j, k, a.ifrc[I_SC(i, j, k)], sa, sg);
   rhs_store(a, I_SC(i, j, k), sa, sg);
+}
+
+__global__ void k_rhs_uy(Grid g, RhsArgs a) {
+  k_rhs_uy_body(hw_block(), g, a);
 }
 
 #if NDIMS == 3
@@ -712,13 +812,17 @@ __device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j
   sg_out = sg;
 }
 
-__global__ void k_rhs_uz(Grid g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs_uz_body(const VBlock & vb, Grid g, RhsArgs a) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
   double sa, sg;
   rhs_uz_at(g, a, i, j, k, a.ifrc[I_SC(i, j, k)], sa, sg);
   rhs_store(a, I_SC(i, j, k), sa, sg);
+}
+
+__global__ void k_rhs_uz(Grid g, RhsArgs a) {
+  k_rhs_uz_body(hw_block(), g, a);
 }
 #endif
 
@@ -776,13 +880,17 @@ __device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j,
   sg_out = sg;
 }
 
-__global__ void k_rhs_t(Grid g, RhsArgs a) {
+__device__ __forceinline__ void k_rhs_t_body(const VBlock & vb, Grid g, RhsArgs a) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
   double sa, sg;
   rhs_t_at(g, a, i, j, k, 0., sa, sg);
   rhs_store(a, I_SC(i, j, k), sa, sg);
+}
+
+__global__ void k_rhs_t(Grid g, RhsArgs a) {
+  k_rhs_t_body(hw_block(), g, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -883,12 +991,12 @@ constexpr int kLLines = 256;
 constexpr int kLStride = 264;
 
 template <int CH>
-__global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
+__device__ __forceinline__ void k_lines_body(const VBlock & vb, LineJob J) {
   __shared__ double T[CH][kLStride];
   __shared__ double U[CH][kLStride];
-  const LineField & F = J.f[blockIdx.y];
+  const LineField & F = J.f[vb.y];
   const int tid = threadIdx.x;
-  const int line0 = blockIdx.x * kLLines;
+  const int line0 = vb.x * kLLines;
   const int myline = line0 + tid;
   const int n = F.n;
   const int nlines = J.nlines;
@@ -974,6 +1082,11 @@ __global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
   }
 }
 
+template <int CH>
+__global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
+  k_lines_body<CH>(hw_block(), J);
+}
+
 // Whole-line variant: a work-group keeps `lpb` complete lines in LDS (odd
 // row stride, so the lanes walking different lines hit different banks).
 // Phase 1 copies the increments (one contiguous block: the lines of a group
@@ -984,13 +1097,13 @@ __global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
 constexpr int kWThreads = 256;
 constexpr int kWUnroll = 4;
 
-__global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
+__device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, int lpb) {
   extern __shared__ double S[];
-  const LineField & F = J.f[blockIdx.y];
+  const LineField & F = J.f[vb.y];
   const int n = F.n;
   const int ld = n | 1;
   const int tid = threadIdx.x;
-  const int line0 = blockIdx.x * lpb;
+  const int line0 = vb.x * lpb;
   const int nl = min(lpb, J.nlines - line0);
   if (nl <= 0) return;
   const int total = nl * n;
@@ -1109,6 +1222,10 @@ __global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {
+  k_lines_lds_body(hw_block(), J, lpb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,7 +1368,7 @@ constexpr int kFftElems = 2048;   // complex elements per LDS buffer
 // columns are packed into one complex column; the r2c halves are separated
 // after the transform.  Output spec[x][k][ky].
 template <int ELEMS>
-__global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
+__device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb, 
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
     double2 * __restrict__ spec) {
@@ -1259,13 +1376,14 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
   const int NC = ELEMS / N;                // complex columns per block
-  const int x0 = blockIdx.x * 2 * NC;      // first interior x (0-based)
+  const Tile3 tile = xcd_tile(vb);
+  const int x0 = tile.x * 2 * NC;          // first interior x (0-based)
 #if NDIMS == 2
   const int k = 1;
   const int nz = 1;
   (void)uz;
 #else
-  const int k = (int)blockIdx.y + 1;
+  const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
   for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
@@ -1302,16 +1420,25 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
   }
 }
 
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
+    Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
+    const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
+    double2 * __restrict__ spec) {
+  k_div_rfft_y_body<ELEMS>(hw_block(), g, P, tw, ux, uy, uz, prefactor, spec);
+}
+
 #if NDIMS == 3
 // complex transform in z of spec[x][kz][ky] for a run of ky at one x
 template <int ELEMS, class T2>
-__global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const T2 * __restrict__ tw,
+__device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, FftPlan P, const T2 * __restrict__ tw,
                                                       T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
   __shared__ T2 sa[ELEMS], sb[ELEMS];
   const int N = g.nz;
   const int NC = ELEMS / N;
-  const int ky0 = blockIdx.x * NC;
-  const int x = blockIdx.y;
+  const Tile3 tile = xcd_tile(vb);
+  const int ky0 = tile.x * NC;
+  const int x = tile.y;
   T2 * base = spec + (size_t)x * N * nh;
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
     const int c = e % NC, kz = e / NC;
@@ -1325,6 +1452,12 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const 
     const int ky = ky0 + c;
     if (ky < nh) base[(size_t)kz * nh + ky] = res[c * N + kz];
   }
+}
+
+template <int ELEMS, class T2>
+__global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const T2 * __restrict__ tw,
+                                                      T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
+  k_fft_z_body<ELEMS, T2>(hw_block(), g, P, tw, spec, nh, sign);
 }
 #endif
 
@@ -1344,11 +1477,11 @@ template <> __device__ inline float2 from_d2<float2>(double2 a) { return make_fl
 // T2 is the storage type of the spectrum (double2, or float2 in the
 // mixed-precision solver); the elimination itself is FP64 either way
 template <class T2>
-__global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
+__device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
                               double * __restrict__ vbuf) {
-  const int line = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int line = (int)(vb.x * blockDim.x + threadIdx.x);
   if (line >= nlines) return;
   const int ky = line % nh;
 #if NDIMS == 2
@@ -1468,23 +1601,32 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
   }
 }
 
+template <class T2>
+__global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
+                              const double * __restrict__ pu, const double * __restrict__ ey,
+                              const double * __restrict__ ez, T2 * __restrict__ spec,
+                              double * __restrict__ vbuf) {
+  k_thomas_spec_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf);
+}
+
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and
 // k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
 template <int ELEMS>
-__global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
+__device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb, 
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
     double * __restrict__ psi, int images) {
   __shared__ double2 sa[ELEMS], sb[ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
   const int NC = ELEMS / N;
-  const int x0 = blockIdx.x * 2 * NC;
+  const Tile3 tile = xcd_tile(vb);
+  const int x0 = tile.x * 2 * NC;
   const int nh = N / 2 + 1;
 #if NDIMS == 2
   const int k = 1;
   const int nz = 1;
 #else
-  const int k = (int)blockIdx.y + 1;
+  const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
@@ -1521,6 +1663,13 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
   }
 }
 
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
+    Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
+    double * __restrict__ psi, int images) {
+  k_irfft_y_psi_body<ELEMS>(hw_block(), g, P, tw, spec, psi, images);
+}
+
 // ---------------------------------------------------------------------------
 // Mixed-precision Poisson solve: the transforms run in FP32 on an FP32
 // spectrum, the x eliminations in FP64 (k_thomas_spec<float2>), and FP64
@@ -1534,7 +1683,7 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
 // mode 0: r = prefactor * div u, stored in rhs; mode 1: r = rhs - A psi.
 // Then (float)(r * scale) is transformed in y as k_div_rfft_y does.
 template <int ELEMS>
-__global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
+__device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb, 
     Grid g, FftPlan P, const float2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
     double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
@@ -1543,13 +1692,14 @@ __global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
   const int NC = ELEMS / N;
-  const int x0 = blockIdx.x * 2 * NC;
+  const Tile3 tile = xcd_tile(vb);
+  const int x0 = tile.x * 2 * NC;
 #if NDIMS == 2
   const int k = 1;
   const int nz = 1;
   (void)uz;
 #else
-  const int k = (int)blockIdx.y + 1;
+  const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
   const double rdy2 = 1. / (g.dy * g.dy);
@@ -1603,23 +1753,33 @@ __global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
   }
 }
 
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
+    Grid g, FftPlan P, const float2 * __restrict__ tw, const double * __restrict__ ux,
+    const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
+    double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
+    const double * __restrict__ pu, int mode, float2 * __restrict__ spec) {
+  k_res_rfft32_y_body<ELEMS>(hw_block(), g, P, tw, ux, uy, uz, prefactor, scale, rhs, psi, pl, pu, mode, spec);
+}
+
 // inverse real transform in y in FP32; psi = delta (accumulate 0) or
 // psi += delta (accumulate 1) in FP64, with halo images and Neumann walls
 template <int ELEMS>
-__global__ __launch_bounds__(kFftThreads) void k_irfft32_y_psi(
+__device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb, 
     Grid g, FftPlan P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
     double * __restrict__ psi, int accumulate) {
   __shared__ float2 sa[ELEMS], sb[ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
   const int NC = ELEMS / N;
-  const int x0 = blockIdx.x * 2 * NC;
+  const Tile3 tile = xcd_tile(vb);
+  const int x0 = tile.x * 2 * NC;
   const int nh = N / 2 + 1;
 #if NDIMS == 2
   const int k = 1;
   const int nz = 1;
 #else
-  const int k = (int)blockIdx.y + 1;
+  const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
@@ -1651,9 +1811,16 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft32_y_psi(
   }
 }
 
+template <int ELEMS>
+__global__ __launch_bounds__(kFftThreads) void k_irfft32_y_psi(
+    Grid g, FftPlan P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
+    double * __restrict__ psi, int accumulate) {
+  k_irfft32_y_psi_body<ELEMS>(hw_block(), g, P, tw, spec, psi, accumulate);
+}
+
 // ---------------------------------------------------------------------------
 // fluid_correct_velocity (correct_velocity/{ux,uy}.c) for one component
-__global__ void k_correct(Grid g, double prefactor, const double * __restrict__ psi,
+__device__ __forceinline__ void k_correct_body(const VBlock & vb, Grid g, double prefactor, const double * __restrict__ psi,
                           double * __restrict__ ux, double * __restrict__ uy,
                           double * __restrict__ uz) {
   const int nx = g.nx, ny = g.ny;
@@ -1672,9 +1839,15 @@ __global__ void k_correct(Grid g, double prefactor, const double * __restrict__ 
 #endif
 }
 
+__global__ void k_correct(Grid g, double prefactor, const double * __restrict__ psi,
+                          double * __restrict__ ux, double * __restrict__ uy,
+                          double * __restrict__ uz) {
+  k_correct_body(hw_block(), g, prefactor, psi, ux, uy, uz);
+}
+
 // fluid_update_pressure (update_pressure.c:12-109): p += psi, then the
 // implicit-x correction
-__global__ void k_pressure(Grid g, double prefactor, const double * __restrict__ psi,
+__device__ __forceinline__ void k_pressure_body(const VBlock & vb, Grid g, double prefactor, const double * __restrict__ psi,
                            double * __restrict__ p) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
@@ -1687,6 +1860,11 @@ __global__ void k_pressure(Grid g, double prefactor, const double * __restrict__
   const double dpsidx_xp = (-pc + pp) / g.dxc[i];
   v -= prefactor / g.dxf[i - 1] * (-dpsidx_xm + dpsidx_xp);
   p[I_C1(i, j, k)] = v;
+}
+
+__global__ void k_pressure(Grid g, double prefactor, const double * __restrict__ psi,
+                           double * __restrict__ p) {
+  k_pressure_body(hw_block(), g, prefactor, psi, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1704,7 +1882,7 @@ __device__ inline double block_min(double v) {
   return red[0];
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double * __restrict__ ux,
+__device__ __forceinline__ void k_dt_partial_body(const VBlock & vb, Grid g, const double * __restrict__ ux,
                                                             const double * __restrict__ uy,
                                                             const double * __restrict__ uz,
                                                             double * __restrict__ part) {
@@ -1717,8 +1895,8 @@ __global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double
 #endif
   const double small = 1.e-8;
   double m = 1.;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
-       t += (size_t)gridDim.x * blockDim.x) {
+  for (size_t t = (size_t)vb.x * blockDim.x + threadIdx.x; t < ncell;
+       t += (size_t)vb.gx * blockDim.x) {
     const int i = (int)(t % nx) + 1;
     const size_t r = t / nx;
     const int j = (int)(r % ny) + 1;
@@ -1734,15 +1912,27 @@ __global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double
 #endif
   }
   m = block_min(m);
-  if (threadIdx.x == 0) part[blockIdx.x] = m;
+  if (threadIdx.x == 0) part[vb.x] = m;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_min_final(const double * __restrict__ part, int n,
+__global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double * __restrict__ ux,
+                                                            const double * __restrict__ uy,
+                                                            const double * __restrict__ uz,
+                                                            double * __restrict__ part) {
+  k_dt_partial_body(hw_block(), g, ux, uy, uz, part);
+}
+
+__device__ __forceinline__ void k_min_final_body(const VBlock & vb, const double * __restrict__ part, int n,
                                                            double * __restrict__ out) {
   double m = 1.;
   for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmin(m, part[t]);
   m = block_min(m);
   if (threadIdx.x == 0) out[0] = m;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_min_final(const double * __restrict__ part, int n,
+                                                           double * __restrict__ out) {
+  k_min_final_body(hw_block(), part, n, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1753,7 +1943,7 @@ __global__ __launch_bounds__(kRedThreads) void k_min_final(const double * __rest
 // 12 interface measure sum |grad vof| dV
 constexpr int kDiagN = 13;
 
-__global__ __launch_bounds__(kRedThreads) void k_diag_partial(
+__device__ __forceinline__ void k_diag_partial_body(const VBlock & vb, 
     Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
     const double * __restrict__ uz, const double * __restrict__ t,
     const double * __restrict__ vof, double * __restrict__ part) {
@@ -1768,8 +1958,8 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
 #endif
   double q[kDiagN];
   for (int m = 0; m < kDiagN; m++) q[m] = m == 1 ? 1. : 0.;
-  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < ncell;
-       s += (size_t)gridDim.x * blockDim.x) {
+  for (size_t s = (size_t)vb.x * blockDim.x + threadIdx.x; s < ncell;
+       s += (size_t)vb.gx * blockDim.x) {
     const int i = (int)(s % nx) + 1;
     const size_t r = s / nx;
     const int j = (int)(r % ny) + 1;
@@ -1829,7 +2019,14 @@ __global__ __launch_bounds__(kRedThreads) void k_diag_partial(
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < kDiagN; m++) part[(size_t)blockIdx.x * kDiagN + m] = red[m][0];
+    for (int m = 0; m < kDiagN; m++) part[(size_t)vb.x * kDiagN + m] = red[m][0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_diag_partial(
+    Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+    const double * __restrict__ uz, const double * __restrict__ t,
+    const double * __restrict__ vof, double * __restrict__ part) {
+  k_diag_partial_body(hw_block(), g, ux, uy, uz, t, vof, part);
 }
 
 // logging/nusselt: partial sums of {wall heat flux, kinetic energy
@@ -1840,7 +2037,7 @@ constexpr int kNuN = 4;
 
 __device__ inline double sq(double a) { return a * a; }
 
-__global__ __launch_bounds__(kRedThreads) void k_nusselt_partial(
+__device__ __forceinline__ void k_nusselt_partial_body(const VBlock & vb, 
     Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
     const double * __restrict__ uz, const double * __restrict__ t, double md, double td,
     double * __restrict__ part) {
@@ -1859,7 +2056,7 @@ __global__ __launch_bounds__(kRedThreads) void k_nusselt_partial(
   const double * dxc = g.dxc;
   double q[kNuN] = {0., 0., 0., 0.};
   const size_t n = (size_t)(nx + 1) * nrow;
-  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+  for (size_t s = (size_t)vb.x * blockDim.x + threadIdx.x; s < n; s += (size_t)vb.gx * blockDim.x) {
     const int i = (int)(s % (size_t)(nx + 1)) + 1;
     const size_t r = s / (size_t)(nx + 1);
     const int j = (int)(r % ny) + 1;
@@ -1934,7 +2131,14 @@ __global__ __launch_bounds__(kRedThreads) void k_nusselt_partial(
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < kNuN; m++) part[(size_t)blockIdx.x * kNuN + m] = red[m][0];
+    for (int m = 0; m < kNuN; m++) part[(size_t)vb.x * kNuN + m] = red[m][0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_nusselt_partial(
+    Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+    const double * __restrict__ uz, const double * __restrict__ t, double md, double td,
+    double * __restrict__ part) {
+  k_nusselt_partial_body(hw_block(), g, ux, uy, uz, t, md, td, part);
 }
 
 // statistics.c:117-197: add one sample to the accumulators; faces
@@ -1944,12 +2148,12 @@ struct StatAcc {
   double * a[9];   // ux1 ux2 uy1 uy2 t1 t2 uxt uz1 uz2
 };
 
-__global__ void k_stats_collect(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+__device__ __forceinline__ void k_stats_collect_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                                 const double * __restrict__ uz, const double * __restrict__ t, StatAcc acc) {
   const int nx = g.nx, ny = g.ny;
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);   // 0..nx+1
-  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + 1;
-  const int k = (int)blockIdx.z + 1;
+  const int i = (int)(vb.x * blockDim.x + threadIdx.x);   // 0..nx+1
+  const int j = (int)(vb.y * blockDim.y + threadIdx.y) + 1;
+  const int k = (int)vb.z + 1;
   if (i > nx + 1 || j > ny) return;
   const size_t r = (size_t)(j - 1) + (size_t)ny * (size_t)(k - 1);
   if (i >= 1) {
@@ -1976,19 +2180,32 @@ __global__ void k_stats_collect(Grid g, const double * __restrict__ ux, const do
 #endif
 }
 
+__global__ void k_stats_collect(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                                const double * __restrict__ uz, const double * __restrict__ t, StatAcc acc) {
+  k_stats_collect_body(hw_block(), g, ux, uy, uz, t, acc);
+}
+
 // statistics.c:232-275: sum over the rows in row order, one lane per x entry
-__global__ void k_stats_reduce(const double * __restrict__ a, int w, size_t rows, double * __restrict__ out) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void k_stats_reduce_body(const VBlock & vb, const double * __restrict__ a, int w, size_t rows, double * __restrict__ out) {
+  const int i = (int)(vb.x * blockDim.x + threadIdx.x);
   if (i >= w) return;
   double s = 0.;
   for (size_t r = 0; r < rows; r++) s += a[(size_t)i + (size_t)w * r];
   out[i] = s;
 }
 
+__global__ void k_stats_reduce(const double * __restrict__ a, int w, size_t rows, double * __restrict__ out) {
+  k_stats_reduce_body(hw_block(), a, w, rows, out);
+}
+
 // plain copy, the calibration kernel for hardware byte counters
-__global__ void k_copy(const double * __restrict__ a, double * __restrict__ b, size_t n) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void k_copy_body(const VBlock & vb, const double * __restrict__ a, double * __restrict__ b, size_t n) {
+  for (size_t t = (size_t)vb.x * blockDim.x + threadIdx.x; t < n; t += (size_t)vb.gx * blockDim.x)
     b[t] = a[t];
+}
+
+__global__ void k_copy(const double * __restrict__ a, double * __restrict__ b, size_t n) {
+  k_copy_body(hw_block(), a, b, n);
 }
 
 
@@ -2019,7 +2236,7 @@ constexpr int kIX = 32, kIY = 4, kIZ = 4;
 #endif
 constexpr int kIThreads = kIX * kIY * kIZ;
 
-__global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double * __restrict__ vof,
+__device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
                                                            double * __restrict__ normal,
                                                            double * __restrict__ curv) {
   constexpr int VX = kIX + 2, VY = kIY + 2;
@@ -2032,10 +2249,11 @@ __global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double 
   __shared__ double sv[VX * VY * VZ];
   __shared__ double sc[NDIMS][CX * CY * CZ];
   const int nx = g.nx, ny = g.ny;
-  const int i0 = 1 + blockIdx.x * kIX, j0 = 1 + blockIdx.y * kIY;
+  const Tile3 tile = xcd_tile(vb);
+  const int i0 = 1 + tile.x * kIX, j0 = 1 + tile.y * kIY;
 #if NDIMS == 3
   const int nz = g.nz;
-  const int k0 = 1 + blockIdx.z * kIZ;
+  const int k0 = 1 + tile.z * kIZ;
 #else
   const int nz = 1;
   const int k0 = 1;
@@ -2168,10 +2386,16 @@ __global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double 
   }
 }
 
+__global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double * __restrict__ vof,
+                                                           double * __restrict__ normal,
+                                                           double * __restrict__ curv) {
+  k_iface_fused_body(hw_block(), g, vof, normal, curv);
+}
+
 // interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
 // (compute_flux_*), the flux divergence and the RK update, written to the
 // other VOF buffer with its halo images (walls stay zero).
-__global__ void k_vof_fused(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+__device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                             const double * __restrict__ uz, const double * __restrict__ vof,
                             const double * __restrict__ normal, double * __restrict__ srca,
                             const double * __restrict__ srcb, double * __restrict__ vof_new,
@@ -2205,6 +2429,14 @@ __global__ void k_vof_fused(Grid g, const double * __restrict__ ux, const double
   store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
 }
 
+__global__ __launch_bounds__(256) void k_vof_fused(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                            const double * __restrict__ uz, const double * __restrict__ vof,
+                            const double * __restrict__ normal, double * __restrict__ srca,
+                            const double * __restrict__ srcb, double * __restrict__ vof_new,
+                            double dta, double dtb, int use_b) {
+  k_vof_fused_body(hw_block(), g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b);
+}
+
 // fluid_predict_field sources of all fields at once, with the CSF force
 // (force.c:11-66) formed in place from vof and curvature; writes srca and the
 // increment of every DOF
@@ -2220,7 +2452,7 @@ struct FusedRhs {
   double * delta[4];
 };
 
-__global__ void k_rhs_fused(Grid g, FusedRhs f) {
+__device__ __forceinline__ void k_rhs_fused_body(const VBlock & vb, Grid g, FusedRhs f) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
@@ -2271,11 +2503,15 @@ __global__ void k_rhs_fused(Grid g, FusedRhs f) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_rhs_fused(Grid g, FusedRhs f) {
+  k_rhs_fused_body(hw_block(), g, f);
+}
+
 __device__ inline double prefactor_div(double a, double b) { return a / b; }
 
 // fluid_correct_velocity + fluid_update_pressure fused, with halo images
 // and the Neumann walls of p
-__global__ void k_corr_press(Grid g, double pref_c, double pref_p, const double * __restrict__ psi,
+__device__ __forceinline__ void k_corr_press_body(const VBlock & vb, Grid g, double pref_c, double pref_p, const double * __restrict__ psi,
                              double * __restrict__ ux, double * __restrict__ uy,
                              double * __restrict__ uz, double * __restrict__ p) {
   const int nx = g.nx, ny = g.ny;
@@ -2314,6 +2550,12 @@ __global__ void k_corr_press(Grid g, double pref_c, double pref_p, const double 
     store_periodic(p, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
     neumann_walls(p, i, nx, ny + 2, j, PLANE1(k), ny, nz, v);
   }
+}
+
+__global__ __launch_bounds__(256) void k_corr_press(Grid g, double pref_c, double pref_p, const double * __restrict__ psi,
+                             double * __restrict__ ux, double * __restrict__ uy,
+                             double * __restrict__ uz, double * __restrict__ p) {
+  k_corr_press_body(hw_block(), g, pref_c, pref_p, psi, ux, uy, uz, p);
 }
 
 }  // namespace vofk

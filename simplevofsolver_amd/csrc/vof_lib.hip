@@ -14,10 +14,12 @@
 #include <sys/stat.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "vof_solver.h"
 #include "vof_kernels.h"
+#include "vof_adjoint.h"
 
 using vofk::Grid;
 using vofk::FftPlan;
@@ -202,7 +204,13 @@ static const char * const kParamNames[P_COUNT] = {
   "uz_xm", "uz_xp", "t_xm", "t_xp",
 };
 
+struct Recorder;
+
 struct vof_ctx {
+  Recorder * rec;              // non-null: launches are recorded for a batch, not issued
+  bool tab_shared;             // Thomas tables live in a batch's block
+  double ** tape;              // non-null: stage states (vof, ux, uy) are copied here per RK stage
+  double adj_floor;            // |grad vof| dx below which corner normals are constants in the adjoint
   int nx, ny, nz;              // nz = 1 in 2D
   double dy, dz;
   bool x_uniform;
@@ -295,6 +303,7 @@ static dim3 grid3(int ni, int nj, int nk, dim3 blk) {
 
 static const dim3 kBlk(64, 4, 1);
 
+
 static int prof_flush(vof_ctx * ctx) {
   if (ctx->pending.empty()) return 0;
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -331,9 +340,182 @@ static int prof_end(vof_ctx * ctx, const char * name, hipEvent_t e0) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched execution.  A batch advances many independent cases (each a full
+// context of its own, any sizes) with one launch per kernel of the step.  The
+// host code of the step runs once per case in record mode: every launch is
+// captured with its grid and packed arguments instead of being issued.  The
+// i-th launches of all cases (the same kernel instantiation) then become one
+// launch of k_batched, whose work-groups are the concatenation of the cases'
+// work-groups; each work-group finds its case by binary search over the
+// prefix sums and runs the kernel body with that case's arguments and its
+// own tile coordinates.  Results are those of the single-case launches, bit
+// for bit: every tile executes the same body on the same data.
+
+template <class... A> struct Pack;
+template <> struct Pack<> {};
+template <class H, class... T> struct Pack<H, T...> {
+  H h;
+  Pack<T...> t;
+};
+
+template <size_t I, class H, class... T>
+__host__ __device__ constexpr const auto & pack_get(const Pack<H, T...> & p) {
+  if constexpr (I == 0)
+    return p.h;
+  else
+    return pack_get<I - 1>(p.t);
+}
+
+template <class H, class... T>
+static Pack<H, T...> make_pack(H h, T... t) {
+  Pack<H, T...> p;
+  p.h = h;
+  if constexpr (sizeof...(T) > 0) p.t = make_pack(t...);
+  return p;
+}
+
+template <class... A> struct BatchEntry {
+  unsigned first;              // first work-group of this case in the batched launch
+  unsigned gx, gy, gz;         // the case's own grid
+  Pack<A...> args;
+};
+
+template <auto Body, class... A, size_t... I>
+__device__ __forceinline__ void apply_body(const vofk::VBlock & vb, const Pack<A...> & p,
+                                           std::index_sequence<I...>) {
+  Body(vb, pack_get<I>(p)...);
+}
+
+template <auto Body, unsigned BOUNDS, class... A>
+__global__ __launch_bounds__(BOUNDS) void k_batched(const BatchEntry<A...> * __restrict__ e, int ncases) {
+  int lo = 0, hi = ncases - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (e[mid].first <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchEntry<A...> & E = e[lo];
+  const unsigned l = blockIdx.x - E.first;
+  vofk::VBlock vb;
+  vb.gx = E.gx;
+  vb.gy = E.gy;
+  vb.gz = E.gz;
+  vb.x = l % E.gx;
+  vb.y = (l / E.gx) % E.gy;
+  vb.z = l / (E.gx * E.gy);
+  apply_body<Body>(vb, E.args, std::index_sequence_for<A...>{});
+}
+
+struct vof_batch;
+
+// one recorded launch; `merge` is specific to the kernel instantiation and
+// doubles as its identity
+struct LaunchRec {
+  size_t (*merge)(vof_batch * B, size_t idx, unsigned char * host, const unsigned char * dev);
+  const char * name;
+  dim3 grid, block;
+  size_t shm;
+  std::vector<unsigned char> pack;
+};
+
+struct Recorder {
+  std::vector<LaunchRec> recs;
+};
+
+struct vof_batch {
+  std::vector<vof_ctx *> ctx;
+  std::vector<const domain_t *> domain;
+  std::vector<fluid_t *> fluid;
+  std::vector<interface_t *> interface;
+  std::vector<Recorder> rec;
+  hipStream_t stream;
+  // staging of the batched launches' argument tables (pinned + device)
+  unsigned char * h_stage;
+  unsigned char * d_stage;
+  size_t stage_bytes;
+  // Thomas tables of every case, contiguous (cases re-pointed into these)
+  double * h_tab;
+  double * d_tab;
+  size_t tab_doubles;
+  // dt gather
+  double * d_dt;
+  double * h_dt;
+  const double ** d_dt_src;
+  hipEvent_t stage_done;
+};
+
+// build (dev == nullptr: size only / fill host) or launch (dev != nullptr)
+// the merged launch number idx; returns the bytes of its argument table
+template <auto Body, unsigned BOUNDS, class... A>
+static size_t merge_launch(vof_batch * B, size_t idx, unsigned char * host, const unsigned char * dev) {
+  using E = BatchEntry<A...>;
+  const size_t n = B->ctx.size();
+  const size_t bytes = sizeof(E) * n;
+  if (!dev) {
+    if (host) {
+      unsigned first = 0;
+      for (size_t c = 0; c < n; c++) {
+        const LaunchRec & r = B->rec[c].recs[idx];
+        E e;
+        e.first = first;
+        e.gx = r.grid.x;
+        e.gy = r.grid.y;
+        e.gz = r.grid.z;
+        memcpy(&e.args, r.pack.data(), sizeof(Pack<A...>));
+        memcpy(host + sizeof(E) * c, &e, sizeof(E));
+        first += r.grid.x * r.grid.y * r.grid.z;
+      }
+    }
+    return bytes;
+  }
+  unsigned total = 0;
+  size_t shm = 0;
+  for (size_t c = 0; c < n; c++) {
+    const LaunchRec & r = B->rec[c].recs[idx];
+    total += r.grid.x * r.grid.y * r.grid.z;
+    shm = r.shm > shm ? r.shm : shm;
+  }
+  const dim3 blk = B->rec[0].recs[idx].block;
+  hipLaunchKernelGGL((k_batched<Body, BOUNDS, A...>), dim3(total), blk, shm, B->stream,
+                     reinterpret_cast<const E *>(dev), (int)n);
+  return hipGetLastError() == hipSuccess ? bytes : 0;
+}
+
+template <auto Body, unsigned BOUNDS, class... A>
+static void record_launch(Recorder * rec, const char * name, dim3 g, dim3 b, size_t shm, A... args) {
+  LaunchRec r;
+  r.merge = &merge_launch<Body, BOUNDS, A...>;
+  r.name = name;
+  r.grid = g;
+  r.block = b;
+  r.shm = shm;
+  const Pack<A...> p = make_pack(args...);
+  r.pack.resize(sizeof(p));
+  memcpy(r.pack.data(), &p, sizeof(p));
+  rec->recs.push_back(std::move(r));
+}
+
+// launch of a batchable kernel: BODY is its device body, BOUNDS its
+// work-group size bound; recorded instead of issued in record mode
+#define KLB(ctx, name, BOUNDS, BODY, kern, g, b, shm, ...)                           \
+  do {                                                                             \
+    if ((ctx)->rec) {                                                              \
+      record_launch<BODY, BOUNDS>((ctx)->rec, name, g, b, shm, __VA_ARGS__);        \
+    } else {                                                                       \
+      hipEvent_t e0_ = prof_begin(ctx);                                            \
+      hipLaunchKernelGGL(kern, g, b, shm, (ctx)->stream, __VA_ARGS__);             \
+      LAUNCHCHK();                                                                 \
+      if (e0_ && prof_end(ctx, name, e0_)) return 1;                               \
+    }                                                                              \
+  } while (0)
+
 // launch on the context stream, bracketed by events when profiling is on
 #define KLN(ctx, name, kern, g, b, ...)                                            \
   do {                                                                             \
+    if ((ctx)->rec) return fail("%s cannot run inside a batched step", name);      \
     hipEvent_t e0_ = prof_begin(ctx);                                              \
     hipLaunchKernelGGL(kern, g, b, 0, (ctx)->stream, __VA_ARGS__);                 \
     LAUNCHCHK();                                                                   \
@@ -639,6 +821,7 @@ static int ctx_create(domain_t * domain) {
   ctx->fuse = 63;
   ctx->fft_elems = 1024;
   ctx->poisson_refine = -1;
+  ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
   ctx->d_vof_spare = nullptr;
   ctx->vof_bytes = 0;
@@ -820,7 +1003,11 @@ extern "C" int domain_finalise(domain_t * domain) {
   if (ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     dev_free(ctx->d_dxf); dev_free(ctx->d_dxc); dev_free(ctx->d_lapx_ux); dev_free(ctx->d_lapx_c);
-    dev_free(ctx->d_tab); host_free(ctx->h_tab);
+    // tables inside a live batch's block belong to the batch
+    if (!ctx->tab_shared) {
+      dev_free(ctx->d_tab);
+      host_free(ctx->h_tab);
+    }
     for (int m = 0; m < 12; m++) (void)hipEventDestroy(ctx->tab_done[m]);
     dev_free(ctx->d_twy); if (ctx->d_twz) dev_free(ctx->d_twz);
     dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez);
@@ -978,20 +1165,21 @@ static int decide_dt_enqueue(vof_ctx * ctx, const fluid_t * fluid) {
 #else
   const double * uz = nullptr;
 #endif
-  KLN(ctx, "k_dt_partial", vofk::k_dt_partial, dim3(nb), dim3(vofk::kRedThreads),
-                     ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data,
-                     uz, ctx->d_part);
-  LAUNCHCHK();
-  KLN(ctx, "k_min_final", vofk::k_min_final, dim3(1), dim3(vofk::kRedThreads),
-                     ctx->d_part, nb, ctx->d_part + ctx->npart);
-  LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(ctx->h_scalar, ctx->d_part + ctx->npart, sizeof(double),
-                        hipMemcpyDeviceToHost, ctx->stream));
+  KLB(ctx, "k_dt_partial", vofk::kRedThreads, vofk::k_dt_partial_body, vofk::k_dt_partial, dim3(nb),
+      dim3(vofk::kRedThreads), 0, ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data,
+      uz, ctx->d_part);
+  KLB(ctx, "k_min_final", vofk::kRedThreads, vofk::k_min_final_body, vofk::k_min_final, dim3(1),
+      dim3(vofk::kRedThreads), 0, (const double *)ctx->d_part, nb, ctx->d_part + ctx->npart);
+  // a batch gathers the minima of all its cases itself
+  if (!ctx->rec)
+    HIPCHK(hipMemcpyAsync(ctx->h_scalar, ctx->d_part + ctx->npart, sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
   return 0;
 }
 
-static int decide_dt_finish(vof_ctx * ctx, const domain_t * domain, const fluid_t * fluid, double * dt) {
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+static int decide_dt_finish(vof_ctx * ctx, const domain_t * domain, const fluid_t * fluid, double * dt,
+                            bool sync = true) {
+  if (sync) HIPCHK(hipStreamSynchronize(ctx->stream));
   double coef_adv = 0., coef_dif = 0.;
   if (param_get(ctx, P_COEF_ADV, &coef_adv)) return 1;
   if (param_get(ctx, P_COEF_DIF, &coef_dif)) return 1;
@@ -1066,8 +1254,14 @@ static int upload_tab(vof_ctx * ctx, int field, int rkstep, double prefactor, co
   const int slot = field * 3 + rkstep;
   double * h = ctx->h_tab + ctx->tab_stride * slot;
   double * d = ctx->d_tab + ctx->tab_stride * slot;
-  HIPCHK(hipEventSynchronize(ctx->tab_done[slot]));
   const int n = field == 0 ? ctx->nx - 1 : ctx->nx;
+  if (ctx->rec) {
+    // batch: the tables of all cases travel in one copy before the step
+    build_tab(h, n, prefactor, field == 0 ? ctx->lapx_ux.data() : ctx->lapx_c.data());
+    *dtab = d;
+    return 0;
+  }
+  HIPCHK(hipEventSynchronize(ctx->tab_done[slot]));
   build_tab(h, n, prefactor, field == 0 ? ctx->lapx_ux.data() : ctx->lapx_c.data());
   HIPCHK(hipMemcpyAsync(d, h, sizeof(double) * ctx->tab_stride, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipEventRecord(ctx->tab_done[slot], ctx->stream));
@@ -1166,7 +1360,8 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
       fr.srcb[kind] = (const double *)jobs[f].src[1].data;
       fr.delta[kind] = ctx->d_delta + (size_t)f * ncell;
     }
-    KLN(ctx, "k_rhs_fused", vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid, fr);
+    KLB(ctx, "k_rhs_fused", 256, vofk::k_rhs_fused_body, vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
+        ctx->grid, fr);
   }
   for (int f = 0; f < nj; f++) {
     const int kind = jobs[f].kind;
@@ -1210,10 +1405,8 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     if ((size_t)lpb * (nx | 1) * sizeof(double) > 65536) lpb = (int)(65536 / (sizeof(double) * (nx | 1)));
     const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
     const dim3 lg((unsigned)((lj.nlines + lpb - 1) / lpb), (unsigned)nj);
-    hipEvent_t e0_ = prof_begin(ctx);
-    hipLaunchKernelGGL(vofk::k_lines_lds, lg, dim3(vofk::kWThreads), shm, ctx->stream, lj, lpb);
-    LAUNCHCHK();
-    if (e0_ && prof_end(ctx, "k_lines_lds", e0_)) return 1;
+    KLB(ctx, "k_lines_lds", vofk::kWThreads, vofk::k_lines_lds_body, vofk::k_lines_lds, lg,
+        dim3(vofk::kWThreads), shm, lj, lpb);
   }
   if (images) return 0;
   for (int f = 0; f < nj; f++) {
@@ -1257,22 +1450,22 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
   double * psi = (double *)fluid->psi.data;
   const int nlines = ctx->nh * nz;
   for (int pass = 0; pass <= ctx->poisson_refine; pass++) {
-    KLN(ctx, "k_res_rfft32_y", vofk::k_res_rfft32_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy32, (const double *)fluid->ux.data,
+    KLB(ctx, "k_res_rfft32_y", vofk::kFftThreads, vofk::k_res_rfft32_y_body<2048>, vofk::k_res_rfft32_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy32, (const double *)fluid->ux.data,
         (const double *)fluid->uy.data, uz, prefactor, scale, ctx->d_delta, (const double *)psi, ctx->d_pl,
         ctx->d_pu, pass == 0 ? 0 : 1, ctx->d_spec32);
 #if NDIMS == 3
-    KLN(ctx, "k_fft32_z", (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z,
+    KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z,
         ctx->d_twz32, ctx->d_spec32, ctx->nh, -1.f);
 #endif
-    KLN(ctx, "k_thomas_spec32", vofk::k_thomas_spec<float2>, dim3((nlines + 63) / 64), dim3(64), nx, nlines,
+    KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>, dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines,
         ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec32, ctx->d_vbuf);
 #if NDIMS == 3
-    KLN(ctx, "k_fft32_z", (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z,
+    KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z,
         ctx->d_twz32, ctx->d_spec32, ctx->nh, +1.f);
 #endif
-    KLN(ctx, "k_irfft32_y_psi", vofk::k_irfft32_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi,
+    KLB(ctx, "k_irfft32_y_psi", vofk::kFftThreads, vofk::k_irfft32_y_psi_body<2048>, vofk::k_irfft32_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi,
         pass == 0 ? 0 : 1);
   }
   return 0;
@@ -1294,11 +1487,11 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const double * uz = nullptr;
 #endif
   if (elems == 1024) {
-    KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y<1024>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<1024>, vofk::k_div_rfft_y<1024>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
         ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
         (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
   } else {
-    KLN(ctx, "k_div_rfft_y", vofk::k_div_rfft_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads),
+    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<2048>, vofk::k_div_rfft_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
         ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
         (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
   }
@@ -1306,32 +1499,32 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const int NCz = elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
   if (elems == 1024) {
-    KLN(ctx, "k_fft_z", (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2>), (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, -1.);
   } else {
-    KLN(ctx, "k_fft_z", (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, double2>), (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, -1.);
   }
 #endif
   const int nlines = ctx->nh * nz;
-  KLN(ctx, "k_thomas_spec", vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64), dim3(64), nx, nlines,
+  KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines,
       ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
 #if NDIMS == 3
   if (elems == 1024) {
-    KLN(ctx, "k_fft_z", (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2>), (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, +1.);
   } else {
-    KLN(ctx, "k_fft_z", (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), ctx->grid, ctx->plan_z, ctx->d_twz,
+    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, double2>), (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
         ctx->d_spec, ctx->nh, +1.);
   }
 #endif
   if (elems == 1024) {
-    KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi<1024>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
+    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<1024>, vofk::k_irfft_y_psi<1024>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
         images);
   } else {
-    KLN(ctx, "k_irfft_y_psi", vofk::k_irfft_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
+    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<2048>, vofk::k_irfft_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
+        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
         images);
   }
   return images ? 0 : bc_neumann(ctx, (double *)fluid->psi.data);
@@ -1507,30 +1700,52 @@ extern "C" int interface_update_vof(const domain_t * domain, const size_t rkstep
 
 enum { F_IFACE = 1, F_VOF = 2, F_RHS = 4, F_LINES = 8, F_PSI = 16, F_CORR = 32 };
 
+// the VOF ping-pong buffer of the fused VOF update
+static int ensure_vof_spare(vof_ctx * ctx, const interface_t * interface) {
+  if (ctx->d_vof_spare) return 0;
+  ctx->vof_bytes = interface->vof.datasize;
+  HIPCHK(hipMalloc(&ctx->d_vof_spare, ctx->vof_bytes));
+  HIPCHK(hipMemsetAsync(ctx->d_vof_spare, 0, ctx->vof_bytes, ctx->stream));
+  return 0;
+}
+
+static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
+                                  const double dtv);
+
 static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
+  vof_ctx * ctx = ctx_of(domain);
+  if ((ctx->fuse & F_VOF) && ensure_vof_spare(ctx, interface)) return 1;
+  if (fluid_decide_dt(domain, fluid, dt)) return 1;
+  return integrate_fused_stages(domain, fluid, interface, *dt);
+}
+
+// the three RK stages of one step with dt known
+static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
+                                  const double dtv) {
   vof_ctx * ctx = ctx_of(domain);
   const int fz = ctx->fuse;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
-  if ((fz & F_VOF) && !ctx->d_vof_spare) {
-    ctx->vof_bytes = interface->vof.datasize;
-    HIPCHK(hipMalloc(&ctx->d_vof_spare, ctx->vof_bytes));
-    HIPCHK(hipMemsetAsync(ctx->d_vof_spare, 0, ctx->vof_bytes, ctx->stream));
-  }
-  if (fluid_decide_dt(domain, fluid, dt)) return 1;
-  const double dtv = *dt;
 #if NDIMS == 3
   double * uz = (double *)fluid->uz.data;
 #else
   double * uz = nullptr;
 #endif
   for (size_t rk = 0; rk < 3; rk++) {
+    if (ctx->tape) {
+      // the adjoint's recomputation records each stage's input state
+      double ** t = ctx->tape + 3 * rk;
+      HIPCHK(hipMemcpyAsync(t[0], interface->vof.data, interface->vof.datasize, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+      HIPCHK(hipMemcpyAsync(t[1], fluid->ux.data, fluid->ux.datasize, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(t[2], fluid->uy.data, fluid->uy.datasize, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     // curvature tensor (curvature_tensor.c:178-186)
     if (fz & F_IFACE) {
       const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
                     (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
-      KLN(ctx, "k_iface_fused", vofk::k_iface_fused, gi, dim3(vofk::kIThreads), ctx->grid,
-          (const double *)interface->vof.data, (double *)interface->normal.data,
-          (double *)interface->curv.data);
+      KLB(ctx, "k_iface_fused", vofk::kIThreads, vofk::k_iface_fused_body, vofk::k_iface_fused, gi,
+          dim3(vofk::kIThreads), 0, ctx->grid, (const double *)interface->vof.data,
+          (double *)interface->normal.data, (double *)interface->curv.data);
     } else {
       if (interface_compute_curvature_tensor(domain, interface)) return 1;
     }
@@ -1542,8 +1757,8 @@ static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t
       if (rk != 0) swap_data(&interface->src[0], &interface->src[1]);
       const double dta = dtv * kRk[rk][0];
       const double dtb = dtv * kRk[rk][1];
-      KLN(ctx, "k_vof_fused", vofk::k_vof_fused, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid,
-          (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uz,
+      KLB(ctx, "k_vof_fused", 256, vofk::k_vof_fused_body, vofk::k_vof_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
+          ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uz,
           (const double *)interface->vof.data, (const double *)interface->normal.data,
           (double *)interface->src[0].data, (const double *)interface->src[1].data,
           ctx->d_vof_spare, dta, dtb, (int)(rk != 0));
@@ -1564,9 +1779,9 @@ static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t
     if (fz & F_CORR) {
       const double pref_c = kRk[rk][2] * dtv;
       const double pref_p = 0.5 * kRk[rk][2] * dtv * fluid->m_dif;
-      KLN(ctx, "k_corr_press", vofk::k_corr_press, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid, pref_c,
-          pref_p, (const double *)fluid->psi.data, (double *)fluid->ux.data, (double *)fluid->uy.data,
-          uz, (double *)fluid->p.data);
+      KLB(ctx, "k_corr_press", 256, vofk::k_corr_press_body, vofk::k_corr_press, grid3(nx, ny, nz, kBlk), kBlk, 0,
+          ctx->grid, pref_c, pref_p, (const double *)fluid->psi.data, (double *)fluid->ux.data,
+          (double *)fluid->uy.data, uz, (double *)fluid->p.data);
     } else {
       if (fluid_correct_velocity(domain, rk, dtv, fluid)) return 1;
       if (fluid_update_pressure(domain, rk, dtv, fluid)) return 1;
@@ -1607,6 +1822,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     ctx->fft_elems = value;
     return 0;
   }
+  if (!strcmp(name, "adj_floor_exp")) {
+    if (value < 0 || value > 16) return fail("adj_floor_exp must be in 0..16");
+    ctx->adj_floor = pow(10., -(double)value);
+    return 0;
+  }
   if (!strcmp(name, "lines_lpb")) {
     if (value < 0 || value > 64) return fail("lines_lpb must be in 0..64");
     ctx->lines_lpb = value;
@@ -1637,6 +1857,509 @@ extern "C" int vof_set_poisson_mixed(domain_t * domain, int refinements) {
   ctx->poisson_refine = refinements < 0 ? -1 : refinements;
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// batch of cases advanced together (see "Batched execution" above)
+
+__global__ void k_gather_dt(const double * const * __restrict__ src, int n, double * __restrict__ dst) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c < n) dst[c] = *src[c];
+}
+
+static int batch_stage_reserve(vof_batch * B, size_t bytes) {
+  if (bytes <= B->stage_bytes) return 0;
+  HIPCHK(hipStreamSynchronize(B->stream));
+  host_free(B->h_stage);
+  dev_free(B->d_stage);
+  B->h_stage = nullptr;
+  B->d_stage = nullptr;
+  B->stage_bytes = 0;
+  const size_t nb = bytes + bytes / 2 + 4096;
+  HIPCHK(hipHostMalloc(&B->h_stage, nb));
+  HIPCHK(hipMalloc(&B->d_stage, nb));
+  B->stage_bytes = nb;
+  return 0;
+}
+
+// merge the recorded launches of every case, launch by launch, on B->stream
+static int batch_run(vof_batch * B) {
+  const size_t n = B->ctx.size();
+  const size_t nrec = B->rec[0].recs.size();
+  for (size_t c = 1; c < n; c++) {
+    if (B->rec[c].recs.size() != nrec) return fail("batch: case %zu records %zu launches, case 0 %zu", c,
+                                                   B->rec[c].recs.size(), nrec);
+  }
+  std::vector<size_t> off(nrec + 1, 0);
+  for (size_t r = 0; r < nrec; r++) {
+    const LaunchRec & r0 = B->rec[0].recs[r];
+    for (size_t c = 1; c < n; c++) {
+      const LaunchRec & rc = B->rec[c].recs[r];
+      if (rc.merge != r0.merge || rc.block.x != r0.block.x || rc.block.y != r0.block.y || rc.block.z != r0.block.z)
+        return fail("batch: launch %zu differs between case 0 (%s) and case %zu (%s)", r, r0.name, c, rc.name);
+    }
+    const size_t bytes = r0.merge(B, r, nullptr, nullptr);
+    off[r + 1] = off[r] + ((bytes + 255) & ~(size_t)255);
+  }
+  // the previous copy out of the staging buffer must have completed
+  HIPCHK(hipEventSynchronize(B->stage_done));
+  if (batch_stage_reserve(B, off[nrec])) return 1;
+  for (size_t r = 0; r < nrec; r++) B->rec[0].recs[r].merge(B, r, B->h_stage + off[r], nullptr);
+  HIPCHK(hipMemcpyAsync(B->d_stage, B->h_stage, off[nrec], hipMemcpyHostToDevice, B->stream));
+  HIPCHK(hipEventRecord(B->stage_done, B->stream));
+  for (size_t r = 0; r < nrec; r++) {
+    if (B->rec[0].recs[r].merge(B, r, nullptr, B->d_stage + off[r]) == 0)
+      return fail("batch: launch of %s failed", B->rec[0].recs[r].name);
+  }
+  return 0;
+}
+
+extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
+                                interface_t * const interfaces[], vof_batch_t ** out) {
+  if (ncases < 1 || !domains || !fluids || !interfaces || !out) return fail("vof_batch_create: bad arguments");
+  vof_batch * B = new vof_batch();
+  B->stage_done = nullptr;
+  for (int c = 0; c < ncases; c++) {
+    vof_ctx * ctx = ctx_of(domains[c]);
+    if (!ctx) {
+      delete B;
+      return fail("vof_batch_create: case %d has no initialised domain", c);
+    }
+    if (ctx->schedule != 1 || ctx->fuse != 63) {
+      delete B;
+      return fail("vof_batch_create: case %d is not on the fully fused schedule", c);
+    }
+    for (int d = 0; d < c; d++)
+      if (B->ctx[d] == ctx) {
+        delete B;
+        return fail("vof_batch_create: case %d repeats case %d", c, d);
+      }
+    B->ctx.push_back(ctx);
+    B->domain.push_back(domains[c]);
+    B->fluid.push_back(fluids[c]);
+    B->interface.push_back(interfaces[c]);
+  }
+  B->rec.resize(ncases);
+  if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&B->stage_done, hipEventDisableTiming) != hipSuccess) {
+    delete B;
+    return fail("vof_batch_create: stream / event creation failed");
+  }
+  int rc = 0;
+  // every case's tables in one pinned block and one device block
+  size_t tab = 0;
+  for (auto * ctx : B->ctx) tab += ctx->tab_stride * 12;
+  B->tab_doubles = tab;
+  if (hipHostMalloc(&B->h_tab, sizeof(double) * tab) != hipSuccess || hipMalloc(&B->d_tab, sizeof(double) * tab) != hipSuccess)
+    rc = fail("vof_batch_create: table allocation failed");
+  std::vector<const double *> src(ncases);
+  for (int c = 0; c < ncases && !rc; c++) {
+    vof_ctx * ctx = B->ctx[c];
+    rc = ensure_vof_spare(ctx, B->interface[c]);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail("vof_batch_create: case stream failed");
+    src[c] = ctx->d_part + ctx->npart;
+  }
+  if (!rc) {
+    size_t o = 0;
+    for (auto * ctx : B->ctx) {
+      host_free(ctx->h_tab);
+      dev_free(ctx->d_tab);
+      ctx->h_tab = B->h_tab + o;
+      ctx->d_tab = B->d_tab + o;
+      ctx->tab_shared = true;
+      o += ctx->tab_stride * 12;
+    }
+    if (hipMalloc(&B->d_dt, sizeof(double) * ncases) != hipSuccess ||
+        hipHostMalloc(&B->h_dt, sizeof(double) * ncases) != hipSuccess ||
+        hipMalloc(&B->d_dt_src, sizeof(double *) * ncases) != hipSuccess ||
+        hipMemcpy(B->d_dt_src, src.data(), sizeof(double *) * ncases, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail("vof_batch_create: dt buffers failed");
+  }
+  if (rc) {
+    vof_batch_destroy(B);
+    return 1;
+  }
+  *out = B;
+  return 0;
+}
+
+extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
+  if (!B) return fail("vof_batch_integrate: batch is NULL");
+  const size_t n = B->ctx.size();
+  int rc = 0;
+  // dt of every case (decide_dt.c:104-145): batched reductions, one copy back
+  for (size_t c = 0; c < n && !rc; c++) {
+    B->rec[c].recs.clear();
+    B->ctx[c]->rec = &B->rec[c];
+    rc = decide_dt_enqueue(B->ctx[c], B->fluid[c]);
+    B->ctx[c]->rec = nullptr;
+  }
+  if (rc || batch_run(B)) return 1;
+  hipLaunchKernelGGL(k_gather_dt, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, B->stream,
+                     (const double * const *)B->d_dt_src, (int)n, B->d_dt);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(B->h_dt, B->d_dt, sizeof(double) * n, hipMemcpyDeviceToHost, B->stream));
+  HIPCHK(hipStreamSynchronize(B->stream));
+  std::vector<double> dt(n);
+  for (size_t c = 0; c < n; c++) {
+    B->ctx[c]->h_scalar[0] = B->h_dt[c];
+    if (decide_dt_finish(B->ctx[c], B->domain[c], B->fluid[c], &dt[c], false)) return 1;
+  }
+  // the three RK stages of every case, recorded then merged
+  for (size_t c = 0; c < n && !rc; c++) {
+    B->rec[c].recs.clear();
+    B->ctx[c]->rec = &B->rec[c];
+    rc = integrate_fused_stages(B->domain[c], B->fluid[c], B->interface[c], dt[c]);
+    B->ctx[c]->rec = nullptr;
+  }
+  if (rc) return 1;
+  HIPCHK(hipMemcpyAsync(B->d_tab, B->h_tab, sizeof(double) * B->tab_doubles, hipMemcpyHostToDevice, B->stream));
+  if (batch_run(B)) return 1;
+  if (dts)
+    for (size_t c = 0; c < n; c++) dts[c] = dt[c];
+  return 0;
+}
+
+extern "C" int vof_batch_synchronize(vof_batch_t * B) {
+  if (!B) return fail("vof_batch_synchronize: batch is NULL");
+  HIPCHK(hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+extern "C" int vof_batch_destroy(vof_batch_t * B) {
+  if (!B) return 0;
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  // cases keep working on their own after the batch: give them private tables
+  for (auto * ctx : B->ctx) {
+    if (!ctx->tab_shared) continue;
+    double * h = nullptr;
+    double * d = nullptr;
+    if (hipHostMalloc(&h, sizeof(double) * ctx->tab_stride * 12) == hipSuccess &&
+        hipMalloc(&d, sizeof(double) * ctx->tab_stride * 12) == hipSuccess) {
+      ctx->h_tab = h;
+      ctx->d_tab = d;
+    } else {
+      ctx->h_tab = nullptr;
+      ctx->d_tab = nullptr;
+    }
+    ctx->tab_shared = false;
+  }
+  host_free(B->h_stage);
+  dev_free(B->d_stage);
+  host_free(B->h_tab);
+  dev_free(B->d_tab);
+  dev_free(B->d_dt);
+  host_free(B->h_dt);
+  dev_free((void *)B->d_dt_src);
+  if (B->stage_done) (void)hipEventDestroy(B->stage_done);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return 0;
+}
+
+// one step with a given dt (the frozen dt sequence of the adjoint and its
+// finite-difference checks); fused schedule
+extern "C" int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, interface_t * interface,
+                                      double dt) {
+  CTX_OR_FAIL(domain);
+  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_integrate_fixed_dt: fused schedule only");
+  if (ensure_vof_spare(ctx, interface)) return 1;
+  return integrate_fused_stages(domain, fluid, interface, dt);
+}
+
+#if NDIMS == 2
+// ---------------------------------------------------------------------------
+// Adjoint of the 2D step (hand-written reverse mode, vof_adjoint.h).
+//
+// Loss after nsteps: the droplet centroid x = sum xc vof dV / sum vof dV
+// plus the interface length sum |grad vof| dV (central differences, x walls
+// at vof 0, periodic y).  The forward runs with the given dt sequence and
+// keeps the state (ux, uy, p, vof) every `every` steps; each segment is then
+// recomputed with its stage states recorded and reversed stage by stage.
+// The Poisson and x-line solves are transposed as D S D^-1 around the forward
+// solvers S (the operators are D^-1 K with K symmetric, D the cell widths).
+
+static double loss_of(const vof_ctx * ctx, const domain_t * domain, const std::vector<double> & vof,
+                      std::vector<double> * grad) {
+  const int nx = ctx->nx, ny = ctx->ny;
+  const double dy = ctx->dy;
+  const double * xc = domain->xc;
+  const double * dxf = domain->dxf;
+  const double * dxc = domain->dxc;
+  auto V = [&](int i, int j) -> double {
+    if (i < 1 || i > nx) return 0.;
+    j = j < 1 ? j + ny : (j > ny ? j - ny : j);
+    return vof[I_VOF(i, j, 1)];
+  };
+  double s0 = 0., s1 = 0., len = 0.;
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      const double dv = dxf[i - 1] * dy;
+      s0 += xc[i] * V(i, j) * dv;
+      s1 += V(i, j) * dv;
+    }
+  const double X = s0 / s1;
+  if (grad) grad->assign(vof.size(), 0.);
+  auto G = [&](int i, int j, double v) {
+    if (i < 1 || i > nx) return;
+    j = j < 1 ? j + ny : (j > ny ? j - ny : j);
+    (*grad)[I_VOF(i, j, 1)] += v;
+  };
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      const double dv = dxf[i - 1] * dy;
+      const double hx = dxc[i - 1] + dxc[i];
+      const double gx = (V(i + 1, j) - V(i - 1, j)) / hx;
+      const double gy = (V(i, j + 1) - V(i, j - 1)) / (2. * dy);
+      const double nrm = sqrt(gx * gx + gy * gy);
+      len += nrm * dv;
+      if (grad) {
+        G(i, j, dv * (xc[i] - X) / s1);
+        if (nrm > 0.) {
+          G(i + 1, j, dv * gx / nrm / hx);
+          G(i - 1, j, -dv * gx / nrm / hx);
+          G(i, j + 1, dv * gy / nrm / (2. * dy));
+          G(i, j - 1, -dv * gy / nrm / (2. * dy));
+        }
+      }
+    }
+  return X + len;
+}
+
+extern "C" int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, double * loss) {
+  CTX_OR_FAIL(domain);
+  std::vector<double> h(interface->vof.datasize / sizeof(double));
+  HIPCHK(hipMemcpyAsync(h.data(), interface->vof.data, interface->vof.datasize, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *loss = loss_of(ctx, domain, h, nullptr);
+  return 0;
+}
+
+struct AdjBuf {
+  std::vector<void *> all;
+  int get(double ** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return fail("adjoint: allocation of %zu bytes failed", bytes);
+    all.push_back(*p);
+    return 0;
+  }
+  ~AdjBuf() {
+    for (void * p : all) dev_free(p);
+  }
+};
+
+extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, interface_t * interface, int nsteps,
+                                    const double * dts, int every, double * loss, double * grad_ux,
+                                    double * grad_uy) {
+  CTX_OR_FAIL(domain);
+  if (nsteps < 1 || !dts || every < 1) return fail("vof_adjoint_gradient: bad arguments");
+  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_adjoint_gradient: fused schedule only");
+  {
+    double buoy = 0.;
+    if (param_get(ctx, P_BUOY, &buoy)) return 1;
+    if (buoy != 0.) return fail("vof_adjoint_gradient: buoyancy (temperature coupling) is not differentiated");
+  }
+  if (ctx->poisson_refine >= 0) return fail("vof_adjoint_gradient: plain FP64 Poisson only");
+  if (ensure_vof_spare(ctx, interface)) return 1;
+  const int nx = ctx->nx, ny = ctx->ny;
+  const size_t bvof = interface->vof.datasize, bux = fluid->ux.datasize, buy = fluid->uy.datasize;
+  const size_t bp = fluid->p.datasize;
+  const size_t ncell = (size_t)nx * ny;
+  hipStream_t st = ctx->stream;
+  AdjBuf B;
+  // checkpoints
+  const int nck = (nsteps + every - 1) / every;
+  std::vector<double *> ck(4 * (size_t)nck);
+  for (int c = 0; c < nck; c++) {
+    if (B.get(&ck[4 * c + 0], bvof) || B.get(&ck[4 * c + 1], bux) || B.get(&ck[4 * c + 2], buy) ||
+        B.get(&ck[4 * c + 3], bp))
+      return 1;
+  }
+  auto save = [&](int c) -> int {
+    HIPCHK(hipMemcpyAsync(ck[4 * c + 0], interface->vof.data, bvof, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ck[4 * c + 1], fluid->ux.data, bux, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ck[4 * c + 2], fluid->uy.data, buy, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ck[4 * c + 3], fluid->p.data, bp, hipMemcpyDeviceToDevice, st));
+    return 0;
+  };
+  auto restore = [&](int c) -> int {
+    HIPCHK(hipMemcpyAsync(interface->vof.data, ck[4 * c + 0], bvof, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(fluid->ux.data, ck[4 * c + 1], bux, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(fluid->uy.data, ck[4 * c + 2], buy, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(fluid->p.data, ck[4 * c + 3], bp, hipMemcpyDeviceToDevice, st));
+    return 0;
+  };
+  // forward with checkpoints
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0 && save(n / every)) return 1;
+    if (integrate_fused_stages(domain, fluid, interface, dts[n])) return 1;
+  }
+  // loss and its vof gradient on the host
+  std::vector<double> hv(bvof / sizeof(double)), hg;
+  HIPCHK(hipMemcpyAsync(hv.data(), interface->vof.data, bvof, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double L = loss_of(ctx, domain, hv, &hg);
+  // adjoint state and scratch
+  double *lux, *luy, *lp, *lvof, *lpsi, *lcurv, *lm, *lnrm, *lfx, *lfy, *ls, *csx, *csy, *cs, *qx, *qy, *vx, *vy,
+      *b, *x;
+  const size_t bcurv = interface->curv.datasize, bdvof = interface->dvof.datasize;
+  const size_t bnrm = interface->normal.datasize;
+  if (B.get(&lux, bux) || B.get(&luy, buy) || B.get(&lp, bp) || B.get(&lvof, bvof) || B.get(&lpsi, bp) ||
+      B.get(&lcurv, bcurv) || B.get(&lm, bdvof) || B.get(&lnrm, bnrm) ||
+      B.get(&lfx, sizeof(double) * (size_t)(nx - 1) * ny) || B.get(&lfy, sizeof(double) * ncell) ||
+      B.get(&ls, sizeof(double) * ncell) || B.get(&csx, sizeof(double) * (size_t)(nx - 1) * ny) ||
+      B.get(&csy, sizeof(double) * ncell) || B.get(&cs, sizeof(double) * ncell) || B.get(&qx, bux) ||
+      B.get(&qy, buy) || B.get(&vx, sizeof(double) * (size_t)(nx - 1) * ny) || B.get(&vy, sizeof(double) * ncell) ||
+      B.get(&b, sizeof(double) * ncell) || B.get(&x, bp))
+    return 1;
+  HIPCHK(hipMemsetAsync(lux, 0, bux, st));
+  HIPCHK(hipMemsetAsync(luy, 0, buy, st));
+  HIPCHK(hipMemsetAsync(lp, 0, bp, st));
+  HIPCHK(hipMemsetAsync(csx, 0, sizeof(double) * (size_t)(nx - 1) * ny, st));
+  HIPCHK(hipMemsetAsync(csy, 0, sizeof(double) * ncell, st));
+  HIPCHK(hipMemsetAsync(cs, 0, sizeof(double) * ncell, st));
+  HIPCHK(hipMemcpyAsync(lvof, hg.data(), bvof, hipMemcpyHostToDevice, st));
+  // tapes of one segment: (vof, ux, uy) per stage
+  std::vector<double *> tape(9 * (size_t)every);
+  for (size_t t = 0; t < tape.size(); t += 3) {
+    if (B.get(&tape[t + 0], bvof) || B.get(&tape[t + 1], bux) || B.get(&tape[t + 2], buy)) return 1;
+  }
+  const dim3 blk(64, 4);
+  const dim3 gc((unsigned)((nx + 2 + 63) / 64), (unsigned)((ny + 3) / 4));
+  const Grid g = ctx->grid;
+  const double m_dif = fluid->m_dif;
+  const int NC = 1024 / ny > 0 ? 1024 / ny : 1;
+  const int nbx = (nx + 2 * NC - 1) / (2 * NC);
+  const int use1024 = ny <= 1024;
+  for (int c = nck - 1; c >= 0; c--) {
+    const int s0 = c * every, s1 = s0 + every < nsteps ? s0 + every : nsteps;
+    if (restore(c)) return 1;
+    for (int n = s0; n < s1; n++) {
+      ctx->tape = &tape[9 * (size_t)(n - s0)];
+      const int rc = integrate_fused_stages(domain, fluid, interface, dts[n]);
+      ctx->tape = nullptr;
+      if (rc) return 1;
+    }
+    for (int n = s1 - 1; n >= s0; n--) {
+      const double dt = dts[n];
+      for (int rk = 2; rk >= 0; rk--) {
+        double ** t = &tape[9 * (size_t)(n - s0) + 3 * rk];
+        const double * vof = t[0];
+        const double * ux = t[1];
+        const double * uy = t[2];
+        const double ca = kRk[rk][0] * dt, cb = kRk[rk][1] * dt, cg = kRk[rk][2] * dt;
+        const double pc = kRk[rk][2] * dt;
+        const double pp = 0.5 * kRk[rk][2] * dt * m_dif;
+        const double pref = 1. / (kRk[rk][2] * dt);
+        // forward quantities of the stage: corner normals, curvature, normals
+        hipLaunchKernelGGL(vofk::k_gradient, grid3(nx + 1, ny + 3, 1, kBlk), kBlk, 0, st, g, vof,
+                           (double *)interface->dvof.data);
+        hipLaunchKernelGGL(vofk::k_normal_curv, grid3(nx, ny + 2, 1, kBlk), kBlk, 0, st, g, vof,
+                           (const double *)interface->dvof.data, (double *)interface->normal.data,
+                           (double *)interface->curv.data);
+        LAUNCHCHK();
+        // R1 correction / pressure
+        HIPCHK(hipMemsetAsync(lpsi, 0, bp, st));
+        hipLaunchKernelGGL(vofa::ka_corr, gc, blk, 0, st, g, pc, pp, (const double *)lux, (const double *)luy,
+                           (const double *)lp, lpsi);
+        // R2 Poisson: x = G D^-1 lambda_psi, lambda_u* += pref div^T D x
+        hipLaunchKernelGGL(vofa::ka_psi_rhs, gc, blk, 0, st, g, (const double *)lpsi, b);
+        if (use1024) {
+          hipLaunchKernelGGL(vofa::ka_vec_rfft_y<1024>, dim3(nbx), dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y,
+                             (const double2 *)ctx->d_twy, (const double *)b, 1. / (double)ny, ctx->d_spec);
+        } else {
+          hipLaunchKernelGGL(vofa::ka_vec_rfft_y<2048>, dim3((nx + 2 * (2048 / ny) - 1) / (2 * (2048 / ny))),
+                             dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y, (const double2 *)ctx->d_twy,
+                             (const double *)b, 1. / (double)ny, ctx->d_spec);
+        }
+        hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
+                           ctx->nh, (const double *)ctx->d_pl, (const double *)ctx->d_pu,
+                           (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+        if (use1024) {
+          hipLaunchKernelGGL(vofk::k_irfft_y_psi<1024>, dim3(nbx), dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y,
+                             (const double2 *)ctx->d_twy, (const double2 *)ctx->d_spec, x, 1);
+        } else {
+          hipLaunchKernelGGL(vofk::k_irfft_y_psi<2048>, dim3((nx + 2 * (2048 / ny) - 1) / (2 * (2048 / ny))),
+                             dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y, (const double2 *)ctx->d_twy,
+                             (const double2 *)ctx->d_spec, x, 1);
+        }
+        hipLaunchKernelGGL(vofa::ka_div_t, gc, blk, 0, st, g, pref, (const double *)x, lux, luy);
+        LAUNCHCHK();
+        // R3 x-line solves: q = M^-1 D^-1 lambda_u*
+        hipLaunchKernelGGL(vofa::ka_lines_in, gc, blk, 0, st, g, (const double *)lux, (const double *)luy, vx, vy);
+        HIPCHK(hipMemsetAsync(qx, 0, bux, st));
+        HIPCHK(hipMemsetAsync(qy, 0, buy, st));
+        {
+          vofk::LineJob lj{};
+          lj.nlines = ny;
+          lj.ny = ny;
+          lj.n1 = 0;
+          lj.images = 0;
+          lj.nz = 1;
+          const double pm = 0.5 * kRk[rk][2] * dt * m_dif;
+          const double * tx = nullptr;
+          const double * ty = nullptr;
+          if (upload_tab(ctx, 0, rk, pm, &tx) || upload_tab(ctx, 1, rk, pm, &ty)) return 1;
+          lj.f[0] = vofk::LineField{tx, nullptr, nullptr, nullptr, vx, qx, nx - 1, nx + 1, 1};
+          lj.f[1] = vofk::LineField{ty, nullptr, nullptr, nullptr, vy, qy, nx, nx + 2, 1};
+          int lpb = (int)(32768 / (sizeof(double) * (nx | 1)));
+          lpb = lpb > 64 ? 64 : (lpb < 1 ? 1 : lpb);
+          const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
+          hipLaunchKernelGGL(vofk::k_lines_lds, dim3((unsigned)((ny + lpb - 1) / lpb), 2), dim3(vofk::kWThreads), shm,
+                             st, lj, lpb);
+          LAUNCHCHK();
+        }
+        // R7 vof source (before anything else adds to lambda_vof)
+        hipLaunchKernelGGL(vofa::ka_vof_src, gc, blk, 0, st, g, kRk[rk][0] * dt, kRk[rk][1] * dt,
+                           (const double *)lvof, ls, cs);
+        // R4 + R5 predictor sources
+        {
+          vofa::AdjRhs a{};
+          a.ux = ux;
+          a.uy = uy;
+          a.lapx_ux = ctx->d_lapx_ux;
+          a.lapx_c = ctx->d_lapx_c;
+          a.qx = qx;
+          a.qy = qy;
+          a.lux = lux;
+          a.luy = luy;
+          a.lp = lp;
+          a.csx = csx;
+          a.csy = csy;
+          a.lfx = lfx;
+          a.lfy = lfy;
+          a.ca = ca;
+          a.cb = cb;
+          a.cg = cg;
+          a.dif = m_dif;
+          hipLaunchKernelGGL(vofa::ka_rhs, gc, blk, 0, st, g, a);
+        }
+        // R8 fluxes, R6 force, R9-R11 interface geometry
+        HIPCHK(hipMemsetAsync(lnrm, 0, bnrm, st));
+        HIPCHK(hipMemsetAsync(lcurv, 0, bcurv, st));
+        HIPCHK(hipMemsetAsync(lm, 0, bdvof, st));
+        hipLaunchKernelGGL(vofa::ka_flux, gc, blk, 0, st, g, ux, uy, vof, (const double *)interface->normal.data,
+                           (const double *)ls, lux, luy, lvof, lnrm);
+        hipLaunchKernelGGL(vofa::ka_force, gc, blk, 0, st, g, interface->tension, vof,
+                           (const double *)interface->curv.data, (const double *)lfx, (const double *)lfy, lvof,
+                           lcurv);
+        hipLaunchKernelGGL(vofa::ka_normal_curv, gc, blk, 0, st, g, vof, (const double *)interface->dvof.data,
+                           (const double *)interface->normal.data, (const double *)lnrm, (const double *)lcurv,
+                           lvof, lm);
+        hipLaunchKernelGGL(vofa::ka_gradient, dim3((unsigned)((nx + 1 + 63) / 64), (unsigned)((ny + 3) / 4)), blk,
+                           0, st, g, ctx->adj_floor, vof, (const double *)lm, lvof);
+        LAUNCHCHK();
+      }
+    }
+  }
+  // the solver is left at its initial state; the gradient is lambda_u
+  if (restore(0)) return 1;
+  HIPCHK(hipMemcpyAsync(grad_ux, lux, bux, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(grad_uy, luy, buy, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *loss = L;
+  return 0;
+}
+#endif
 
 // stages of the fused schedule (bit mask; the default is all of them)
 extern "C" int vof_set_fusion(domain_t * domain, int mask) {

@@ -404,3 +404,52 @@ def random_fields(glsizes, lengths=None, seed=0, uniformx=True, amp=0.1,
     params = dict(Ra=1.0e4, Pr=1.0, We=10.0, coef_dt_adv=0.4, coef_dt_dif=0.95,
                   add_buoyancy=1, t_xm=0.5, t_xp=-0.5)
     return InitialCondition(tuple(lengths), tuple(glsizes), xf, xc, fields, params)
+
+
+def batch_2d_cases(ncases=96, seed=4, sizes=(64, 96, 128, 192, 256), urms=0.05):
+    """Batched workload: ``ncases`` independent 2D cases with seeded uneven
+    grids (nx from ``sizes``, ny = 2 nx, lx = 1, ly = 2, uniform x), each with
+    1-4 non-overlapping droplets of radius 0.05-0.2 kept 2 cells off the x
+    walls, a divergence-free velocity of rms ``urms``, Re in [50, 500] and We
+    in [5, 50] (Ra = Re^2, Pr = 1, buoyancy off)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(ncases):
+        nx = int(rng.choice(sizes))
+        ny = 2 * nx
+        lx, ly = 1.0, 2.0
+        xf, xc = make_xgrid(lx, nx, True)
+        ndrops = int(rng.integers(1, 5))
+        discs = []
+        tries = 0
+        while len(discs) < ndrops and tries < 10000:
+            tries += 1
+            r = float(rng.uniform(0.05, 0.2))
+            margin = r + 2.0 * lx / nx
+            cx = float(rng.uniform(margin, lx - margin))
+            cy = float(rng.uniform(0.0, ly))
+            ok = True
+            for (ox, oy, orad) in discs:
+                dy = abs(cy - oy)
+                dy = min(dy, ly - dy)
+                if (cx - ox) ** 2 + dy ** 2 < (r + orad + 2.0 * lx / nx) ** 2:
+                    ok = False
+                    break
+            if ok:
+                discs.append((cx, cy, r))
+        vof = _pad_cells(vof_discs(xf, ny, ly, discs))
+        modes = _random_modes(rng, 8, lx, ly, None)
+        fn = _modal_fn(modes, 0, lx, ly, None)
+        ux, uy = velocity_from_streamfunction_2d(xf, ny, ly, lambda X, Y: fn(X, Y, 0.0 * X))
+        ms = 0.5 * (np.mean(ux[:, 1:-1] ** 2) + np.mean(uy[:, 1:-1] ** 2))
+        sc = urms / np.sqrt(ms) if ms > 0 else 0.0
+        ux, uy = ux * sc, uy * sc
+        Re = float(rng.uniform(50.0, 500.0))
+        We = float(rng.uniform(5.0, 50.0))
+        shp = (ny, nx + 2)
+        params = dict(Ra=Re * Re, Pr=1.0, We=We, coef_dt_adv=0.4, coef_dt_dif=0.95,
+                      add_buoyancy=0, t_xm=0.0, t_xp=0.0)
+        out.append(InitialCondition((lx, ly), (nx, ny), xf, xc,
+                                    dict(ux=ux, uy=uy, p=np.zeros(shp), t=np.zeros(shp), vof=vof),
+                                    params))
+    return out

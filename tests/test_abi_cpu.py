@@ -24,6 +24,9 @@ def _declared_symbols(ndims):
         if line.startswith("#if NDIMS == 3"):
             skip = ndims != 3
             continue
+        if line.startswith("#if NDIMS == 2"):
+            skip = ndims != 2
+            continue
         if line.startswith("#endif") and skip:
             skip = False
             continue
@@ -42,7 +45,7 @@ def test_library_exports_every_declared_symbol(ndims):
     assert "integrate" in declared and "tdm" in declared and len(declared) > 35
     for name in sorted(declared):
         assert hasattr(lib, name), name
-    listed = set(sva.ABI_SYMBOLS_COMMON) | (set(sva.ABI_SYMBOLS_3D) if ndims == 3 else set())
+    listed = set(sva.ABI_SYMBOLS_COMMON) | (set(sva.ABI_SYMBOLS_3D) if ndims == 3 else set(sva.ABI_SYMBOLS_2D))
     assert declared == listed
 
 
