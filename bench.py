@@ -172,6 +172,87 @@ def batched_form(sva, ncases, steps):
                         "Re in [50,500], We in [5,50], FP64"}
 
 
+def adjoint_form(sva, nsteps=50, every=10, ndirs=16, n=256):
+    """Differentiation workload: 2D n x n, one droplet r = 0.2 in a
+    divergence-free velocity (rms 0.05); loss = centroid x + interface
+    length after nsteps; gradient w.r.t. the initial velocity by the adjoint
+    with checkpoints every `every` steps, timed against the forward, and
+    checked against central differences along `ndirs` divergence-free
+    directions with a frozen dt sequence, with surface tension (We = 10) and
+    without (We -> infinity)."""
+    import numpy as np
+    from simplevofsolver_amd.ic import divfree_direction, droplet_2d_divfree
+    out = {"directions": ndirs, "steps": nsteps, "checkpoint_every": every,
+           "workload": f"2D {n}x{n}, droplet r=0.2, divergence-free velocity rms 0.05, Re=100, loss = centroid x "
+                       f"+ interface length after {nsteps} steps, FP64"}
+    for tag, We in (("We10", 10.0), ("no_tension", 1e12)):
+        ic = droplet_2d_divfree(nx=n, ny=n, radius=0.2, urms=0.05, nmodes=8, seed=3, We=We)
+        with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+            ref = sva.solver_from_ic(ic, os.path.join(d, "ref"))
+            s = sva.solver_from_ic(ic, os.path.join(d, "adj"))
+        try:
+            dts = [ref.integrate() for _ in range(nsteps)]
+        finally:
+            ref.close()
+        try:
+            state = tuple(s.get(k) for k in ("ux", "uy", "p", "vof"))
+
+            def set_state(vx=None, vy=None, eps=0.0):
+                s.set("ux", state[0] + eps * vx if vx is not None else state[0])
+                s.set("uy", state[1] + eps * vy if vy is not None else state[1])
+                s.boundaries("ux")
+                s.boundaries("uy")
+                s.set("p", state[2])
+                s.set("vof", state[3])
+
+            def loss_at(vx, vy, eps):
+                set_state(vx, vy, eps)
+                for dt in dts:
+                    s.integrate_fixed_dt(dt)
+                return s.adjoint_loss()
+
+            set_state()
+            s.adjoint_gradient(dts, every=every)          # warm-up
+            set_state()
+            s.synchronize()
+            t0 = time.perf_counter()
+            for dt in dts:
+                s.integrate_fixed_dt(dt)
+            s.synchronize()
+            t_fwd = time.perf_counter() - t0
+            lf = s.adjoint_loss()
+            set_state()
+            s.synchronize()
+            t0 = time.perf_counter()
+            loss, gux, guy = s.adjoint_gradient(dts, every=every)
+            t_adj = time.perf_counter() - t0
+            rng = np.random.default_rng(32)
+            errs = {1e-6: [], 1e-7: [], 3e-8: [], 1e-8: []}
+            for _ in range(ndirs):
+                vx, vy = divfree_direction(ic, rng)
+                ad = float(gux @ vx + guy @ vy)
+                for eps in errs:
+                    fd = (loss_at(vx, vy, eps) - loss_at(vx, vy, -eps)) / (2 * eps)
+                    errs[eps].append(abs(ad - fd) / max(abs(fd), 1e-300))
+        finally:
+            s.close()
+        best = np.min(np.array([errs[e] for e in errs]), axis=0)
+        rec = {"adjoint_over_forward": round(t_adj / t_fwd, 3), "forward_ms": round(t_fwd * 1e3, 3),
+               "adjoint_ms": round(t_adj * 1e3, 3), "loss": loss, "loss_forward": lf}
+        for e, v in errs.items():
+            v = np.array(v)
+            rec[f"fd_rel_err_eps{e:g}"] = {"median": float(np.median(v)), "max": float(v.max()),
+                                           "within_1e-6": int((v <= 1e-6).sum())}
+        rec["fd_best_eps_within_1e-6"] = int((best <= 1e-6).sum())
+        out[tag] = rec
+        print(f"[bench] adjoint ({tag}) {t_adj * 1e3:.1f} ms / forward {t_fwd * 1e3:.1f} ms = {t_adj / t_fwd:.2f}; "
+              f"FD rel err median " + " / ".join(f"{np.median(errs[e]):.1e}" for e in errs) +
+              f" (eps 1e-6 / 1e-7 / 3e-8 / 1e-8); best-eps within 1e-6: {rec['fd_best_eps_within_1e-6']}/{ndirs}",
+              file=sys.stderr, flush=True)
+    out["adjoint_over_forward"] = out["We10"]["adjoint_over_forward"]
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +342,10 @@ def main(argv=None) -> int:
             forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps)
         except Exception as e:
             forms["batched"] = {"error": repr(e)}
+        try:
+            forms["adjoint"] = adjoint_form(sva)
+        except Exception as e:
+            forms["adjoint"] = {"error": repr(e)}
     out = {
         "metric": METRIC,
         "value": round(value, 3),

@@ -453,3 +453,19 @@ def batch_2d_cases(ncases=96, seed=4, sizes=(64, 96, 128, 192, 256), urms=0.05):
                                     dict(ux=ux, uy=uy, p=np.zeros(shp), t=np.zeros(shp), vof=vof),
                                     params))
     return out
+
+
+def divfree_direction(ic_obj, rng, nmodes=8):
+    """A random discretely divergence-free 2D velocity field (stream function
+    of ``nmodes`` random modes, zero normal velocity at the walls) in the
+    solver's padded ux / uy layouts, flattened: a perturbation direction for
+    derivative checks that keeps the flow incompressible."""
+    nx, ny = ic_obj.glsizes
+    lx, ly = ic_obj.lengths
+    fn = _modal_fn(_random_modes(rng, nmodes, lx, ly, None), 0, lx, ly, None)
+    ux, uy = velocity_from_streamfunction_2d(ic_obj.xf, ny, ly, lambda X, Y: fn(X, Y, 0.0 * X))
+    px = np.zeros((ny + 2, nx + 1))
+    py = np.zeros((ny + 2, nx + 2))
+    px[1:-1] = ux
+    py[1:-1] = uy
+    return px.ravel(), py.ravel()
