@@ -47,7 +47,17 @@ struct Grid {
   double dy, dz;
   const double * __restrict__ dxf;  // [nx]   DXF(i) = dxf[i-1]
   const double * __restrict__ dxc;  // [nx+1] DXC(i) = dxc[i-1]
+  int bands;                        // 3D tile kernels: one band of tile rows per XCD
+  // quotients the stencils multiply by, computed on the host with the same
+  // IEEE divisions the expressions spell (so the products are unchanged):
+  // hdy = 0.5/dy, rdy = 1./dy, lyy = 1./dy/dy, cyy = -2./dy/dy (and z)
+  double hdy, rdy, lyy, cyy, hdz, rdz, lzz, czz;
+  // per x index i (0..nx+1), kXT entries: 0.5/DXC(i), 1./DXC(i),
+  // 0.5*DXF(i-1)/DXC(i), 0.5*DXF(i)/DXC(i), 0.5/DXF(i), 1./DXF(i)
+  const double * __restrict__ xt;
 };
+constexpr int kXT = 8;
+enum { XT_HXC = 0, XT_RXC, XT_WXM, XT_WXP, XT_HXF, XT_RXF };
 
 // Work-groups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Tile kernels renumber their work-groups so that every XCD processes
@@ -102,6 +112,25 @@ __device__ inline Tile3 plain_tile(const VBlock & vb) {
   return t;
 }
 
+// 3D tile kernels: XCD x (work-group w goes to XCD w % 8) takes the band
+// of tile rows y in [x gy/8, (x+1) gy/8) over all planes, walking x fastest,
+// then the rows of its band, then z.  Neighbour tiles in x, y and z then
+// share one L2 (except at band edges), and the planes in flight on an XCD
+// (its work-groups span a few planes of its band) fit its L2.  A bijection of
+// the work-groups when gy is a multiple of 8; otherwise the plain order.
+__device__ inline Tile3 band_tile(const VBlock & vb) {
+  if (vb.gy % kXcds != 0 || vb.gz == 1) return plain_tile(vb);
+  const unsigned w = vb.x + vb.gx * (vb.y + vb.gy * vb.z);
+  const unsigned x = w % kXcds, s = w / kXcds;
+  const unsigned rows = vb.gy / kXcds;
+  Tile3 t;
+  t.x = s % vb.gx;
+  const unsigned r = s / vb.gx;
+  t.y = x * rows + r % rows;
+  t.z = r / rows;
+  return t;
+}
+
 #if NDIMS == 2
 #define KIDX 1
 #define THREAD_IJK(ilo, jlo, klo)                               \
@@ -110,9 +139,10 @@ __device__ inline Tile3 plain_tile(const VBlock & vb) {
   const int k = 1; (void)k;
 #else
 #define THREAD_IJK(ilo, jlo, klo)                               \
-  const int i = (int)(vb.x * blockDim.x + threadIdx.x) + (ilo); \
-  const int j = (int)(vb.y * blockDim.y + threadIdx.y) + (jlo); \
-  const int k = (int)vb.z + (klo);
+  const Tile3 tile_ = g.bands ? band_tile(vb) : plain_tile(vb); \
+  const int i = (int)(tile_.x * blockDim.x + threadIdx.x) + (ilo); \
+  const int j = (int)(tile_.y * blockDim.y + threadIdx.y) + (jlo); \
+  const int k = (int)tile_.z + (klo);
 #endif
 
 // ---------------------------------------------------------------------------
@@ -613,26 +643,26 @@ __device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j
   (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
-  const double dxci = g.dxc[i - 1];
+  const double * xt = g.xt + kXT * i;
   const double u_c = ux[I_UX(i, j, k)];
   const double u_xm = ux[I_UX(i - 1, j, k)], u_xp = ux[I_UX(i + 1, j, k)];
   const double u_ym = ux[I_UX(i, j - 1, k)], u_yp = ux[I_UX(i, j + 1, k)];
   double sa = 0., sg = 0.;
-  {  // advection_x
+  {  // advection_x: +-0.5 / DXC(i) * ...
     const double ux_l = +0.5 * u_xm + 0.5 * u_c;
     const double ux_u = +0.5 * u_c + 0.5 * u_xp;
-    const double l = +0.5 / dxci * ux_l;
-    const double u = -0.5 / dxci * ux_u;
+    const double l = xt[XT_HXC] * ux_l;
+    const double u = -xt[XT_HXC] * ux_u;
     const double c = -l - u;
     sa += +l * u_xm + c * u_c + u * u_xp;
   }
-  const double w_xm = 0.5 * g.dxf[i - 2] / dxci;
-  const double w_xp = 0.5 * g.dxf[i - 1] / dxci;
+  const double w_xm = xt[XT_WXM];   // 0.5 * DXF(i-1) / DXC(i)
+  const double w_xp = xt[XT_WXP];   // 0.5 * DXF(i) / DXC(i)
   {  // advection_y
     const double uy_l = w_xm * uy[I_C1(i - 1, j, k)] + w_xp * uy[I_C1(i, j, k)];
     const double uy_u = w_xm * uy[I_C1(i - 1, j + 1, k)] + w_xp * uy[I_C1(i, j + 1, k)];
-    const double l = +0.5 / g.dy * uy_l;
-    const double u = -0.5 / g.dy * uy_u;
+    const double l = g.hdy * uy_l;
+    const double u = -g.hdy * uy_u;
     const double c = -l - u;
     sa += +l * u_ym + c * u_c + u * u_yp;
   }
@@ -642,8 +672,8 @@ __device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j
     const double * uz = a.uz;
     const double uz_l = w_xm * uz[I_C1(i - 1, j, k)] + w_xp * uz[I_C1(i, j, k)];
     const double uz_u = w_xm * uz[I_C1(i - 1, j, k + 1)] + w_xp * uz[I_C1(i, j, k + 1)];
-    const double l = +0.5 / g.dz * uz_l;
-    const double u = -0.5 / g.dz * uz_u;
+    const double l = g.hdz * uz_l;
+    const double u = -g.hdz * uz_u;
     const double c = -l - u;
     sa += +l * u_zm + c * u_c + u * u_zp;
   }
@@ -653,16 +683,14 @@ __device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j
     sg += a.dif * (L[0] * u_xm + L[1] * u_c + L[2] * u_xp);
   }
   {  // diffusion_y -> srca
-    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
-    sa += a.dif * (ly * u_ym + cy * u_c + ly * u_yp);
+    sa += a.dif * (g.lyy * u_ym + g.cyy * u_c + g.lyy * u_yp);
   }
 #if NDIMS == 3
   {
-    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
-    sa += a.dif * (lz * u_zm + cz * u_c + lz * u_zp);
+    sa += a.dif * (g.lzz * u_zm + g.czz * u_c + g.lzz * u_zp);
   }
 #endif
-  sg -= 1. / dxci * (-a.p[I_C1(i - 1, j, k)] + a.p[I_C1(i, j, k)]);
+  sg -= xt[XT_RXC] * (-a.p[I_C1(i - 1, j, k)] + a.p[I_C1(i, j, k)]);
   if (a.add_buoyancy) sa += +0.5 * a.t[I_C1(i - 1, j, k)] + 0.5 * a.t[I_C1(i, j, k)];
   sa += ifrc;
   sa_out = sa;
@@ -689,7 +717,7 @@ __device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j
   (void)nx; (void)ny;
   const double * ux = a.ux;
   const double * uy = a.uy;
-  const double dxfi = g.dxf[i - 1];
+  const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
   const double v_c = uy[I_C1(i, j, k)];
   const double v_xm = uy[I_C1(i - 1, j, k)], v_xp = uy[I_C1(i + 1, j, k)];
   const double v_ym = uy[I_C1(i, j - 1, k)], v_yp = uy[I_C1(i, j + 1, k)];
@@ -697,16 +725,16 @@ __device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j
   {  // advection_x
     const double ux_l = +0.5 * ux[I_UX(i, j - 1, k)] + 0.5 * ux[I_UX(i, j, k)];
     const double ux_u = +0.5 * ux[I_UX(i + 1, j - 1, k)] + 0.5 * ux[I_UX(i + 1, j, k)];
-    const double l = +0.5 / dxfi * ux_l;
-    const double u = -0.5 / dxfi * ux_u;
+    const double l = hxf * ux_l;
+    const double u = -hxf * ux_u;
     const double c = -l - u;
     sa += +l * v_xm + c * v_c + u * v_xp;
   }
   {  // advection_y
     const double uy_l = +0.5 * v_ym + 0.5 * v_c;
     const double uy_u = +0.5 * v_c + 0.5 * v_yp;
-    const double l = +0.5 / g.dy * uy_l;
-    const double u = -0.5 / g.dy * uy_u;
+    const double l = g.hdy * uy_l;
+    const double u = -g.hdy * uy_u;
     const double c = -l - u;
     sa += +l * v_ym + c * v_c + u * v_yp;
   }
@@ -716,8 +744,8 @@ __device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j
     const double * uz = a.uz;
     const double uz_l = +0.5 * uz[I_C1(i, j - 1, k)] + 0.5 * uz[I_C1(i, j, k)];
     const double uz_u = +0.5 * uz[I_C1(i, j - 1, k + 1)] + 0.5 * uz[I_C1(i, j, k + 1)];
-    const double l = +0.5 / g.dz * uz_l;
-    const double u = -0.5 / g.dz * uz_u;
+    const double l = g.hdz * uz_l;
+    const double u = -g.hdz * uz_u;
     const double c = -l - u;
     sa += +l * v_zm + c * v_c + u * v_zp;
   }
@@ -727,16 +755,14 @@ __device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j
     sg += a.dif * (L[0] * v_xm + L[1] * v_c + L[2] * v_xp);
   }
   {
-    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
-    sa += a.dif * (ly * v_ym + cy * v_c + ly * v_yp);
+    sa += a.dif * (g.lyy * v_ym + g.cyy * v_c + g.lyy * v_yp);
   }
 #if NDIMS == 3
   {
-    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
-    sa += a.dif * (lz * v_zm + cz * v_c + lz * v_zp);
+    sa += a.dif * (g.lzz * v_zm + g.czz * v_c + g.lzz * v_zp);
   }
 #endif
-  sg -= 1. / g.dy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
+  sg -= g.rdy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
   sa += ifrc;
   sa_out = sa;
   sg_out = sg;
@@ -764,7 +790,7 @@ __device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double * uz = a.uz;
-  const double dxfi = g.dxf[i - 1];
+  const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
   const double w_c = uz[I_C1(i, j, k)];
   const double w_xm = uz[I_C1(i - 1, j, k)], w_xp = uz[I_C1(i + 1, j, k)];
   const double w_ym = uz[I_C1(i, j - 1, k)], w_yp = uz[I_C1(i, j + 1, k)];
@@ -773,24 +799,24 @@ __device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j
   {
     const double ux_l = +0.5 * ux[I_UX(i, j, k - 1)] + 0.5 * ux[I_UX(i, j, k)];
     const double ux_u = +0.5 * ux[I_UX(i + 1, j, k - 1)] + 0.5 * ux[I_UX(i + 1, j, k)];
-    const double l = +0.5 / dxfi * ux_l;
-    const double u = -0.5 / dxfi * ux_u;
+    const double l = hxf * ux_l;
+    const double u = -hxf * ux_u;
     const double c = -l - u;
     sa += +l * w_xm + c * w_c + u * w_xp;
   }
   {
     const double uy_l = +0.5 * uy[I_C1(i, j, k - 1)] + 0.5 * uy[I_C1(i, j, k)];
     const double uy_u = +0.5 * uy[I_C1(i, j + 1, k - 1)] + 0.5 * uy[I_C1(i, j + 1, k)];
-    const double l = +0.5 / g.dy * uy_l;
-    const double u = -0.5 / g.dy * uy_u;
+    const double l = g.hdy * uy_l;
+    const double u = -g.hdy * uy_u;
     const double c = -l - u;
     sa += +l * w_ym + c * w_c + u * w_yp;
   }
   {
     const double uz_l = +0.5 * w_zm + 0.5 * w_c;
     const double uz_u = +0.5 * w_c + 0.5 * w_zp;
-    const double l = +0.5 / g.dz * uz_l;
-    const double u = -0.5 / g.dz * uz_u;
+    const double l = g.hdz * uz_l;
+    const double u = -g.hdz * uz_u;
     const double c = -l - u;
     sa += +l * w_zm + c * w_c + u * w_zp;
   }
@@ -799,14 +825,12 @@ __device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j
     sg += a.dif * (L[0] * w_xm + L[1] * w_c + L[2] * w_xp);
   }
   {
-    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
-    sa += a.dif * (ly * w_ym + cy * w_c + ly * w_yp);
+    sa += a.dif * (g.lyy * w_ym + g.cyy * w_c + g.lyy * w_yp);
   }
   {
-    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
-    sa += a.dif * (lz * w_zm + cz * w_c + lz * w_zp);
+    sa += a.dif * (g.lzz * w_zm + g.czz * w_c + g.lzz * w_zp);
   }
-  sg -= 1. / g.dz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
+  sg -= g.rdz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
   sa += ifrc;
   sa_out = sa;
   sg_out = sg;
@@ -834,20 +858,20 @@ __device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j,
   const double * ux = a.ux;
   const double * uy = a.uy;
   const double * t = a.t;
-  const double dxfi = g.dxf[i - 1];
+  const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
   const double t_c = t[I_C1(i, j, k)];
   const double t_xm = t[I_C1(i - 1, j, k)], t_xp = t[I_C1(i + 1, j, k)];
   const double t_ym = t[I_C1(i, j - 1, k)], t_yp = t[I_C1(i, j + 1, k)];
   double sa = 0., sg = 0.;
   {
-    const double l = +0.5 / dxfi * ux[I_UX(i, j, k)];
-    const double u = -0.5 / dxfi * ux[I_UX(i + 1, j, k)];
+    const double l = hxf * ux[I_UX(i, j, k)];
+    const double u = -hxf * ux[I_UX(i + 1, j, k)];
     const double c = -l - u;
     sa += +l * t_xm + c * t_c + u * t_xp;
   }
   {
-    const double l = +0.5 / g.dy * uy[I_C1(i, j, k)];
-    const double u = -0.5 / g.dy * uy[I_C1(i, j + 1, k)];
+    const double l = g.hdy * uy[I_C1(i, j, k)];
+    const double u = -g.hdy * uy[I_C1(i, j + 1, k)];
     const double c = -l - u;
     sa += +l * t_ym + c * t_c + u * t_yp;
   }
@@ -855,8 +879,8 @@ __device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j,
   const double t_zm = t[I_C1(i, j, k - 1)], t_zp = t[I_C1(i, j, k + 1)];
   {
     const double * uz = a.uz;
-    const double l = +0.5 / g.dz * uz[I_C1(i, j, k)];
-    const double u = -0.5 / g.dz * uz[I_C1(i, j, k + 1)];
+    const double l = g.hdz * uz[I_C1(i, j, k)];
+    const double u = -g.hdz * uz[I_C1(i, j, k + 1)];
     const double c = -l - u;
     sa += +l * t_zm + c * t_c + u * t_zp;
   }
@@ -866,13 +890,11 @@ __device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j,
     sg += a.dif * (L[0] * t_xm + L[1] * t_c + L[2] * t_xp);
   }
   {
-    const double ly = +1. / g.dy / g.dy, cy = -2. / g.dy / g.dy;
-    sa += a.dif * (ly * t_ym + cy * t_c + ly * t_yp);
+    sa += a.dif * (g.lyy * t_ym + g.cyy * t_c + g.lyy * t_yp);
   }
 #if NDIMS == 3
   {
-    const double lz = +1. / g.dz / g.dz, cz = -2. / g.dz / g.dz;
-    sa += a.dif * (lz * t_zm + cz * t_c + lz * t_zp);
+    sa += a.dif * (g.lzz * t_zm + g.czz * t_c + g.lzz * t_zp);
   }
 #endif
   (void)ifrc;
@@ -2249,7 +2271,7 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
   __shared__ double sv[VX * VY * VZ];
   __shared__ double sc[NDIMS][CX * CY * CZ];
   const int nx = g.nx, ny = g.ny;
-  const Tile3 tile = xcd_tile(vb);
+  const Tile3 tile = g.bands ? band_tile(vb) : plain_tile(vb);
   const int i0 = 1 + tile.x * kIX, j0 = 1 + tile.y * kIY;
 #if NDIMS == 3
   const int nz = g.nz;
@@ -2461,46 +2483,53 @@ __device__ __forceinline__ void k_rhs_fused_body(const VBlock & vb, Grid g, Fuse
   const double v = vof[I_VOF(i, j, k)];
   const double cv = curv[I_C1(i, j, k)];
   RhsArgs a = f.base;
-  double sa, sg;
   const double ca = a.ca, cb = a.cb, cg = a.cg;
+  // every source first, every store last: no store sits between the loads
+  // of the four stencils, so shared neighbours are loaded once
+  double sa0 = 0., sg0 = 0., sa1, sg1, sa3, sg3;
   if (i >= 2) {
-    const double grad = 1. / g.dxc[i - 1] * (-vof[I_VOF(i - 1, j, k)] + v);
+    const double grad = g.xt[kXT * i + XT_RXC] * (-vof[I_VOF(i - 1, j, k)] + v);   // 1/DXC(i) * ...
     const double kappa = 0.5 * (curv[I_C1(i - 1, j, k)] + cv);
     a.lapx = f.lapx[0];
     a.dif = f.dif[0];
-    rhs_ux_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
-    const size_t is = I_SX(i, j, k);
-    f.srca[0][is] = sa;
-    f.delta[0][is] = +ca * sa + cb * f.srcb[0][is] + cg * sg;
+    rhs_ux_at(g, a, i, j, k, f.tension * grad * kappa, sa0, sg0);
   }
-  const size_t is = I_SC(i, j, k);
   {
-    const double grad = 1. / g.dy * (-vof[I_VOF(i, j - 1, k)] + v);
+    const double grad = g.rdy * (-vof[I_VOF(i, j - 1, k)] + v);
     const double kappa = 0.5 * (curv[I_C1(i, j - 1, k)] + cv);
     a.lapx = f.lapx[1];
     a.dif = f.dif[1];
-    rhs_uy_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
-    f.srca[1][is] = sa;
-    f.delta[1][is] = +ca * sa + cb * f.srcb[1][is] + cg * sg;
+    rhs_uy_at(g, a, i, j, k, f.tension * grad * kappa, sa1, sg1);
   }
 #if NDIMS == 3
+  double sa2, sg2;
   {
-    const double grad = 1. / g.dz * (-vof[I_VOF(i, j, k - 1)] + v);
+    const double grad = g.rdz * (-vof[I_VOF(i, j, k - 1)] + v);
     const double kappa = 0.5 * (curv[I_C1(i, j, k - 1)] + cv);
     a.lapx = f.lapx[2];
     a.dif = f.dif[2];
-    rhs_uz_at(g, a, i, j, k, f.tension * grad * kappa, sa, sg);
-    f.srca[2][is] = sa;
-    f.delta[2][is] = +ca * sa + cb * f.srcb[2][is] + cg * sg;
+    rhs_uz_at(g, a, i, j, k, f.tension * grad * kappa, sa2, sg2);
   }
 #endif
   {
     a.lapx = f.lapx[3];
     a.dif = f.dif[3];
-    rhs_t_at(g, a, i, j, k, 0., sa, sg);
-    f.srca[3][is] = sa;
-    f.delta[3][is] = +ca * sa + cb * f.srcb[3][is] + cg * sg;
+    rhs_t_at(g, a, i, j, k, 0., sa3, sg3);
   }
+  const size_t is = I_SC(i, j, k);
+  if (i >= 2) {
+    const size_t ix = I_SX(i, j, k);
+    f.srca[0][ix] = sa0;
+    f.delta[0][ix] = +ca * sa0 + cb * f.srcb[0][ix] + cg * sg0;
+  }
+  f.srca[1][is] = sa1;
+  f.delta[1][is] = +ca * sa1 + cb * f.srcb[1][is] + cg * sg1;
+#if NDIMS == 3
+  f.srca[2][is] = sa2;
+  f.delta[2][is] = +ca * sa2 + cb * f.srcb[2][is] + cg * sg2;
+#endif
+  f.srca[3][is] = sa3;
+  f.delta[3][is] = +ca * sa3 + cb * f.srcb[3][is] + cg * sg3;
 }
 
 __global__ __launch_bounds__(256) void k_rhs_fused(Grid g, FusedRhs f) {

@@ -845,7 +845,39 @@ static int ctx_create(domain_t * domain) {
   HIPCHK(hipMalloc(&ctx->d_dxc, sizeof(double) * (nx + 1)));
   HIPCHK(hipMemcpy(ctx->d_dxf, domain->dxf, sizeof(double) * nx, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx->d_dxc, domain->dxc, sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
-  ctx->grid = Grid{nx, ny, nz, ctx->dy, ctx->dz, ctx->d_dxf, ctx->d_dxc};
+  ctx->grid = Grid{nx, ny, nz, ctx->dy, ctx->dz, ctx->d_dxf, ctx->d_dxc, 0,
+                   0., 0., 0., 0., 0., 0., 0., 0., nullptr};
+  {
+    // the stencils' quotients, by the divisions the reference spells
+    Grid & g = ctx->grid;
+    const double dy = ctx->dy, dz = ctx->dz;
+    g.hdy = +0.5 / dy;
+    g.rdy = 1. / dy;
+    g.lyy = +1. / dy / dy;
+    g.cyy = -2. / dy / dy;
+    g.hdz = +0.5 / dz;
+    g.rdz = 1. / dz;
+    g.lzz = +1. / dz / dz;
+    g.czz = -2. / dz / dz;
+    std::vector<double> xt((size_t)vofk::kXT * (nx + 2), 0.);
+    const double * dxf = domain->dxf;
+    const double * dxc = domain->dxc;
+    for (int i = 1; i <= nx + 1; i++) {
+      double * e = &xt[(size_t)vofk::kXT * i];
+      e[vofk::XT_HXC] = +0.5 / dxc[i - 1];
+      e[vofk::XT_RXC] = 1. / dxc[i - 1];
+      if (i >= 2) e[vofk::XT_WXM] = 0.5 * dxf[i - 2] / dxc[i - 1];
+      if (i <= nx) {
+        e[vofk::XT_WXP] = 0.5 * dxf[i - 1] / dxc[i - 1];
+        e[vofk::XT_HXF] = +0.5 / dxf[i - 1];
+        e[vofk::XT_RXF] = 1. / dxf[i - 1];
+      }
+    }
+    double * d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * xt.size()));
+    HIPCHK(hipMemcpy(d, xt.data(), sizeof(double) * xt.size(), hipMemcpyHostToDevice));
+    g.xt = d;
+  }
   // Laplacians in x (predict/ux.c:35-62, uy.c:35-62, t.c:33-60)
   const double * dxf = domain->dxf;
   const double * dxc = domain->dxc;
@@ -1002,6 +1034,7 @@ extern "C" int domain_finalise(domain_t * domain) {
   vof_ctx * ctx = ctx_of(domain);
   if (ctx) {
     (void)hipStreamSynchronize(ctx->stream);
+    dev_free((void *)ctx->grid.xt);
     dev_free(ctx->d_dxf); dev_free(ctx->d_dxc); dev_free(ctx->d_lapx_ux); dev_free(ctx->d_lapx_c);
     // tables inside a live batch's block belong to the batch
     if (!ctx->tab_shared) {
@@ -1402,6 +1435,7 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     int lpb = ctx->lines_lpb > 0 ? ctx->lines_lpb : (int)(32768 / (sizeof(double) * (nx | 1)));
     if (lpb > 64) lpb = 64;
     if (lpb < 1) lpb = 1;
+    if (sizeof(double) * (nx | 1) > 65536) return fail("x solver: nx = %d exceeds the LDS line tile", nx);
     if ((size_t)lpb * (nx | 1) * sizeof(double) > 65536) lpb = (int)(65536 / (sizeof(double) * (nx | 1)));
     const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
     const dim3 lg((unsigned)((lj.nlines + lpb - 1) / lpb), (unsigned)nj);
@@ -1820,6 +1854,10 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     if (value != 1024 && value != 2048) return fail("fft_elems must be 1024 or 2048");
     if (value < ctx->ny || (NDIMS == 3 && value < ctx->nz)) return fail("fft_elems below the FFT length");
     ctx->fft_elems = value;
+    return 0;
+  }
+  if (!strcmp(name, "xcd_bands")) {
+    ctx->grid.bands = value ? 1 : 0;
     return 0;
   }
   if (!strcmp(name, "adj_floor_exp")) {
