@@ -1202,43 +1202,64 @@ __device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, i
     }
   }
   __syncthreads();
-  // phase 3: field rows += solution (and their halo images)
+  // phase 3: field rows += solution.  Wavefront w owns lines w, w + 4, ...
+  // (two per pass), so a line's row / plane, its address and whether it has
+  // periodic halo images are wave-uniform; lanes walk x contiguously.
   {
-    int e = tid, ll = tid / n, m = tid - (tid / n) * n;
-    while (e < total) {
-      double val[kWUnroll];
-      int xs[kWUnroll], rs[kWUnroll], ps[kWUnroll];
-      int cnt = 0;
+    constexpr int kWaves = kWThreads / 64;
+    const int lane = tid & 63;
+    for (int la = tid >> 6; la < nl; la += 2 * kWaves) {
+      const int lb = la + kWaves;
+      const bool hb = lb < nl;
+      const int linea = line0 + la, lineb = line0 + (hb ? lb : la);
+      const int ja = linea % J.ny, ka = linea / J.ny;
+      const int jb = lineb % J.ny, kb = lineb / J.ny;
+      double * __restrict__ ra = F.field + F.i0 + (size_t)F.n0 * ((size_t)(ja + 1) + (size_t)J.n1 * (size_t)(ka + 1));
+      double * __restrict__ rb = F.field + F.i0 + (size_t)F.n0 * ((size_t)(jb + 1) + (size_t)J.n1 * (size_t)(kb + 1));
+      const double * sa = S + la * ld;
+      const double * sb = S + (hb ? lb : la) * ld;
+      for (int m0 = 0; m0 < n; m0 += 64 * kWUnroll) {
+        double va[kWUnroll], vb[kWUnroll];
 #pragma unroll
-      for (int u = 0; u < kWUnroll; u++) {
-        if (e < total) {
-          const int line = line0 + ll;
-          const int jj = line % J.ny, kk = line / J.ny;
-          xs[u] = F.i0 + m;
-          rs[u] = jj + 1;
-          ps[u] = kk + 1;
-          val[u] = F.field[(size_t)xs[u] + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])] + S[ll * ld + m];
-          cnt = u + 1;
-          e += kWThreads;
-          m += kWThreads;
-          while (m >= n) { m -= n; ll++; }
+        for (int u = 0; u < kWUnroll; u++) {
+          const int m = m0 + 64 * u + lane;
+          if (m < n) {
+            va[u] = ra[m] + sa[m];
+            if (hb) vb[u] = rb[m] + sb[m];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kWUnroll; u++) {
+          const int m = m0 + 64 * u + lane;
+          if (m < n) {
+            ra[m] = va[u];
+            if (hb) rb[m] = vb[u];
+          }
         }
       }
+      if (J.images) {
+        // rows / planes next to the periodic halo: the entries just stored
+        // go to their images too, and the Dirichlet wall entries of the row
+        // (constants the loop never changes) take their images as the
+        // reference's halo copy would (overwriting values left from the
+        // initial load)
 #pragma unroll
-      for (int u = 0; u < kWUnroll; u++) {
-        if (u < cnt) {
-          if (J.images) {
-            store_periodic(F.field, xs[u], F.n0, J.n1, rs[u], ps[u], J.ny, J.nz, 1, val[u]);
-            // the Dirichlet wall entries of this row are constants the loop
-            // never changes; their halo images take them as the reference's
-            // halo copy would (overwriting values left from the initial load)
-            if (xs[u] == F.i0 || xs[u] == F.i0 + n - 1) {
-              const int xw = xs[u] == F.i0 ? F.i0 - 1 : F.i0 + n;
-              const double wv = F.field[(size_t)xw + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])];
-              store_images(F.field, xw, F.n0, J.n1, rs[u], ps[u], J.ny, J.nz, 1, wv);
-            }
-          } else {
-            F.field[(size_t)xs[u] + (size_t)F.n0 * ((size_t)rs[u] + (size_t)J.n1 * (size_t)ps[u])] = val[u];
+        for (int h = 0; h < 2; h++) {
+          if (h == 1 && !hb) break;
+          const int jj = h ? jb : ja, kk = h ? kb : ka;
+          const bool edge = jj + 1 < 2 || jj + 1 >= J.ny
+#if NDIMS == 3
+                            || kk + 1 < 2 || kk + 1 >= J.nz
+#endif
+              ;
+          if (!edge) continue;
+          const double * row = h ? rb : ra;
+          for (int m = lane; m < n; m += 64)
+            store_images(F.field, F.i0 + m, F.n0, J.n1, jj + 1, kk + 1, J.ny, J.nz, 1, row[m]);
+          if (lane < 2) {
+            const int xw = lane == 0 ? F.i0 - 1 : F.i0 + n;
+            const double wv = F.field[(size_t)xw + (size_t)F.n0 * ((size_t)(jj + 1) + (size_t)J.n1 * (size_t)(kk + 1))];
+            store_images(F.field, xw, F.n0, J.n1, jj + 1, kk + 1, J.ny, J.nz, 1, wv);
           }
         }
       }
