@@ -65,6 +65,7 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         # ux uy uz p t, vof, curvature, srcb of 4 fields read; srca and the
         # increment of 4 fields written
         "k_rhs_fused": c * (40 + 8 + 8 + 32 + 32 + 32),
+        "k_rhs_march": c * (40 + 8 + 8 + 32 + 32 + 32),
         # psi, ux uy uz p read and written back
         "k_corr_press": c * (8 + 32 + 32),
     }
@@ -267,7 +268,7 @@ def main(argv=None) -> int:
                     help="time the mixed-precision form (FP32 transforms + K FP64 refinements) "
                          "instead of the plain FP64 step")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
-                    help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb, zchunk, xcd_remap)")
+                    help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb, xcd_bands, rhs_rows)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the mixed-precision and batched forms")
     ap.add_argument("--batch-cases", type=int, default=96)

@@ -322,8 +322,11 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
 
 /* tile parameters (results do not depend on them): "fft_elems" = 1024 or
  * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
- * per work-group of the predictor's x solver (0 = automatic);
- * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e */
+ * per work-group of the predictor's x solver (0 = automatic); "xcd_bands" =
+ * 1 walks the 3D stencil tiles band by band per XCD; "rhs_rows" = 1, 2, 4,
+ * 8 (default) or 16 rows of one column per thread of the source kernel;
+ * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e
+ * (this one changes adjoint results) */
 int vof_set_tuning(domain_t * domain, const char name[], int value);
 /* step.npy / time.npy of an IC or save directory (the reference's main.c
  * reads and writes them with fileio.r_serial / w_serial) */

@@ -636,65 +636,97 @@ __device__ inline void rhs_store(const RhsArgs & a, size_t is, double sa, double
   }
 }
 
-// compute_rhs_ux (ux.c:71-249), DOFs i = 2..nx
-__device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
-                               double ifrc, double & sa_out, double & sg_out) {
-  const int nx = g.nx, ny = g.ny;
-  (void)nx; (void)ny;
-  const double * ux = a.ux;
-  const double * uy = a.uy;
+// Stencil values of one ux DOF (i, j, k): ux and its six neighbours, uy at
+// (i-1, j), (i, j), (i-1, j+1), (i, j+1), uz at (i-1, j, k), (i, j, k),
+// (i-1, j, k+1), (i, j, k+1), p and t at (i-1, j, k), (i, j, k).
+struct UxSt {
+  double c, xm, xp, ym, yp, zm, zp;
+  double vlm, vlc, vum, vuc;
+  double wlm, wlc, wum, wuc;
+  double pm, pc, tm, tc;
+};
+
+// compute_rhs_ux (ux.c:71-249), DOFs i = 2..nx, from the stencil values
+__device__ inline void rhs_ux_v(const Grid & g, const RhsArgs & a, int i, const UxSt & s,
+                                double ifrc, double & sa_out, double & sg_out) {
   const double * xt = g.xt + kXT * i;
-  const double u_c = ux[I_UX(i, j, k)];
-  const double u_xm = ux[I_UX(i - 1, j, k)], u_xp = ux[I_UX(i + 1, j, k)];
-  const double u_ym = ux[I_UX(i, j - 1, k)], u_yp = ux[I_UX(i, j + 1, k)];
   double sa = 0., sg = 0.;
   {  // advection_x: +-0.5 / DXC(i) * ...
-    const double ux_l = +0.5 * u_xm + 0.5 * u_c;
-    const double ux_u = +0.5 * u_c + 0.5 * u_xp;
+    const double ux_l = +0.5 * s.xm + 0.5 * s.c;
+    const double ux_u = +0.5 * s.c + 0.5 * s.xp;
     const double l = xt[XT_HXC] * ux_l;
     const double u = -xt[XT_HXC] * ux_u;
     const double c = -l - u;
-    sa += +l * u_xm + c * u_c + u * u_xp;
+    sa += +l * s.xm + c * s.c + u * s.xp;
   }
   const double w_xm = xt[XT_WXM];   // 0.5 * DXF(i-1) / DXC(i)
   const double w_xp = xt[XT_WXP];   // 0.5 * DXF(i) / DXC(i)
   {  // advection_y
-    const double uy_l = w_xm * uy[I_C1(i - 1, j, k)] + w_xp * uy[I_C1(i, j, k)];
-    const double uy_u = w_xm * uy[I_C1(i - 1, j + 1, k)] + w_xp * uy[I_C1(i, j + 1, k)];
+    const double uy_l = w_xm * s.vlm + w_xp * s.vlc;
+    const double uy_u = w_xm * s.vum + w_xp * s.vuc;
     const double l = g.hdy * uy_l;
     const double u = -g.hdy * uy_u;
     const double c = -l - u;
-    sa += +l * u_ym + c * u_c + u * u_yp;
+    sa += +l * s.ym + c * s.c + u * s.yp;
   }
 #if NDIMS == 3
-  const double u_zm = ux[I_UX(i, j, k - 1)], u_zp = ux[I_UX(i, j, k + 1)];
   {  // advection_z
-    const double * uz = a.uz;
-    const double uz_l = w_xm * uz[I_C1(i - 1, j, k)] + w_xp * uz[I_C1(i, j, k)];
-    const double uz_u = w_xm * uz[I_C1(i - 1, j, k + 1)] + w_xp * uz[I_C1(i, j, k + 1)];
+    const double uz_l = w_xm * s.wlm + w_xp * s.wlc;
+    const double uz_u = w_xm * s.wum + w_xp * s.wuc;
     const double l = g.hdz * uz_l;
     const double u = -g.hdz * uz_u;
     const double c = -l - u;
-    sa += +l * u_zm + c * u_c + u * u_zp;
+    sa += +l * s.zm + c * s.c + u * s.zp;
   }
 #endif
   {  // diffusion_x -> srcg
     const double * L = a.lapx + 3 * (i - 2);
-    sg += a.dif * (L[0] * u_xm + L[1] * u_c + L[2] * u_xp);
+    sg += a.dif * (L[0] * s.xm + L[1] * s.c + L[2] * s.xp);
   }
   {  // diffusion_y -> srca
-    sa += a.dif * (g.lyy * u_ym + g.cyy * u_c + g.lyy * u_yp);
+    sa += a.dif * (g.lyy * s.ym + g.cyy * s.c + g.lyy * s.yp);
   }
 #if NDIMS == 3
   {
-    sa += a.dif * (g.lzz * u_zm + g.czz * u_c + g.lzz * u_zp);
+    sa += a.dif * (g.lzz * s.zm + g.czz * s.c + g.lzz * s.zp);
   }
 #endif
-  sg -= xt[XT_RXC] * (-a.p[I_C1(i - 1, j, k)] + a.p[I_C1(i, j, k)]);
-  if (a.add_buoyancy) sa += +0.5 * a.t[I_C1(i - 1, j, k)] + 0.5 * a.t[I_C1(i, j, k)];
+  sg -= xt[XT_RXC] * (-s.pm + s.pc);
+  if (a.add_buoyancy) sa += +0.5 * s.tm + 0.5 * s.tc;
   sa += ifrc;
   sa_out = sa;
   sg_out = sg;
+}
+
+__device__ inline void rhs_ux_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                                 double ifrc, double & sa_out, double & sg_out) {
+  const int nx = g.nx, ny = g.ny;
+  (void)nx; (void)ny;
+  UxSt s;
+  s.c = a.ux[I_UX(i, j, k)];
+  s.xm = a.ux[I_UX(i - 1, j, k)];
+  s.xp = a.ux[I_UX(i + 1, j, k)];
+  s.ym = a.ux[I_UX(i, j - 1, k)];
+  s.yp = a.ux[I_UX(i, j + 1, k)];
+  s.vlm = a.uy[I_C1(i - 1, j, k)];
+  s.vlc = a.uy[I_C1(i, j, k)];
+  s.vum = a.uy[I_C1(i - 1, j + 1, k)];
+  s.vuc = a.uy[I_C1(i, j + 1, k)];
+#if NDIMS == 3
+  s.zm = a.ux[I_UX(i, j, k - 1)];
+  s.zp = a.ux[I_UX(i, j, k + 1)];
+  s.wlm = a.uz[I_C1(i - 1, j, k)];
+  s.wlc = a.uz[I_C1(i, j, k)];
+  s.wum = a.uz[I_C1(i - 1, j, k + 1)];
+  s.wuc = a.uz[I_C1(i, j, k + 1)];
+#else
+  s.zm = s.zp = s.wlm = s.wlc = s.wum = s.wuc = 0.;
+#endif
+  s.pm = a.p[I_C1(i - 1, j, k)];
+  s.pc = a.p[I_C1(i, j, k)];
+  s.tm = a.add_buoyancy ? a.t[I_C1(i - 1, j, k)] : 0.;
+  s.tc = a.add_buoyancy ? a.t[I_C1(i, j, k)] : 0.;
+  rhs_ux_v(g, a, i, s, ifrc, sa_out, sg_out);
 }
 
 __device__ __forceinline__ void k_rhs_ux_body(const VBlock & vb, Grid g, RhsArgs a) {
@@ -710,62 +742,92 @@ __global__ void k_rhs_ux(Grid g, RhsArgs a) {
   k_rhs_ux_body(hw_block(), g, a);
 }
 
-// compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx
-__device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
-                               double ifrc, double & sa_out, double & sg_out) {
-  const int nx = g.nx, ny = g.ny;
-  (void)nx; (void)ny;
-  const double * ux = a.ux;
-  const double * uy = a.uy;
+// Stencil values of one uy DOF (i, j, k): uy and its six neighbours, ux at
+// (i, j-1), (i, j), (i+1, j-1), (i+1, j), uz at (i, j-1, k), (i, j, k),
+// (i, j-1, k+1), (i, j, k+1), p at (i, j-1, k), (i, j, k).
+struct UySt {
+  double c, xm, xp, ym, yp, zm, zp;
+  double ulm, ulc, uum, uuc;
+  double wlm, wlc, wum, wuc;
+  double pm, pc;
+};
+
+// compute_rhs_uy (uy.c:70-221), DOFs i = 1..nx, from the stencil values
+__device__ inline void rhs_uy_v(const Grid & g, const RhsArgs & a, int i, const UySt & s,
+                                double ifrc, double & sa_out, double & sg_out) {
   const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
-  const double v_c = uy[I_C1(i, j, k)];
-  const double v_xm = uy[I_C1(i - 1, j, k)], v_xp = uy[I_C1(i + 1, j, k)];
-  const double v_ym = uy[I_C1(i, j - 1, k)], v_yp = uy[I_C1(i, j + 1, k)];
   double sa = 0., sg = 0.;
   {  // advection_x
-    const double ux_l = +0.5 * ux[I_UX(i, j - 1, k)] + 0.5 * ux[I_UX(i, j, k)];
-    const double ux_u = +0.5 * ux[I_UX(i + 1, j - 1, k)] + 0.5 * ux[I_UX(i + 1, j, k)];
+    const double ux_l = +0.5 * s.ulm + 0.5 * s.ulc;
+    const double ux_u = +0.5 * s.uum + 0.5 * s.uuc;
     const double l = hxf * ux_l;
     const double u = -hxf * ux_u;
     const double c = -l - u;
-    sa += +l * v_xm + c * v_c + u * v_xp;
+    sa += +l * s.xm + c * s.c + u * s.xp;
   }
   {  // advection_y
-    const double uy_l = +0.5 * v_ym + 0.5 * v_c;
-    const double uy_u = +0.5 * v_c + 0.5 * v_yp;
+    const double uy_l = +0.5 * s.ym + 0.5 * s.c;
+    const double uy_u = +0.5 * s.c + 0.5 * s.yp;
     const double l = g.hdy * uy_l;
     const double u = -g.hdy * uy_u;
     const double c = -l - u;
-    sa += +l * v_ym + c * v_c + u * v_yp;
+    sa += +l * s.ym + c * s.c + u * s.yp;
   }
 #if NDIMS == 3
-  const double v_zm = uy[I_C1(i, j, k - 1)], v_zp = uy[I_C1(i, j, k + 1)];
   {  // advection_z
-    const double * uz = a.uz;
-    const double uz_l = +0.5 * uz[I_C1(i, j - 1, k)] + 0.5 * uz[I_C1(i, j, k)];
-    const double uz_u = +0.5 * uz[I_C1(i, j - 1, k + 1)] + 0.5 * uz[I_C1(i, j, k + 1)];
+    const double uz_l = +0.5 * s.wlm + 0.5 * s.wlc;
+    const double uz_u = +0.5 * s.wum + 0.5 * s.wuc;
     const double l = g.hdz * uz_l;
     const double u = -g.hdz * uz_u;
     const double c = -l - u;
-    sa += +l * v_zm + c * v_c + u * v_zp;
+    sa += +l * s.zm + c * s.c + u * s.zp;
   }
 #endif
   {
     const double * L = a.lapx + 3 * (i - 1);
-    sg += a.dif * (L[0] * v_xm + L[1] * v_c + L[2] * v_xp);
+    sg += a.dif * (L[0] * s.xm + L[1] * s.c + L[2] * s.xp);
   }
   {
-    sa += a.dif * (g.lyy * v_ym + g.cyy * v_c + g.lyy * v_yp);
+    sa += a.dif * (g.lyy * s.ym + g.cyy * s.c + g.lyy * s.yp);
   }
 #if NDIMS == 3
   {
-    sa += a.dif * (g.lzz * v_zm + g.czz * v_c + g.lzz * v_zp);
+    sa += a.dif * (g.lzz * s.zm + g.czz * s.c + g.lzz * s.zp);
   }
 #endif
-  sg -= g.rdy * (-a.p[I_C1(i, j - 1, k)] + a.p[I_C1(i, j, k)]);
+  sg -= g.rdy * (-s.pm + s.pc);
   sa += ifrc;
   sa_out = sa;
   sg_out = sg;
+}
+
+__device__ inline void rhs_uy_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                                 double ifrc, double & sa_out, double & sg_out) {
+  const int nx = g.nx, ny = g.ny;
+  (void)nx; (void)ny;
+  UySt s;
+  s.c = a.uy[I_C1(i, j, k)];
+  s.xm = a.uy[I_C1(i - 1, j, k)];
+  s.xp = a.uy[I_C1(i + 1, j, k)];
+  s.ym = a.uy[I_C1(i, j - 1, k)];
+  s.yp = a.uy[I_C1(i, j + 1, k)];
+  s.ulm = a.ux[I_UX(i, j - 1, k)];
+  s.ulc = a.ux[I_UX(i, j, k)];
+  s.uum = a.ux[I_UX(i + 1, j - 1, k)];
+  s.uuc = a.ux[I_UX(i + 1, j, k)];
+#if NDIMS == 3
+  s.zm = a.uy[I_C1(i, j, k - 1)];
+  s.zp = a.uy[I_C1(i, j, k + 1)];
+  s.wlm = a.uz[I_C1(i, j - 1, k)];
+  s.wlc = a.uz[I_C1(i, j, k)];
+  s.wum = a.uz[I_C1(i, j - 1, k + 1)];
+  s.wuc = a.uz[I_C1(i, j, k + 1)];
+#else
+  s.zm = s.zp = s.wlm = s.wlc = s.wum = s.wuc = 0.;
+#endif
+  s.pm = a.p[I_C1(i, j - 1, k)];
+  s.pc = a.p[I_C1(i, j, k)];
+  rhs_uy_v(g, a, i, s, ifrc, sa_out, sg_out);
 }
 
 __device__ __forceinline__ void k_rhs_uy_body(const VBlock & vb, Grid g, RhsArgs a) {
@@ -782,58 +844,84 @@ __global__ void k_rhs_uy(Grid g, RhsArgs a) {
 }
 
 #if NDIMS == 3
+// Stencil values of one uz DOF (i, j, k): uz and its six neighbours, ux at
+// (i, j, k-1), (i, j, k), (i+1, j, k-1), (i+1, j, k), uy at (i, j, k-1),
+// (i, j, k), (i, j+1, k-1), (i, j+1, k), p at (i, j, k-1), (i, j, k).
+struct UzSt {
+  double c, xm, xp, ym, yp, zm, zp;
+  double ulm, ulc, uum, uuc;
+  double vlm, vlc, vum, vuc;
+  double pm, pc;
+};
+
 // compute_rhs_uz: the z counterpart of compute_rhs_uy, DOFs i = 1..nx
-__device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
-                               double ifrc, double & sa_out, double & sg_out) {
-  const int nx = g.nx, ny = g.ny;
-  (void)nx; (void)ny;
-  const double * ux = a.ux;
-  const double * uy = a.uy;
-  const double * uz = a.uz;
+__device__ inline void rhs_uz_v(const Grid & g, const RhsArgs & a, int i, const UzSt & s,
+                                double ifrc, double & sa_out, double & sg_out) {
   const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
-  const double w_c = uz[I_C1(i, j, k)];
-  const double w_xm = uz[I_C1(i - 1, j, k)], w_xp = uz[I_C1(i + 1, j, k)];
-  const double w_ym = uz[I_C1(i, j - 1, k)], w_yp = uz[I_C1(i, j + 1, k)];
-  const double w_zm = uz[I_C1(i, j, k - 1)], w_zp = uz[I_C1(i, j, k + 1)];
   double sa = 0., sg = 0.;
   {
-    const double ux_l = +0.5 * ux[I_UX(i, j, k - 1)] + 0.5 * ux[I_UX(i, j, k)];
-    const double ux_u = +0.5 * ux[I_UX(i + 1, j, k - 1)] + 0.5 * ux[I_UX(i + 1, j, k)];
+    const double ux_l = +0.5 * s.ulm + 0.5 * s.ulc;
+    const double ux_u = +0.5 * s.uum + 0.5 * s.uuc;
     const double l = hxf * ux_l;
     const double u = -hxf * ux_u;
     const double c = -l - u;
-    sa += +l * w_xm + c * w_c + u * w_xp;
+    sa += +l * s.xm + c * s.c + u * s.xp;
   }
   {
-    const double uy_l = +0.5 * uy[I_C1(i, j, k - 1)] + 0.5 * uy[I_C1(i, j, k)];
-    const double uy_u = +0.5 * uy[I_C1(i, j + 1, k - 1)] + 0.5 * uy[I_C1(i, j + 1, k)];
+    const double uy_l = +0.5 * s.vlm + 0.5 * s.vlc;
+    const double uy_u = +0.5 * s.vum + 0.5 * s.vuc;
     const double l = g.hdy * uy_l;
     const double u = -g.hdy * uy_u;
     const double c = -l - u;
-    sa += +l * w_ym + c * w_c + u * w_yp;
+    sa += +l * s.ym + c * s.c + u * s.yp;
   }
   {
-    const double uz_l = +0.5 * w_zm + 0.5 * w_c;
-    const double uz_u = +0.5 * w_c + 0.5 * w_zp;
+    const double uz_l = +0.5 * s.zm + 0.5 * s.c;
+    const double uz_u = +0.5 * s.c + 0.5 * s.zp;
     const double l = g.hdz * uz_l;
     const double u = -g.hdz * uz_u;
     const double c = -l - u;
-    sa += +l * w_zm + c * w_c + u * w_zp;
+    sa += +l * s.zm + c * s.c + u * s.zp;
   }
   {
     const double * L = a.lapx + 3 * (i - 1);
-    sg += a.dif * (L[0] * w_xm + L[1] * w_c + L[2] * w_xp);
+    sg += a.dif * (L[0] * s.xm + L[1] * s.c + L[2] * s.xp);
   }
   {
-    sa += a.dif * (g.lyy * w_ym + g.cyy * w_c + g.lyy * w_yp);
+    sa += a.dif * (g.lyy * s.ym + g.cyy * s.c + g.lyy * s.yp);
   }
   {
-    sa += a.dif * (g.lzz * w_zm + g.czz * w_c + g.lzz * w_zp);
+    sa += a.dif * (g.lzz * s.zm + g.czz * s.c + g.lzz * s.zp);
   }
-  sg -= g.rdz * (-a.p[I_C1(i, j, k - 1)] + a.p[I_C1(i, j, k)]);
+  sg -= g.rdz * (-s.pm + s.pc);
   sa += ifrc;
   sa_out = sa;
   sg_out = sg;
+}
+
+__device__ inline void rhs_uz_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                                 double ifrc, double & sa_out, double & sg_out) {
+  const int nx = g.nx, ny = g.ny;
+  (void)nx; (void)ny;
+  UzSt s;
+  s.c = a.uz[I_C1(i, j, k)];
+  s.xm = a.uz[I_C1(i - 1, j, k)];
+  s.xp = a.uz[I_C1(i + 1, j, k)];
+  s.ym = a.uz[I_C1(i, j - 1, k)];
+  s.yp = a.uz[I_C1(i, j + 1, k)];
+  s.zm = a.uz[I_C1(i, j, k - 1)];
+  s.zp = a.uz[I_C1(i, j, k + 1)];
+  s.ulm = a.ux[I_UX(i, j, k - 1)];
+  s.ulc = a.ux[I_UX(i, j, k)];
+  s.uum = a.ux[I_UX(i + 1, j, k - 1)];
+  s.uuc = a.ux[I_UX(i + 1, j, k)];
+  s.vlm = a.uy[I_C1(i, j, k - 1)];
+  s.vlc = a.uy[I_C1(i, j, k)];
+  s.vum = a.uy[I_C1(i, j + 1, k - 1)];
+  s.vuc = a.uy[I_C1(i, j + 1, k)];
+  s.pm = a.p[I_C1(i, j, k - 1)];
+  s.pc = a.p[I_C1(i, j, k)];
+  rhs_uz_v(g, a, i, s, ifrc, sa_out, sg_out);
 }
 
 __device__ __forceinline__ void k_rhs_uz_body(const VBlock & vb, Grid g, RhsArgs a) {
@@ -850,56 +938,79 @@ __global__ void k_rhs_uz(Grid g, RhsArgs a) {
 }
 #endif
 
-// compute_rhs_t (t.c:68-182), DOFs i = 1..nx
-__device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
-                               double ifrc, double & sa_out, double & sg_out) {
-  const int nx = g.nx, ny = g.ny;
-  (void)nx; (void)ny;
-  const double * ux = a.ux;
-  const double * uy = a.uy;
-  const double * t = a.t;
+// Stencil values of one temperature DOF (i, j, k): t and its six
+// neighbours, ux at (i, j, k), (i+1, j, k), uy at (i, j, k), (i, j+1, k), uz
+// at (i, j, k), (i, j, k+1).
+struct TSt {
+  double c, xm, xp, ym, yp, zm, zp;
+  double ul, uu, vl, vu, wl, wu;
+};
+
+// compute_rhs_t (t.c:68-182), DOFs i = 1..nx, from the stencil values
+__device__ inline void rhs_t_v(const Grid & g, const RhsArgs & a, int i, const TSt & s,
+                               double & sa_out, double & sg_out) {
   const double hxf = g.xt[kXT * i + XT_HXF];   // 0.5 / DXF(i)
-  const double t_c = t[I_C1(i, j, k)];
-  const double t_xm = t[I_C1(i - 1, j, k)], t_xp = t[I_C1(i + 1, j, k)];
-  const double t_ym = t[I_C1(i, j - 1, k)], t_yp = t[I_C1(i, j + 1, k)];
   double sa = 0., sg = 0.;
   {
-    const double l = hxf * ux[I_UX(i, j, k)];
-    const double u = -hxf * ux[I_UX(i + 1, j, k)];
+    const double l = hxf * s.ul;
+    const double u = -hxf * s.uu;
     const double c = -l - u;
-    sa += +l * t_xm + c * t_c + u * t_xp;
+    sa += +l * s.xm + c * s.c + u * s.xp;
   }
   {
-    const double l = g.hdy * uy[I_C1(i, j, k)];
-    const double u = -g.hdy * uy[I_C1(i, j + 1, k)];
+    const double l = g.hdy * s.vl;
+    const double u = -g.hdy * s.vu;
     const double c = -l - u;
-    sa += +l * t_ym + c * t_c + u * t_yp;
+    sa += +l * s.ym + c * s.c + u * s.yp;
   }
 #if NDIMS == 3
-  const double t_zm = t[I_C1(i, j, k - 1)], t_zp = t[I_C1(i, j, k + 1)];
   {
-    const double * uz = a.uz;
-    const double l = g.hdz * uz[I_C1(i, j, k)];
-    const double u = -g.hdz * uz[I_C1(i, j, k + 1)];
+    const double l = g.hdz * s.wl;
+    const double u = -g.hdz * s.wu;
     const double c = -l - u;
-    sa += +l * t_zm + c * t_c + u * t_zp;
+    sa += +l * s.zm + c * s.c + u * s.zp;
   }
 #endif
   {
     const double * L = a.lapx + 3 * (i - 1);
-    sg += a.dif * (L[0] * t_xm + L[1] * t_c + L[2] * t_xp);
+    sg += a.dif * (L[0] * s.xm + L[1] * s.c + L[2] * s.xp);
   }
   {
-    sa += a.dif * (g.lyy * t_ym + g.cyy * t_c + g.lyy * t_yp);
+    sa += a.dif * (g.lyy * s.ym + g.cyy * s.c + g.lyy * s.yp);
   }
 #if NDIMS == 3
   {
-    sa += a.dif * (g.lzz * t_zm + g.czz * t_c + g.lzz * t_zp);
+    sa += a.dif * (g.lzz * s.zm + g.czz * s.c + g.lzz * s.zp);
   }
 #endif
-  (void)ifrc;
   sa_out = sa;
   sg_out = sg;
+}
+
+__device__ inline void rhs_t_at(const Grid & g, const RhsArgs & a, int i, int j, int k,
+                                double ifrc, double & sa_out, double & sg_out) {
+  const int nx = g.nx, ny = g.ny;
+  (void)nx; (void)ny;
+  (void)ifrc;
+  TSt s;
+  s.c = a.t[I_C1(i, j, k)];
+  s.xm = a.t[I_C1(i - 1, j, k)];
+  s.xp = a.t[I_C1(i + 1, j, k)];
+  s.ym = a.t[I_C1(i, j - 1, k)];
+  s.yp = a.t[I_C1(i, j + 1, k)];
+  s.ul = a.ux[I_UX(i, j, k)];
+  s.uu = a.ux[I_UX(i + 1, j, k)];
+  s.vl = a.uy[I_C1(i, j, k)];
+  s.vu = a.uy[I_C1(i, j + 1, k)];
+#if NDIMS == 3
+  s.zm = a.t[I_C1(i, j, k - 1)];
+  s.zp = a.t[I_C1(i, j, k + 1)];
+  s.wl = a.uz[I_C1(i, j, k)];
+  s.wu = a.uz[I_C1(i, j, k + 1)];
+#else
+  s.zm = s.zp = s.wl = s.wu = 0.;
+#endif
+  rhs_t_v(g, a, i, s, sa_out, sg_out);
 }
 
 __device__ __forceinline__ void k_rhs_t_body(const VBlock & vb, Grid g, RhsArgs a) {
@@ -2555,6 +2666,174 @@ __device__ __forceinline__ void k_rhs_fused_body(const VBlock & vb, Grid g, Fuse
 
 __global__ __launch_bounds__(256) void k_rhs_fused(Grid g, FusedRhs f) {
   k_rhs_fused_body(hw_block(), g, f);
+}
+
+// k_rhs_fused with each thread marching R consecutive rows j of one (i, k)
+// column: the rows j-1, j, j+1 of the stencils' centre columns (and the
+// other columns one row apart) stay in registers from one DOF to the next,
+// so each DOF loads one new row per column (35 loads instead of ~70) and
+// the next row's loads are independent of the current row's arithmetic.
+// Values and arithmetic are those of k_rhs_fused (the same rhs_*_v).
+template <int R>
+__device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, FusedRhs f) {
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 3
+  const Tile3 tl = g.bands ? band_tile(vb) : plain_tile(vb);
+  const int k = (int)tl.z + 1;
+#else
+  const Tile3 tl = plain_tile(vb);
+  const int k = 1;
+#endif
+  const int i = (int)(tl.x * blockDim.x + threadIdx.x) + 1;
+  const int jb = (int)(tl.y * blockDim.y + threadIdx.y) * R + 1;
+  if (i > nx || jb > ny) return;
+  const int nr = min(R, ny + 1 - jb);
+  const ptrdiff_t sx = nx + 1, sc = nx + 2;   // row strides of ux and cell arrays
+#if NDIMS == 3
+  const ptrdiff_t pc = sc * (ny + 2), pv = sc * (ny + 4);
+#endif
+  const RhsArgs & b = f.base;
+  const bool has_ux = i >= 2;
+  // column pointers at row j (advanced by one row per DOF)
+  const double * ux = b.ux + I_UX(i, jb, k);
+  const double * uy = b.uy + I_C1(i, jb, k);
+  const double * t = b.t + I_C1(i, jb, k);
+  const double * pp = b.p + I_C1(i, jb, k);
+  const double * vf = f.vof + I_VOF(i, jb, k);
+  const double * cu = f.curv + I_C1(i, jb, k);
+#if NDIMS == 3
+  const ptrdiff_t px = sx * (ny + 2);
+  const double * uz = b.uz + I_C1(i, jb, k);
+#endif
+  // rows carried from the previous DOF: centre columns at j-1 and j, and the
+  // off-centre columns read at two rows
+  double a0m = ux[-sx], a0c = ux[0], axpm = ux[1 - sx];
+  double b0m = uy[-sc], b0c = uy[0], bxmc = uy[-1];
+  double t0m = t[-sc], t0c = t[0];
+  double p0m = pp[-sc], v0m = vf[-sc], k0m = cu[-sc];
+#if NDIMS == 3
+  double bzmc = uy[-pc];
+  double c0m = uz[-sc], c0c = uz[0], czpm = uz[pc - sc];
+#endif
+  size_t is = I_SC(i, jb, k);
+  size_t ix = has_ux ? I_SX(i, jb, k) : 0;
+  for (int r = 0; r < nr; r++) {
+    // the new row of every column
+    const double a0p = ux[sx], axpc = ux[1];
+    const double axm = has_ux ? ux[-1] : 0.;
+    const double b0p = uy[sc], bxmp = uy[sc - 1], bxp = uy[1];
+    const double t0p = t[sc], txm = t[-1], txp = t[1];
+    const double p0c = pp[0], pxm = pp[-1];
+    const double v = vf[0], vxm = vf[-1];
+    const double cv = cu[0], kxm = cu[-1];
+#if NDIMS == 3
+    const double azm = ux[-px], azp = ux[px], axpzm = ux[1 - px];
+    const double bzmp = uy[sc - pc], bzp = uy[pc];
+    const double c0p = uz[sc], cxm = uz[-1], cxp = uz[1], czpc = uz[pc], czm = uz[-pc], cxmzp = uz[pc - 1];
+    const double tzm = t[-pc], tzp = t[pc];
+    const double pzm = pp[-pc], vzm = vf[-pv], kzm = cu[-pc];
+#endif
+    const double sb1 = f.srcb[1][is], sb3 = f.srcb[3][is];
+    const double sb0 = has_ux ? f.srcb[0][ix] : 0.;
+#if NDIMS == 3
+    const double sb2 = f.srcb[2][is];
+#endif
+    RhsArgs a = b;
+    double sa0 = 0., sg0 = 0., sa1, sg1, sa3, sg3;
+    if (has_ux) {
+      UxSt s;
+      s.c = a0c; s.xm = axm; s.xp = axpc; s.ym = a0m; s.yp = a0p;
+      s.vlm = bxmc; s.vlc = b0c; s.vum = bxmp; s.vuc = b0p;
+#if NDIMS == 3
+      s.zm = azm; s.zp = azp;
+      s.wlm = cxm; s.wlc = c0c; s.wum = cxmzp; s.wuc = czpc;
+#else
+      s.zm = s.zp = s.wlm = s.wlc = s.wum = s.wuc = 0.;
+#endif
+      s.pm = pxm; s.pc = p0c; s.tm = txm; s.tc = t0c;
+      const double grad = g.xt[kXT * i + XT_RXC] * (-vxm + v);   // 1/DXC(i) * ...
+      const double kappa = 0.5 * (kxm + cv);
+      a.lapx = f.lapx[0];
+      a.dif = f.dif[0];
+      rhs_ux_v(g, a, i, s, f.tension * grad * kappa, sa0, sg0);
+    }
+    {
+      UySt s;
+      s.c = b0c; s.xm = bxmc; s.xp = bxp; s.ym = b0m; s.yp = b0p;
+      s.ulm = a0m; s.ulc = a0c; s.uum = axpm; s.uuc = axpc;
+#if NDIMS == 3
+      s.zm = bzmc; s.zp = bzp;
+      s.wlm = c0m; s.wlc = c0c; s.wum = czpm; s.wuc = czpc;
+#else
+      s.zm = s.zp = s.wlm = s.wlc = s.wum = s.wuc = 0.;
+#endif
+      s.pm = p0m; s.pc = p0c;
+      const double grad = g.rdy * (-v0m + v);
+      const double kappa = 0.5 * (k0m + cv);
+      a.lapx = f.lapx[1];
+      a.dif = f.dif[1];
+      rhs_uy_v(g, a, i, s, f.tension * grad * kappa, sa1, sg1);
+    }
+#if NDIMS == 3
+    double sa2, sg2;
+    {
+      UzSt s;
+      s.c = c0c; s.xm = cxm; s.xp = cxp; s.ym = c0m; s.yp = c0p; s.zm = czm; s.zp = czpc;
+      s.ulm = azm; s.ulc = a0c; s.uum = axpzm; s.uuc = axpc;
+      s.vlm = bzmc; s.vlc = b0c; s.vum = bzmp; s.vuc = b0p;
+      s.pm = pzm; s.pc = p0c;
+      const double grad = g.rdz * (-vzm + v);
+      const double kappa = 0.5 * (kzm + cv);
+      a.lapx = f.lapx[2];
+      a.dif = f.dif[2];
+      rhs_uz_v(g, a, i, s, f.tension * grad * kappa, sa2, sg2);
+    }
+#endif
+    {
+      TSt s;
+      s.c = t0c; s.xm = txm; s.xp = txp; s.ym = t0m; s.yp = t0p;
+      s.ul = a0c; s.uu = axpc; s.vl = b0c; s.vu = b0p;
+#if NDIMS == 3
+      s.zm = tzm; s.zp = tzp; s.wl = c0c; s.wu = czpc;
+#else
+      s.zm = s.zp = s.wl = s.wu = 0.;
+#endif
+      a.lapx = f.lapx[3];
+      a.dif = f.dif[3];
+      rhs_t_v(g, a, i, s, sa3, sg3);
+    }
+    const double ca = b.ca, cb = b.cb, cg = b.cg;
+    if (has_ux) {
+      f.srca[0][ix] = sa0;
+      f.delta[0][ix] = +ca * sa0 + cb * sb0 + cg * sg0;
+    }
+    f.srca[1][is] = sa1;
+    f.delta[1][is] = +ca * sa1 + cb * sb1 + cg * sg1;
+#if NDIMS == 3
+    f.srca[2][is] = sa2;
+    f.delta[2][is] = +ca * sa2 + cb * sb2 + cg * sg2;
+#endif
+    f.srca[3][is] = sa3;
+    f.delta[3][is] = +ca * sa3 + cb * sb3 + cg * sg3;
+    // shift the carried rows and advance one row
+    a0m = a0c; a0c = a0p; axpm = axpc;
+    b0m = b0c; b0c = b0p; bxmc = bxmp;
+    t0m = t0c; t0c = t0p;
+    p0m = p0c; v0m = v; k0m = cv;
+    ux += sx; uy += sc; t += sc; pp += sc; vf += sc; cu += sc;
+    is += nx;
+    ix += nx - 1;
+#if NDIMS == 3
+    bzmc = bzmp;
+    c0m = c0c; c0c = c0p; czpm = czpc;
+    uz += sc;
+#endif
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rhs_march(Grid g, FusedRhs f) {
+  k_rhs_march_body<R>(hw_block(), g, f);
 }
 
 __device__ inline double prefactor_div(double a, double b) { return a / b; }

@@ -221,6 +221,7 @@ struct vof_ctx {
   int fuse;                              // F_* stages fused in schedule 1
   int fft_elems;                         // LDS complex elements per FFT buffer (1024 / 2048)
   int lines_lpb;                         // lines per work-group of the x solver (0 = auto)
+  int rhs_rows;                          // rows marched per thread by the source kernel (1 = one DOF per thread)
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
   Grid grid;
@@ -823,6 +824,7 @@ static int ctx_create(domain_t * domain) {
   ctx->poisson_refine = -1;
   ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
+  ctx->rhs_rows = 8;
   ctx->d_vof_spare = nullptr;
   ctx->vof_bytes = 0;
   ctx->prof_on = false;
@@ -1393,8 +1395,28 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
       fr.srcb[kind] = (const double *)jobs[f].src[1].data;
       fr.delta[kind] = ctx->d_delta + (size_t)f * ncell;
     }
-    KLB(ctx, "k_rhs_fused", 256, vofk::k_rhs_fused_body, vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
-        ctx->grid, fr);
+    const int rows = ctx->rhs_rows;
+    switch (rows) {
+      case 2:
+        KLB(ctx, "k_rhs_march", 256, vofk::k_rhs_march_body<2>, vofk::k_rhs_march<2>, grid3(nx, (ny + 1) / 2, nz, kBlk),
+            kBlk, 0, ctx->grid, fr);
+        break;
+      case 4:
+        KLB(ctx, "k_rhs_march", 256, vofk::k_rhs_march_body<4>, vofk::k_rhs_march<4>, grid3(nx, (ny + 3) / 4, nz, kBlk),
+            kBlk, 0, ctx->grid, fr);
+        break;
+      case 8:
+        KLB(ctx, "k_rhs_march", 256, vofk::k_rhs_march_body<8>, vofk::k_rhs_march<8>, grid3(nx, (ny + 7) / 8, nz, kBlk),
+            kBlk, 0, ctx->grid, fr);
+        break;
+      case 16:
+        KLB(ctx, "k_rhs_march", 256, vofk::k_rhs_march_body<16>, vofk::k_rhs_march<16>, grid3(nx, (ny + 15) / 16, nz, kBlk),
+            kBlk, 0, ctx->grid, fr);
+        break;
+      default:
+        KLB(ctx, "k_rhs_fused", 256, vofk::k_rhs_fused_body, vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
+            ctx->grid, fr);
+    }
   }
   for (int f = 0; f < nj; f++) {
     const int kind = jobs[f].kind;
@@ -1846,8 +1868,11 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
   return 0;
 }
 
-// tile parameters: "fft_elems" (1024 or 2048 complex per LDS buffer) and
-// "lines_lpb" (lines per work-group of the x solver, 0 = automatic)
+// tile parameters: "fft_elems" (1024 or 2048 complex per LDS buffer),
+// "lines_lpb" (lines per work-group of the x solver, 0 = automatic),
+// "xcd_bands" (band tile order of the 3D stencil kernels) and "rhs_rows"
+// (rows of one column marched per thread by the source kernel: 1, 2, 4, 8,
+// 16)
 extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   CTX_OR_FAIL(domain);
   if (!strcmp(name, "fft_elems")) {
@@ -1868,6 +1893,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   if (!strcmp(name, "lines_lpb")) {
     if (value < 0 || value > 64) return fail("lines_lpb must be in 0..64");
     ctx->lines_lpb = value;
+    return 0;
+  }
+  if (!strcmp(name, "rhs_rows")) {
+    if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return fail("rhs_rows must be 1, 2, 4, 8 or 16");
+    ctx->rhs_rows = value;
     return 0;
   }
   return fail("vof_set_tuning: unknown parameter %s", name);
