@@ -454,88 +454,66 @@ __device__ inline double indicator(const double * n, const double * x) {
 #endif
 }
 
-// compute_flux_x (flxx.c:11-44): THINC flux through the x face i (2..nx)
+// THINC flux through a face normal to direction D (0 x, 1 y, 2 z) with face
+// velocity vel between the cells m (lower index) and p: the upwind cell's
+// vof, or where it holds an interface its indicator integrated over the face
+// with the 2-point Gauss rule (compute_flux_x flxx.c:11-44, compute_flux_y
+// flxy.c:11-44 and the z counterpart; the same quadrature order).  Both
+// cells' vof are loaded by the caller, so the upwind choice does not wait on
+// the velocity; only the normal is read through the choice.
+template <int D>
+__device__ inline double face_flux(double vel, double vm, double vp, const double * nm, const double * np) {
+  const double lvof = vel < 0. ? vp : vm;
+  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
+  const double * n = vel < 0. ? np : nm;
+  const double c = vel < 0. ? -0.5 : +0.5;
+  double flux = 0.;
+#if NDIMS == 2
+  for (int q = 0; q < NGAUSS; q++) {
+    const double pt[2] = {D == 0 ? c : kGaussP[q], D == 0 ? kGaussP[q] : c};
+    flux += kGaussW[q] * indicator(n, pt);
+  }
+#else
+  for (int q = 0; q < NGAUSS; q++) {
+    double sr = 0.;
+    for (int r = 0; r < NGAUSS; r++) {
+      // x faces: (c, P[q], P[r]); y faces: (P[q], c, P[r]); z faces: (P[q], P[r], c)
+      const double pt[3] = {D == 0 ? c : kGaussP[q], D == 0 ? kGaussP[q] : D == 1 ? c : kGaussP[r],
+                            D == 2 ? c : kGaussP[r]};
+      sr += kGaussW[r] * indicator(n, pt);
+    }
+    flux += kGaussW[q] * sr;
+  }
+#endif
+  return vel * flux;
+}
+
+// x face i (2..nx)
 __device__ inline double flux_x_at(const double * __restrict__ ux, const double * __restrict__ vof,
                                    const double * __restrict__ normal, int nx, int ny, int i, int j,
                                    int k) {
   constexpr int NC = NDIMS + 1;
-  const double vel = ux[I_UX(i, j, k)];
-  const int ii = vel < 0. ? i : i - 1;
-  const double x = vel < 0. ? -0.5 : +0.5;
-  const double lvof = vof[I_VOF(ii, j, k)];
-  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
-  const double * n = normal + NC * I_NRM(ii, j, k);
-  double flux = 0.;
-#if NDIMS == 2
-  for (int jj = 0; jj < NGAUSS; jj++) {
-    const double pt[2] = {x, kGaussP[jj]};
-    flux += kGaussW[jj] * indicator(n, pt);
-  }
-#else
-  for (int jj = 0; jj < NGAUSS; jj++) {
-    double sz = 0.;
-    for (int kk = 0; kk < NGAUSS; kk++) {
-      const double pt[3] = {x, kGaussP[jj], kGaussP[kk]};
-      sz += kGaussW[kk] * indicator(n, pt);
-    }
-    flux += kGaussW[jj] * sz;
-  }
-#endif
-  return vel * flux;
+  return face_flux<0>(ux[I_UX(i, j, k)], vof[I_VOF(i - 1, j, k)], vof[I_VOF(i, j, k)],
+                      normal + NC * I_NRM(i - 1, j, k), normal + NC * I_NRM(i, j, k));
 }
 
-// compute_flux_y (flxy.c:11-44): face j (1..ny+1)
+// y face j (1..ny+1)
 __device__ inline double flux_y_at(const double * __restrict__ uy, const double * __restrict__ vof,
                                    const double * __restrict__ normal, int nx, int ny, int i, int j,
                                    int k) {
   constexpr int NC = NDIMS + 1;
-  const double vel = uy[I_C1(i, j, k)];
-  const int jj0 = vel < 0. ? j : j - 1;
-  const double y = vel < 0. ? -0.5 : +0.5;
-  const double lvof = vof[I_VOF(i, jj0, k)];
-  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
-  const double * n = normal + NC * I_NRM(i, jj0, k);
-  double flux = 0.;
-#if NDIMS == 2
-  for (int ii = 0; ii < NGAUSS; ii++) {
-    const double pt[2] = {kGaussP[ii], y};
-    flux += kGaussW[ii] * indicator(n, pt);
-  }
-#else
-  for (int ii = 0; ii < NGAUSS; ii++) {
-    double sz = 0.;
-    for (int kk = 0; kk < NGAUSS; kk++) {
-      const double pt[3] = {kGaussP[ii], y, kGaussP[kk]};
-      sz += kGaussW[kk] * indicator(n, pt);
-    }
-    flux += kGaussW[ii] * sz;
-  }
-#endif
-  return vel * flux;
+  return face_flux<1>(uy[I_C1(i, j, k)], vof[I_VOF(i, j - 1, k)], vof[I_VOF(i, j, k)],
+                      normal + NC * I_NRM(i, j - 1, k), normal + NC * I_NRM(i, j, k));
 }
 
 #if NDIMS == 3
-// the z counterpart: face k (1..nz+1)
+// z face k (1..nz+1)
 __device__ inline double flux_z_at(const double * __restrict__ uz, const double * __restrict__ vof,
                                    const double * __restrict__ normal, int nx, int ny, int i, int j,
                                    int k) {
   constexpr int NC = NDIMS + 1;
-  const double vel = uz[I_C1(i, j, k)];
-  const int kk0 = vel < 0. ? k : k - 1;
-  const double z = vel < 0. ? -0.5 : +0.5;
-  const double lvof = vof[I_VOF(i, j, kk0)];
-  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
-  const double * n = normal + NC * I_NRM(i, j, kk0);
-  double flux = 0.;
-  for (int ii = 0; ii < NGAUSS; ii++) {
-    double sy = 0.;
-    for (int jj = 0; jj < NGAUSS; jj++) {
-      const double pt[3] = {kGaussP[ii], kGaussP[jj], z};
-      sy += kGaussW[jj] * indicator(n, pt);
-    }
-    flux += kGaussW[ii] * sy;
-  }
-  return vel * flux;
+  return face_flux<2>(uz[I_C1(i, j, k)], vof[I_VOF(i, j, k - 1)], vof[I_VOF(i, j, k)],
+                      normal + NC * I_NRM(i, j, k - 1), normal + NC * I_NRM(i, j, k));
 }
 #endif
 
