@@ -2414,14 +2414,14 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
     const int b = r % CY;
     const int c = r / CY;
     const int i = i0 + a;
-    const double dx = g.dxc[min(i, nx + 1) - 1];
+    const double rdx = g.xt[kXT * min(i, nx + 1) + XT_RXC];   // 1 / DXC
 #define SV(A, B, C) sv[(A) + VX * ((B) + VY * (C))]
 #if NDIMS == 2
     (void)c;
     const double va = SV(a, b, 0), vb = SV(a + 1, b, 0);
     const double vc = SV(a, b + 1, 0), vd = SV(a + 1, b + 1, 0);
-    const double gx = 1. / dx * (-va + vb - vc + vd);
-    const double gy = 1. / g.dy * (-va - vb + vc + vd);
+    const double gx = rdx * (-va + vb - vc + vd);
+    const double gy = g.rdy * (-va - vb + vc + vd);
     const double norm = sqrt(gx * gx + gy * gy);
     const double ninv = 1. / fmax(norm, DBL_EPSILON);
     sc[0][e] = gx * ninv;
@@ -2432,9 +2432,9 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
     const double sx0 = -a0 + b0 - c0 + d0, sx1 = -a1 + b1 - c1 + d1;
     const double sy0 = -a0 - b0 + c0 + d0, sy1 = -a1 - b1 + c1 + d1;
     const double sz0 = a0 + b0 + c0 + d0, sz1 = a1 + b1 + c1 + d1;
-    const double gx = 1. / dx * (sx0 + sx1);
-    const double gy = 1. / g.dy * (sy0 + sy1);
-    const double gz = 1. / g.dz * (-sz0 + sz1);
+    const double gx = rdx * (sx0 + sx1);
+    const double gy = g.rdy * (sy0 + sy1);
+    const double gz = g.rdz * (-sz0 + sz1);
     const double norm = sqrt(gx * gx + gy * gy + gz * gz);
     const double ninv = 1. / fmax(norm, DBL_EPSILON);
     sc[0][e] = gx * ninv;
@@ -2457,20 +2457,21 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
 #endif
 #define SC(D, A, B, C) sc[D][(A) + CX * ((B) + CY * (C))]
   const double dx = g.dxf[i - 1];
+  const double rdx = g.xt[kXT * i + XT_RXF];   // 1 / DXF
   double cv;
 #if NDIMS == 2
   {
-    const double dnxdx = 1. / dx * (-SC(0, a, b, 0) + SC(0, a + 1, b, 0) - SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0));
-    const double dnydy = 1. / g.dy * (-SC(1, a, b, 0) - SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0));
+    const double dnxdx = rdx * (-SC(0, a, b, 0) + SC(0, a + 1, b, 0) - SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0));
+    const double dnydy = g.rdy * (-SC(1, a, b, 0) - SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0));
     cv = 0.5 * (-dnxdx - dnydy);
   }
 #else
   {
-    const double dnxdx = 1. / dx * ((-SC(0, a, b, c) + SC(0, a + 1, b, c) - SC(0, a, b + 1, c) + SC(0, a + 1, b + 1, c))
+    const double dnxdx = rdx * ((-SC(0, a, b, c) + SC(0, a + 1, b, c) - SC(0, a, b + 1, c) + SC(0, a + 1, b + 1, c))
                                     + (-SC(0, a, b, c + 1) + SC(0, a + 1, b, c + 1) - SC(0, a, b + 1, c + 1) + SC(0, a + 1, b + 1, c + 1)));
-    const double dnydy = 1. / g.dy * ((-SC(1, a, b, c) - SC(1, a + 1, b, c) + SC(1, a, b + 1, c) + SC(1, a + 1, b + 1, c))
+    const double dnydy = g.rdy * ((-SC(1, a, b, c) - SC(1, a + 1, b, c) + SC(1, a, b + 1, c) + SC(1, a + 1, b + 1, c))
                                       + (-SC(1, a, b, c + 1) - SC(1, a + 1, b, c + 1) + SC(1, a, b + 1, c + 1) + SC(1, a + 1, b + 1, c + 1)));
-    const double dnzdz = 1. / g.dz * (-(SC(2, a, b, c) + SC(2, a + 1, b, c) + SC(2, a, b + 1, c) + SC(2, a + 1, b + 1, c))
+    const double dnzdz = g.rdz * (-(SC(2, a, b, c) + SC(2, a + 1, b, c) + SC(2, a, b + 1, c) + SC(2, a + 1, b + 1, c))
                                       + (SC(2, a, b, c + 1) + SC(2, a + 1, b, c + 1) + SC(2, a, b + 1, c + 1) + SC(2, a + 1, b + 1, c + 1)));
     cv = 0.25 * (-dnxdx - dnydy - dnzdz);
   }
@@ -2546,12 +2547,12 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
   const double fym = flux_y_at(uy, vof, normal, nx, ny, i, j, k);
   const double fyp = flux_y_at(uy, vof, normal, nx, ny, i, j + 1, k);
   double s = 0.;
-  s += 1. / g.dxf[i - 1] * (fxm - fxp);
-  s += 1. / g.dy * (fym - fyp);
+  s += g.xt[kXT * i + XT_RXF] * (fxm - fxp);   // 1 / DXF(i)
+  s += g.rdy * (fym - fyp);
 #if NDIMS == 3
   const double fzm = flux_z_at(uz, vof, normal, nx, ny, i, j, k);
   const double fzp = flux_z_at(uz, vof, normal, nx, ny, i, j, k + 1);
-  s += 1. / g.dz * (fzm - fzp);
+  s += g.rdz * (fzm - fzp);
 #endif
   const size_t is = I_SC(i, j, k);
   srca[is] = s;
