@@ -1494,6 +1494,7 @@ __device__ T2 * lds_fft(T2 * a, T2 * b, int ncols, const FftPlan & P, const T2 *
 }
 
 constexpr int kFftThreads = 256;
+constexpr int kFftLoadU = 4;      // elements per thread whose global loads are batched
 constexpr int kFftElems = 2048;   // complex elements per LDS buffer
 
 // assign_input (dft.c:315-345) fused with the real transform in y.  Two x
@@ -1518,23 +1519,44 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
   const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
-  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
-    const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
-    const int i = x0 + xl + 1;
-    double r = 0.;
-    if (i <= nx) {
-      const double dx = g.dxf[i - 1];
-      const double ux_xm = ux[I_UX(i, j, k)], ux_xp = ux[I_UX(i + 1, j, k)];
-      const double uy_ym = uy[I_C1(i, j, k)], uy_yp = uy[I_C1(i, j + 1, k)];
-#if NDIMS == 2
-      r = prefactor * ((ux_xp - ux_xm) / dx + (uy_yp - uy_ym) / g.dy);
-#else
-      const double uz_zm = uz[I_C1(i, j, k)], uz_zp = uz[I_C1(i, j, k + 1)];
-      r = prefactor * ((ux_xp - ux_xm) / dx + (uy_yp - uy_ym) / g.dy + (uz_zp - uz_zm) / g.dz);
+  // kFftLoadU elements per thread have their loads in flight together
+  const int total = 2 * NC * N;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    double xm[kFftLoadU], xp[kFftLoadU], ym[kFftLoadU], yp[kFftLoadU], zm[kFftLoadU], zp[kFftLoadU];
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+      const int i = x0 + xl + 1;
+      if (e < total && i <= nx) {
+        xm[u] = ux[I_UX(i, j, k)];
+        xp[u] = ux[I_UX(i + 1, j, k)];
+        ym[u] = uy[I_C1(i, j, k)];
+        yp[u] = uy[I_C1(i, j + 1, k)];
+#if NDIMS == 3
+        zm[u] = uz[I_C1(i, j, k)];
+        zp[u] = uz[I_C1(i, j, k + 1)];
 #endif
+      }
     }
-    double * slot = reinterpret_cast<double *>(&sa[(xl >> 1) * N + (j - 1)]);
-    slot[xl & 1] = r;
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e >= total) break;
+      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+      const int i = x0 + xl + 1;
+      double r = 0.;
+      if (i <= nx) {
+        const double dx = g.dxf[i - 1];
+#if NDIMS == 2
+        r = prefactor * ((xp[u] - xm[u]) / dx + (yp[u] - ym[u]) / g.dy);
+#else
+        r = prefactor * ((xp[u] - xm[u]) / dx + (yp[u] - ym[u]) / g.dy + (zp[u] - zm[u]) / g.dz);
+#endif
+      }
+      double * slot = reinterpret_cast<double *>(&sa[(xl >> 1) * N + (j - 1)]);
+      slot[xl & 1] = r;
+    }
   }
   __syncthreads();
   double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, -1.);
@@ -1572,10 +1594,21 @@ __device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, FftPlan 
   const int ky0 = tile.x * NC;
   const int x = tile.y;
   T2 * base = spec + (size_t)x * N * nh;
-  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
-    const int c = e % NC, kz = e / NC;
-    const int ky = ky0 + c;
-    sa[c * N + kz] = ky < nh ? base[(size_t)kz * nh + ky] : Cx<T2>::mk(0, 0);
+  const int total = NC * N;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    T2 v[kFftLoadU];
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      const int c = e % NC, kz = e / NC;
+      const int ky = ky0 + c;
+      v[u] = e < total && ky < nh ? base[(size_t)kz * nh + ky] : Cx<T2>::mk(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e < total) sa[(e % NC) * N + e / NC] = v[u];
+    }
   }
   __syncthreads();
   T2 * res = lds_fft<T2>(sa, sb, NC, P, tw, sign);
@@ -1761,19 +1794,34 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
   const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
-  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
-    const int c = e / N, ky = e % N;
-    const int xa = x0 + 2 * c, xb = xa + 1;
-    const int kk = ky <= N / 2 ? ky : N - ky;
-    const double cs = ky <= N / 2 ? 1. : -1.;   // conjugate for the upper half
-    double2 A = make_double2(0., 0.), B = make_double2(0., 0.);
-    if (xa < nx) A = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
-    if (xb < nx) B = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
-    if (kk == 0 || 2 * kk == N) { A.y = 0.; B.y = 0.; }
-    A.y *= cs;
-    B.y *= cs;
-    // Z = A + i B
-    sa[c * N + ky] = make_double2(A.x - B.y, A.y + B.x);
+  const int total = NC * N;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    double2 As[kFftLoadU], Bs[kFftLoadU];
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      const int c = e / N, ky = e % N;
+      const int xa = x0 + 2 * c, xb = xa + 1;
+      const int kk = ky <= N / 2 ? ky : N - ky;
+      As[u] = make_double2(0., 0.);
+      Bs[u] = make_double2(0., 0.);
+      if (e < total && xa < nx) As[u] = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
+      if (e < total && xb < nx) Bs[u] = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
+    }
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e >= total) break;
+      const int c = e / N, ky = e % N;
+      const int kk = ky <= N / 2 ? ky : N - ky;
+      const double cs = ky <= N / 2 ? 1. : -1.;   // conjugate for the upper half
+      double2 A = As[u], B = Bs[u];
+      if (kk == 0 || 2 * kk == N) { A.y = 0.; B.y = 0.; }
+      A.y *= cs;
+      B.y *= cs;
+      // Z = A + i B
+      sa[c * N + ky] = make_double2(A.x - B.y, A.y + B.x);
+    }
   }
   __syncthreads();
   double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, +1.);

@@ -1453,8 +1453,11 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     F.i0 = 1;
   }
   {
-    // lines per work-group so that the LDS tile stays within 32 KiB
-    int lpb = ctx->lines_lpb > 0 ? ctx->lines_lpb : (int)(32768 / (sizeof(double) * (nx | 1)));
+    // lines per work-group: an LDS tile of about 16 KiB (8 lines at nx = 256,
+    // the measured optimum between more resident groups and longer serial
+    // sweeps per group)
+    const int line_bytes = (int)(sizeof(double) * (nx | 1));
+    int lpb = ctx->lines_lpb > 0 ? ctx->lines_lpb : (16384 + line_bytes - 1) / line_bytes;
     if (lpb > 64) lpb = 64;
     if (lpb < 1) lpb = 1;
     if (sizeof(double) * (nx | 1) > 65536) return fail("x solver: nx = %d exceeds the LDS line tile", nx);
