@@ -1495,20 +1495,125 @@ __device__ T2 * lds_fft(T2 * a, T2 * b, int ncols, const FftPlan & P, const T2 *
 
 constexpr int kFftThreads = 256;
 constexpr int kFftLoadU = 4;      // elements per thread whose global loads are batched
+
+// 16-point DFT in registers, radix 4 x 4, in place: X[k] = sum_n v[n] w^(n k),
+// w = exp(sign 2 pi i / 16), n = a + 4 b, k = kb + 4 ka; X[k] is left in
+// v[dft16_slot(k)] = v[4 kb + ka]
+__device__ inline int dft16_slot(int k) { return 4 * (k & 3) + (k >> 2); }
+__device__ __forceinline__ void dft16(double2 * v, double sign) {
+  // cos, sin of 2 pi m / 16, m = 0..15
+  constexpr double kC[16] = {1., 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                             0., -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                             -1., -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                             0., 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+  constexpr double kS[16] = {0., 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                             1., 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                             0., -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                             -1., -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+  // DFT4 over b of v[a + 4 b], twiddled by w^(a kb), back into v[a + 4 kb]
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double2 u[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+    butterfly<4, double2>(u, sign);
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const int m = a * kb;
+      if (m == 0) {
+        v[a + 4 * kb] = u[kb];
+      } else {
+        const double wr = kC[m], wi = sign * kS[m];
+        v[a + 4 * kb] = make_double2(u[kb].x * wr - u[kb].y * wi, u[kb].x * wi + u[kb].y * wr);
+      }
+    }
+  }
+  // DFT4 over a of v[a + 4 kb] -> X[kb + 4 ka], left in v[4 kb + ka]
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    double2 u[4] = {v[4 * kb], v[4 * kb + 1], v[4 * kb + 2], v[4 * kb + 3]};
+    butterfly<4, double2>(u, sign);
+#pragma unroll
+    for (int ka = 0; ka < 4; ka++) v[4 * kb + ka] = u[ka];
+  }
+}
+
+// 256-point transforms of `ncols` LDS columns in place, as two passes of
+// 16-point DFTs in registers (n = n2 + 16 n1, k = k1 + 16 k2):
+//   pass 1, thread (c, n2): Y[n2][k1] = W^(n2 k1) DFT16_n1(x[n2 + 16 n1])[k1]
+//   pass 2, thread (c, k1): X[k1 + 16 k2] = DFT16_n2(Y[n2][k1])[k2]
+// with W = exp(sign 2 pi i / 256), tw[m] = (cos, sin)(2 pi m / 256).  A
+// column is a 16 x 17 matrix (row stride 17: both passes are bank-conflict
+// free); x[n] sits at row n / 16, column n % 16 (f256_in), Y[n2][k1] at
+// (k1, n2) and X[k] at (k % 16, k / 16) (f256_out).  Each thread owns one
+// matrix column in pass 1 and one row in pass 2, so both run in place.
+constexpr int kF256Cols = 8;      // transforms per work-group
+constexpr int kF256Stride = 272;  // complex values per column
+__device__ inline int f256_in(int n) { return (n >> 4) * 17 + (n & 15); }
+__device__ inline int f256_out(int k) { return (k & 15) * 17 + (k >> 4); }
+
+__device__ __forceinline__ void fft256(double2 * buf, int ncols, const double2 * __restrict__ tw, double sign) {
+  for (int t = threadIdx.x; t < ncols * 16; t += blockDim.x) {
+    double2 * col = buf + (t >> 4) * kF256Stride;
+    const int n2 = t & 15;
+    double2 v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) v[n1] = col[n1 * 17 + n2];
+    dft16(v, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+      double2 x = v[dft16_slot(k1)];
+      if (k1 != 0 && n2 != 0) {
+        const double2 w = tw[n2 * k1];
+        const double ws = sign * w.y;
+        x = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
+      }
+      col[k1 * 17 + n2] = x;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ncols * 16; t += blockDim.x) {
+    double2 * row = buf + (t >> 4) * kF256Stride + (t & 15) * 17;
+    double2 v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; n2++) v[n2] = row[n2];
+    dft16(v, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) row[k2] = v[dft16_slot(k2)];
+  }
+  __syncthreads();
+}
+
+// column layout of the two transform engines: F256 = the 256-point register
+// path (one padded buffer), else the mixed-radix Stockham path (two buffers)
+template <bool F256> __device__ inline int fft_in(int n) { return F256 ? f256_in(n) : n; }
+template <bool F256> __device__ inline int fft_out(int k) { return F256 ? f256_out(k) : k; }
+template <bool F256>
+__device__ __forceinline__ double2 * fft_cols(double2 * a, double2 * b, int ncols, const FftPlan & P,
+                                     const double2 * __restrict__ tw, double sign) {
+  if constexpr (F256) {
+    (void)b; (void)P;
+    fft256(a, ncols, tw, sign);
+    return a;
+  } else {
+    return lds_fft<double2>(a, b, ncols, P, tw, sign);
+  }
+}
+
 constexpr int kFftElems = 2048;   // complex elements per LDS buffer
 
 // assign_input (dft.c:315-345) fused with the real transform in y.  Two x
 // columns are packed into one complex column; the r2c halves are separated
 // after the transform.  Output spec[x][k][ky].
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb, 
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
     double2 * __restrict__ spec) {
-  __shared__ double2 sa[ELEMS], sb[ELEMS];
+  __shared__ double2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ double2 sb[F256 ? 1 : ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = ELEMS / N;                // complex columns per block
+  const int NC = F256 ? kF256Cols : ELEMS / N;   // complex columns per block
+  const int CS = F256 ? kF256Stride : N;         // LDS column stride
   const Tile3 tile = xcd_tile(vb);
   const int x0 = tile.x * 2 * NC;          // first interior x (0-based)
 #if NDIMS == 2
@@ -1554,17 +1659,17 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
         r = prefactor * ((xp[u] - xm[u]) / dx + (yp[u] - ym[u]) / g.dy + (zp[u] - zm[u]) / g.dz);
 #endif
       }
-      double * slot = reinterpret_cast<double *>(&sa[(xl >> 1) * N + (j - 1)]);
+      double * slot = reinterpret_cast<double *>(&sa[(xl >> 1) * CS + fft_in<F256>(j - 1)]);
       slot[xl & 1] = r;
     }
   }
   __syncthreads();
-  double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, -1.);
+  double2 * res = fft_cols<F256>(sa, sb, NC, P, tw, -1.);
   const int nh = N / 2 + 1;
   for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
     const int c = e / nh, ky = e % nh;
-    const double2 z = res[c * N + ky];
-    const double2 zc = res[c * N + (N - ky) % N];
+    const double2 z = res[c * CS + fft_out<F256>(ky)];
+    const double2 zc = res[c * CS + fft_out<F256>((N - ky) % N)];
     // A = (Z[k] + conj Z[N-k]) / 2, B = (Z[k] - conj Z[N-k]) / (2 i)
     const double2 A = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
     const double2 B = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
@@ -1574,22 +1679,24 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
   }
 }
 
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
     double2 * __restrict__ spec) {
-  k_div_rfft_y_body<ELEMS>(hw_block(), g, P, tw, ux, uy, uz, prefactor, spec);
+  k_div_rfft_y_body<ELEMS, F256>(hw_block(), g, P, tw, ux, uy, uz, prefactor, spec);
 }
 
 #if NDIMS == 3
 // complex transform in z of spec[x][kz][ky] for a run of ky at one x
-template <int ELEMS, class T2>
+template <int ELEMS, class T2, bool F256 = false>
 __device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, FftPlan P, const T2 * __restrict__ tw,
                                                       T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
-  __shared__ T2 sa[ELEMS], sb[ELEMS];
+  __shared__ T2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ T2 sb[F256 ? 1 : ELEMS];
   const int N = g.nz;
-  const int NC = ELEMS / N;
+  const int NC = F256 ? kF256Cols : ELEMS / N;
+  const int CS = F256 ? kF256Stride : N;
   const Tile3 tile = xcd_tile(vb);
   const int ky0 = tile.x * NC;
   const int x = tile.y;
@@ -1607,22 +1714,32 @@ __device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, FftPlan 
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
-      if (e < total) sa[(e % NC) * N + e / NC] = v[u];
+      if (e < total) sa[(e % NC) * CS + fft_in<F256>(e / NC)] = v[u];
     }
   }
   __syncthreads();
-  T2 * res = lds_fft<T2>(sa, sb, NC, P, tw, sign);
+  T2 * res;
+  if constexpr (F256) {
+    static_assert(!F256 || sizeof(T2) == sizeof(double2), "256-point path is FP64");
+    // the direction as a constant, so the twiddle signs fold
+    if (sign < 0)
+      res = fft_cols<true>(sa, sb, NC, P, tw, -1.);
+    else
+      res = fft_cols<true>(sa, sb, NC, P, tw, +1.);
+  } else {
+    res = lds_fft<T2>(sa, sb, NC, P, tw, sign);
+  }
   for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
     const int c = e % NC, kz = e / NC;
     const int ky = ky0 + c;
-    if (ky < nh) base[(size_t)kz * nh + ky] = res[c * N + kz];
+    if (ky < nh) base[(size_t)kz * nh + ky] = res[c * CS + fft_out<F256>(kz)];
   }
 }
 
-template <int ELEMS, class T2>
+template <int ELEMS, class T2, bool F256 = false>
 __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const T2 * __restrict__ tw,
                                                       T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
-  k_fft_z_body<ELEMS, T2>(hw_block(), g, P, tw, spec, nh, sign);
+  k_fft_z_body<ELEMS, T2, F256>(hw_block(), g, P, tw, spec, nh, sign);
 }
 #endif
 
@@ -1776,14 +1893,16 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
 
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and
 // k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb, 
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
     double * __restrict__ psi, int images) {
-  __shared__ double2 sa[ELEMS], sb[ELEMS];
+  __shared__ double2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ double2 sb[F256 ? 1 : ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = ELEMS / N;
+  const int NC = F256 ? kF256Cols : ELEMS / N;
+  const int CS = F256 ? kF256Stride : N;
   const Tile3 tile = xcd_tile(vb);
   const int x0 = tile.x * 2 * NC;
   const int nh = N / 2 + 1;
@@ -1820,16 +1939,16 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
       A.y *= cs;
       B.y *= cs;
       // Z = A + i B
-      sa[c * N + ky] = make_double2(A.x - B.y, A.y + B.x);
+      sa[c * CS + fft_in<F256>(ky)] = make_double2(A.x - B.y, A.y + B.x);
     }
   }
   __syncthreads();
-  double2 * res = lds_fft<double2>(sa, sb, NC, P, tw, +1.);
+  double2 * res = fft_cols<F256>(sa, sb, NC, P, tw, +1.);
   for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
     const int xl = e % (2 * NC), j = e / (2 * NC);
     const int i = x0 + xl + 1;
     if (i <= nx) {
-      const double2 z = res[(xl >> 1) * N + j];
+      const double2 z = res[(xl >> 1) * CS + fft_out<F256>(j)];
       const double v = (xl & 1) ? z.y : z.x;
       if (images) {
         // fluid_update_boundaries_psi folded in: halo images, Neumann walls
@@ -1843,11 +1962,11 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
   }
 }
 
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
     Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
     double * __restrict__ psi, int images) {
-  k_irfft_y_psi_body<ELEMS>(hw_block(), g, P, tw, spec, psi, images);
+  k_irfft_y_psi_body<ELEMS, F256>(hw_block(), g, P, tw, spec, psi, images);
 }
 
 // ---------------------------------------------------------------------------

@@ -1538,54 +1538,69 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const double norm = (double)ny * (NDIMS == 3 ? (double)nz : 1.);
   const double prefactor = 1. / (kRk[rkstep][2] * dt) / norm;
   const int elems = (ctx->fft_elems == 1024 && ny <= 1024 && (NDIMS == 2 || nz <= 1024)) ? 1024 : 2048;
-  const int NC = elems / ny;
+  // 256-point transforms take the register radix-16 path (8 columns per
+  // work-group), other lengths the mixed-radix Stockham path
+  const bool f256y = ny == 256;
+  const int NC = f256y ? vofk::kF256Cols : elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
+  const dim3 gy(nbx, NDIMS == 3 ? nz : 1), bf(vofk::kFftThreads);
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
 #else
   const double * uz = nullptr;
 #endif
-  if (elems == 1024) {
-    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<1024>, vofk::k_div_rfft_y<1024>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
-        ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
-        (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
+#define DIV_ARGS ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz, \
+                 prefactor, ctx->d_spec
+  if (f256y) {
+    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, (vofk::k_div_rfft_y_body<1024, true>), (vofk::k_div_rfft_y<1024, true>), gy,
+        bf, 0, DIV_ARGS);
+  } else if (elems == 1024) {
+    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<1024>, vofk::k_div_rfft_y<1024>, gy, bf, 0, DIV_ARGS);
   } else {
-    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<2048>, vofk::k_div_rfft_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1), dim3(vofk::kFftThreads), 0,
-        ctx->grid, ctx->plan_y, ctx->d_twy, (const double *)fluid->ux.data,
-        (const double *)fluid->uy.data, uz, prefactor, ctx->d_spec);
+    KLB(ctx, "k_div_rfft_y", vofk::kFftThreads, vofk::k_div_rfft_y_body<2048>, vofk::k_div_rfft_y<2048>, gy, bf, 0, DIV_ARGS);
   }
+#undef DIV_ARGS
 #if NDIMS == 3
-  const int NCz = elems / nz;
+  const bool f256z = nz == 256;
+  const int NCz = f256z ? vofk::kF256Cols : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
-  if (elems == 1024) {
-    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2>), (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
-        ctx->d_spec, ctx->nh, -1.);
-  } else {
-    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, double2>), (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
-        ctx->d_spec, ctx->nh, -1.);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      const int nlines = ctx->nh * nz;
+      KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
+          dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+    }
+    const double sign = pass == 0 ? -1. : +1.;
+    if (f256z) {
+      KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2, true>), (vofk::k_fft_z<1024, double2, true>),
+          gz, bf, 0, ctx->grid, ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, sign);
+    } else if (elems == 1024) {
+      KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2>), (vofk::k_fft_z<1024, double2>), gz, bf, 0,
+          ctx->grid, ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, sign);
+    } else {
+      KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, double2>), (vofk::k_fft_z<2048, double2>), gz, bf, 0,
+          ctx->grid, ctx->plan_z, ctx->d_twz, ctx->d_spec, ctx->nh, sign);
+    }
+  }
+#else
+  {
+    const int nlines = ctx->nh * nz;
+    KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
+        dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
   }
 #endif
-  const int nlines = ctx->nh * nz;
-  KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines,
-      ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
-#if NDIMS == 3
-  if (elems == 1024) {
-    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, double2>), (vofk::k_fft_z<1024, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
-        ctx->d_spec, ctx->nh, +1.);
+#define IRF_ARGS ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data, images
+  if (f256y) {
+    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, (vofk::k_irfft_y_psi_body<1024, true>), (vofk::k_irfft_y_psi<1024, true>),
+        gy, bf, 0, IRF_ARGS);
+  } else if (elems == 1024) {
+    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<1024>, vofk::k_irfft_y_psi<1024>, gy, bf, 0,
+        IRF_ARGS);
   } else {
-    KLB(ctx, "k_fft_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, double2>), (vofk::k_fft_z<2048, double2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z, ctx->d_twz,
-        ctx->d_spec, ctx->nh, +1.);
+    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<2048>, vofk::k_irfft_y_psi<2048>, gy, bf, 0,
+        IRF_ARGS);
   }
-#endif
-  if (elems == 1024) {
-    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<1024>, vofk::k_irfft_y_psi<1024>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
-        images);
-  } else {
-    KLB(ctx, "k_irfft_y_psi", vofk::kFftThreads, vofk::k_irfft_y_psi_body<2048>, vofk::k_irfft_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data,
-        images);
-  }
+#undef IRF_ARGS
   return images ? 0 : bc_neumann(ctx, (double *)fluid->psi.data);
 }
 

@@ -21,6 +21,10 @@ CASES = [
     ((24, 40), False, 12),
     ((16, 12, 20), True, 13),
     ((12, 16, 8), False, 14),
+    # 256-point transforms in y, z (the register radix-16 path)
+    ((16, 256), False, 15),
+    ((8, 12, 256), True, 16),
+    ((10, 256, 6), False, 17),
 ]
 
 
