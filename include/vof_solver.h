@@ -325,6 +325,8 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
  * per work-group of the predictor's x solver (0 = automatic); "xcd_bands" =
  * 1 walks the 3D stencil tiles band by band per XCD; "rhs_rows" = 1, 2, 4,
  * 8 (default) or 16 rows of one column per thread of the source kernel;
+ * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
+ * work-group by the 3D interface kernel;
  * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e
  * (this one changes adjoint results) */
 int vof_set_tuning(domain_t * domain, const char name[], int value);

@@ -2692,6 +2692,154 @@ __global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double 
   k_iface_fused_body(hw_block(), g, vof, normal, curv);
 }
 
+// Work-group barrier that waits for LDS operations only: global loads issued
+// before it stay in flight (the __syncthreads fence would wait for them).
+// Only for LDS hand-offs; the "memory" clobber keeps the compiler from
+// moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+#if NDIMS == 3
+// k_iface_fused marching in z: a 32 x 8 tile of (x, y) walks KZ planes.
+// Vof planes live in a 3-plane LDS ring, corner normals in a 2-plane ring;
+// the vof plane two ahead is loaded into registers while the current plane
+// is computed.  For cell plane k: the registers (plane k+1) go to LDS, the
+// loads of plane k+2 are issued, the corner plane between k and k+1 is
+// formed (the one between k-1 and k is kept from the previous plane), then
+// the cells of plane k.  Same arithmetic as k_iface_fused.
+constexpr int kMX = 32, kMY = 8;
+constexpr int kMThreads = kMX * kMY;
+template <int KZ>
+__device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
+                                                   double * __restrict__ normal, double * __restrict__ curv) {
+  constexpr int VX = kMX + 2, VY = kMY + 2, VP = VX * VY;
+  constexpr int CX = kMX + 1, CY = kMY + 1, CP = CX * CY;
+  constexpr int PF = (VP + kMThreads - 1) / kMThreads;   // prefetched values per thread
+  __shared__ double sv[3][VP];
+  __shared__ double sc[2][3][CP];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int i0 = 1 + (int)vb.x * kMX, j0 = 1 + (int)vb.y * kMY, kb = 1 + (int)vb.z * KZ;
+  const int ke = min(kb + KZ - 1, nz);
+  const int tid = threadIdx.x;
+  double pf[PF];
+  // vof plane kk (cells i0-1 .. i0+kMX, j0-1 .. j0+kMY) into registers
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int r = 0; r < PF; r++) {
+      const int e = tid + r * kMThreads;
+      const int i = i0 - 1 + e % VX, j = j0 - 1 + e / VX;
+      pf[r] = e < VP && i <= nx + 1 && j <= ny + 2 ? vof[I_VOF(i, j, kk)] : 0.;
+    }
+  };
+  auto put = [&](int kk) {
+#pragma unroll
+    for (int r = 0; r < PF; r++) {
+      const int e = tid + r * kMThreads;
+      if (e < VP) sv[kk % 3][e] = pf[r];
+    }
+  };
+  // corner normals between cell planes kk-1 and kk (compute_gradient) into
+  // ring slot kk % 2
+  auto corners = [&](int kk) {
+    const double * lo = sv[(kk + 2) % 3];   // plane kk - 1
+    const double * hi = sv[kk % 3];
+    double * o0 = sc[kk & 1][0];
+    double * o1 = sc[kk & 1][1];
+    double * o2 = sc[kk & 1][2];
+    for (int e = tid; e < CP; e += kMThreads) {
+      const int a = e % CX, b = e / CX;
+      const int i = i0 + a;
+      const double rdx = g.xt[kXT * min(i, nx + 1) + XT_RXC];   // 1 / DXC
+      const int p = a + VX * b;
+      const double a0 = lo[p], b0 = lo[p + 1], c0 = lo[p + VX], d0 = lo[p + VX + 1];
+      const double a1 = hi[p], b1 = hi[p + 1], c1 = hi[p + VX], d1 = hi[p + VX + 1];
+      const double sx0 = -a0 + b0 - c0 + d0, sx1 = -a1 + b1 - c1 + d1;
+      const double sy0 = -a0 - b0 + c0 + d0, sy1 = -a1 - b1 + c1 + d1;
+      const double sz0 = a0 + b0 + c0 + d0, sz1 = a1 + b1 + c1 + d1;
+      const double gx = rdx * (sx0 + sx1);
+      const double gy = g.rdy * (sy0 + sy1);
+      const double gz = g.rdz * (-sz0 + sz1);
+      const double norm = sqrt(gx * gx + gy * gy + gz * gz);
+      const double ninv = 1. / fmax(norm, DBL_EPSILON);
+      o0[e] = gx * ninv;
+      o1[e] = gy * ninv;
+      o2[e] = gz * ninv;
+    }
+  };
+  fetch(kb - 1);
+  put(kb - 1);
+  fetch(kb);
+  put(kb);
+  fetch(kb + 1);
+  lds_barrier();
+  corners(kb);
+  const int a = tid % kMX, b = tid / kMX;
+  const int i = i0 + a, j = j0 + b;
+  const bool own = i <= nx && j <= ny;
+  const double dx = own ? g.dxf[i - 1] : 1.;
+  const double rdx = own ? g.xt[kXT * i + XT_RXF] : 0.;   // 1 / DXF
+  for (int k = kb; k <= ke; k++) {
+    put(k + 1);
+    if (k + 2 <= ke + 1) fetch(k + 2);
+    lds_barrier();
+    corners(k + 1);
+    lds_barrier();
+    if (own) {
+      const double * L0 = sc[k & 1][0];
+      const double * L1 = sc[k & 1][1];
+      const double * L2 = sc[k & 1][2];
+      const double * H0 = sc[(k + 1) & 1][0];
+      const double * H1 = sc[(k + 1) & 1][1];
+      const double * H2 = sc[(k + 1) & 1][2];
+      const int q = a + CX * b;
+      double cv;
+      {
+        const double dnxdx = rdx * ((-L0[q] + L0[q + 1] - L0[q + CX] + L0[q + CX + 1])
+                                    + (-H0[q] + H0[q + 1] - H0[q + CX] + H0[q + CX + 1]));
+        const double dnydy = g.rdy * ((-L1[q] - L1[q + 1] + L1[q + CX] + L1[q + CX + 1])
+                                      + (-H1[q] - H1[q + 1] + H1[q + CX] + H1[q + CX + 1]));
+        const double dnzdz = g.rdz * (-(L2[q] + L2[q + 1] + L2[q + CX] + L2[q + CX + 1])
+                                      + (H2[q] + H2[q + 1] + H2[q + CX] + H2[q + CX + 1]));
+        cv = 0.25 * (-dnxdx - dnydy - dnzdz);
+      }
+      store_periodic(curv, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, cv);
+      const double lvof = sv[k % 3][(a + 1) + VX * (b + 1)];
+      if (!(lvof < kVofMin || 1. - kVofMin < lvof)) {
+        double nv[4];
+        const double * Ld[3] = {L0, L1, L2};
+        const double * Hd[3] = {H0, H1, H2};
+        for (int d = 0; d < 3; d++) {
+          nv[d] = (Ld[d][q] + Ld[d][q + 1] + Ld[d][q + CX] + Ld[d][q + CX + 1])
+                + (Hd[d][q] + Hd[d][q + 1] + Hd[d][q + CX] + Hd[d][q + CX + 1]);
+        }
+        nv[0] /= dx;
+        nv[1] /= g.dy;
+        nv[2] /= g.dz;
+        {
+          const double norm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+          const double ninv = 1. / fmax(norm, DBL_EPSILON);
+          nv[0] *= ninv;
+          nv[1] *= ninv;
+          nv[2] *= ninv;
+        }
+        nv[3] = intercept(lvof, nv);
+        for (int d = 0; d <= 3; d++) {
+          double * base = normal + d;
+          store_periodic(base, (i - 1) * 4, nx * 4, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
+        }
+      }
+    }
+  }
+}
+
+template <int KZ>
+__global__ __launch_bounds__(kMThreads) void k_iface_march(Grid g, const double * __restrict__ vof,
+                                                           double * __restrict__ normal, double * __restrict__ curv) {
+  k_iface_march_body<KZ>(hw_block(), g, vof, normal, curv);
+}
+#endif
+
 // interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
 // (compute_flux_*), the flux divergence and the RK update, written to the
 // other VOF buffer with its halo images (walls stay zero).

@@ -222,6 +222,7 @@ struct vof_ctx {
   int fft_elems;                         // LDS complex elements per FFT buffer (1024 / 2048)
   int lines_lpb;                         // lines per work-group of the x solver (0 = auto)
   int rhs_rows;                          // rows marched per thread by the source kernel (1 = one DOF per thread)
+  int iface_planes;                      // z planes marched per work-group by the 3D interface kernel (0 = blocks)
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
   Grid grid;
@@ -825,6 +826,7 @@ static int ctx_create(domain_t * domain) {
   ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
   ctx->rhs_rows = 8;
+  ctx->iface_planes = 8;
   ctx->d_vof_spare = nullptr;
   ctx->vof_bytes = 0;
   ctx->prof_on = false;
@@ -1815,11 +1817,31 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
     }
     // curvature tensor (curvature_tensor.c:178-186)
     if (fz & F_IFACE) {
-      const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
-                    (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
-      KLB(ctx, "k_iface_fused", vofk::kIThreads, vofk::k_iface_fused_body, vofk::k_iface_fused, gi,
-          dim3(vofk::kIThreads), 0, ctx->grid, (const double *)interface->vof.data,
-          (double *)interface->normal.data, (double *)interface->curv.data);
+      const double * v = (const double *)interface->vof.data;
+      double * nrm = (double *)interface->normal.data;
+      double * cu = (double *)interface->curv.data;
+#if NDIMS == 3
+      const int kz = ctx->iface_planes;
+      const dim3 gm((unsigned)((nx + vofk::kMX - 1) / vofk::kMX), (unsigned)((ny + vofk::kMY - 1) / vofk::kMY),
+                    (unsigned)((nz + (kz > 0 ? kz : 1) - 1) / (kz > 0 ? kz : 1)));
+      const dim3 bm(vofk::kMThreads);
+      if (kz == 4) {
+        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<4>, vofk::k_iface_march<4>, gm, bm, 0,
+            ctx->grid, v, nrm, cu);
+      } else if (kz == 8) {
+        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<8>, vofk::k_iface_march<8>, gm, bm, 0,
+            ctx->grid, v, nrm, cu);
+      } else if (kz == 16) {
+        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<16>, vofk::k_iface_march<16>, gm, bm, 0,
+            ctx->grid, v, nrm, cu);
+      } else
+#endif
+      {
+        const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
+                      (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
+        KLB(ctx, "k_iface_fused", vofk::kIThreads, vofk::k_iface_fused_body, vofk::k_iface_fused, gi,
+            dim3(vofk::kIThreads), 0, ctx->grid, v, nrm, cu);
+      }
     } else {
       if (interface_compute_curvature_tensor(domain, interface)) return 1;
     }
@@ -1888,9 +1910,10 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
 
 // tile parameters: "fft_elems" (1024 or 2048 complex per LDS buffer),
 // "lines_lpb" (lines per work-group of the x solver, 0 = automatic),
-// "xcd_bands" (band tile order of the 3D stencil kernels) and "rhs_rows"
+// "xcd_bands" (band tile order of the 3D stencil kernels), "rhs_rows"
 // (rows of one column marched per thread by the source kernel: 1, 2, 4, 8,
-// 16)
+// 16) and "iface_planes" (z planes marched per work-group by the 3D
+// interface kernel: 0 = 32 x 4 x 4 blocks, 4, 8, 16)
 extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   CTX_OR_FAIL(domain);
   if (!strcmp(name, "fft_elems")) {
@@ -1911,6 +1934,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   if (!strcmp(name, "lines_lpb")) {
     if (value < 0 || value > 64) return fail("lines_lpb must be in 0..64");
     ctx->lines_lpb = value;
+    return 0;
+  }
+  if (!strcmp(name, "iface_planes")) {
+    if (value != 0 && value != 4 && value != 8 && value != 16) return fail("iface_planes must be 0, 4, 8 or 16");
+    ctx->iface_planes = value;
     return 0;
   }
   if (!strcmp(name, "rhs_rows")) {
