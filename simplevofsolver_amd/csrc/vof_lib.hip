@@ -416,7 +416,8 @@ struct vof_batch;
 // one recorded launch; `merge` is specific to the kernel instantiation and
 // doubles as its identity
 struct LaunchRec {
-  size_t (*merge)(vof_batch * B, size_t idx, unsigned char * host, const unsigned char * dev);
+  size_t (*merge)(vof_batch * B, size_t idx, const std::vector<int> & cases, unsigned char * host,
+                  const unsigned char * dev);
   const char * name;
   dim3 grid, block;
   size_t shm;
@@ -450,24 +451,26 @@ struct vof_batch {
 };
 
 // build (dev == nullptr: size only / fill host) or launch (dev != nullptr)
-// the merged launch number idx; returns the bytes of its argument table
+// the merged launch number idx of the listed cases (which all recorded the
+// same kernel there); returns the bytes of its argument table
 template <auto Body, unsigned BOUNDS, class... A>
-static size_t merge_launch(vof_batch * B, size_t idx, unsigned char * host, const unsigned char * dev) {
+static size_t merge_launch(vof_batch * B, size_t idx, const std::vector<int> & cases, unsigned char * host,
+                           const unsigned char * dev) {
   using E = BatchEntry<A...>;
-  const size_t n = B->ctx.size();
+  const size_t n = cases.size();
   const size_t bytes = sizeof(E) * n;
   if (!dev) {
     if (host) {
       unsigned first = 0;
-      for (size_t c = 0; c < n; c++) {
-        const LaunchRec & r = B->rec[c].recs[idx];
+      for (size_t m = 0; m < n; m++) {
+        const LaunchRec & r = B->rec[cases[m]].recs[idx];
         E e;
         e.first = first;
         e.gx = r.grid.x;
         e.gy = r.grid.y;
         e.gz = r.grid.z;
         memcpy(&e.args, r.pack.data(), sizeof(Pack<A...>));
-        memcpy(host + sizeof(E) * c, &e, sizeof(E));
+        memcpy(host + sizeof(E) * m, &e, sizeof(E));
         first += r.grid.x * r.grid.y * r.grid.z;
       }
     }
@@ -475,12 +478,12 @@ static size_t merge_launch(vof_batch * B, size_t idx, unsigned char * host, cons
   }
   unsigned total = 0;
   size_t shm = 0;
-  for (size_t c = 0; c < n; c++) {
-    const LaunchRec & r = B->rec[c].recs[idx];
+  for (size_t m = 0; m < n; m++) {
+    const LaunchRec & r = B->rec[cases[m]].recs[idx];
     total += r.grid.x * r.grid.y * r.grid.z;
     shm = r.shm > shm ? r.shm : shm;
   }
-  const dim3 blk = B->rec[0].recs[idx].block;
+  const dim3 blk = B->rec[cases[0]].recs[idx].block;
   hipLaunchKernelGGL((k_batched<Body, BOUNDS, A...>), dim3(total), blk, shm, B->stream,
                      reinterpret_cast<const E *>(dev), (int)n);
   return hipGetLastError() == hipSuccess ? bytes : 0;
@@ -1995,7 +1998,15 @@ static int batch_stage_reserve(vof_batch * B, size_t bytes) {
   return 0;
 }
 
-// merge the recorded launches of every case, launch by launch, on B->stream
+// merge the recorded launches of every case, launch by launch, on B->stream.
+// Cases whose i-th launch is the same kernel (instantiation and work-group
+// shape) share one merged launch; sizes that select another instantiation
+// (e.g. the 256-point transforms) form a group of their own.
+struct BatchGroup {
+  size_t r;                  // launch number
+  std::vector<int> cases;
+};
+
 static int batch_run(vof_batch * B) {
   const size_t n = B->ctx.size();
   const size_t nrec = B->rec[0].recs.size();
@@ -2003,26 +2014,41 @@ static int batch_run(vof_batch * B) {
     if (B->rec[c].recs.size() != nrec) return fail("batch: case %zu records %zu launches, case 0 %zu", c,
                                                    B->rec[c].recs.size(), nrec);
   }
-  std::vector<size_t> off(nrec + 1, 0);
+  std::vector<BatchGroup> groups;
   for (size_t r = 0; r < nrec; r++) {
-    const LaunchRec & r0 = B->rec[0].recs[r];
-    for (size_t c = 1; c < n; c++) {
+    const size_t g0 = groups.size();
+    for (size_t c = 0; c < n; c++) {
       const LaunchRec & rc = B->rec[c].recs[r];
-      if (rc.merge != r0.merge || rc.block.x != r0.block.x || rc.block.y != r0.block.y || rc.block.z != r0.block.z)
-        return fail("batch: launch %zu differs between case 0 (%s) and case %zu (%s)", r, r0.name, c, rc.name);
+      size_t gi = g0;
+      for (; gi < groups.size(); gi++) {
+        const LaunchRec & rg = B->rec[groups[gi].cases[0]].recs[r];
+        if (rg.merge == rc.merge && rg.block.x == rc.block.x && rg.block.y == rc.block.y && rg.block.z == rc.block.z)
+          break;
+      }
+      if (gi == groups.size()) groups.push_back({r, {}});
+      groups[gi].cases.push_back((int)c);
     }
-    const size_t bytes = r0.merge(B, r, nullptr, nullptr);
-    off[r + 1] = off[r] + ((bytes + 255) & ~(size_t)255);
+  }
+  const size_t ng = groups.size();
+  std::vector<size_t> off(ng + 1, 0);
+  for (size_t q = 0; q < ng; q++) {
+    const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
+    const size_t bytes = r0.merge(B, groups[q].r, groups[q].cases, nullptr, nullptr);
+    off[q + 1] = off[q] + ((bytes + 255) & ~(size_t)255);
   }
   // the previous copy out of the staging buffer must have completed
   HIPCHK(hipEventSynchronize(B->stage_done));
-  if (batch_stage_reserve(B, off[nrec])) return 1;
-  for (size_t r = 0; r < nrec; r++) B->rec[0].recs[r].merge(B, r, B->h_stage + off[r], nullptr);
-  HIPCHK(hipMemcpyAsync(B->d_stage, B->h_stage, off[nrec], hipMemcpyHostToDevice, B->stream));
+  if (batch_stage_reserve(B, off[ng])) return 1;
+  for (size_t q = 0; q < ng; q++) {
+    const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
+    r0.merge(B, groups[q].r, groups[q].cases, B->h_stage + off[q], nullptr);
+  }
+  HIPCHK(hipMemcpyAsync(B->d_stage, B->h_stage, off[ng], hipMemcpyHostToDevice, B->stream));
   HIPCHK(hipEventRecord(B->stage_done, B->stream));
-  for (size_t r = 0; r < nrec; r++) {
-    if (B->rec[0].recs[r].merge(B, r, nullptr, B->d_stage + off[r]) == 0)
-      return fail("batch: launch of %s failed", B->rec[0].recs[r].name);
+  for (size_t q = 0; q < ng; q++) {
+    const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
+    if (r0.merge(B, groups[q].r, groups[q].cases, nullptr, B->d_stage + off[q]) == 0)
+      return fail("batch: launch of %s failed", r0.name);
   }
   return 0;
 }

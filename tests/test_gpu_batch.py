@@ -23,14 +23,28 @@ def _cases_3d():
     return [random_fields((16, 12, 20), seed=66), random_fields((12, 16, 8), seed=67, uniformx=False)]
 
 
+def _cases_mixed_transforms():
+    # 256-point y / z transforms select other kernel instantiations than the
+    # other lengths: the batch groups those cases into launches of their own
+    return [random_fields((16, 256), seed=71), random_fields((12, 40), seed=72, uniformx=False),
+            random_fields((8, 256), seed=73, uniformx=False), random_fields((20, 48), seed=74)]
+
+
+def _cases_mixed_transforms_3d():
+    return [random_fields((8, 12, 256), seed=75), random_fields((10, 256, 6), seed=76, uniformx=False),
+            random_fields((12, 16, 8), seed=77)]
+
+
 def _names(nd):
     return ["ux", "uy", "t", "vof", "p", "psi"] + (["uz"] if nd == 3 else [])
 
 
 @pytest.mark.parametrize("mixed", [-1, 2])
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [2, 3, "2-256", "3-256"])
 def test_batch_is_bit_identical_to_single_cases(dim, mixed, tmp_path):
-    cases = _cases_2d() if dim == 2 else _cases_3d()
+    cases = {2: _cases_2d, 3: _cases_3d, "2-256": _cases_mixed_transforms,
+             "3-256": _cases_mixed_transforms_3d}[dim]()
+    dim = len(cases[0].glsizes)
     singles = [sva.solver_from_ic(ic, str(tmp_path / f"a{c}")) for c, ic in enumerate(cases)]
     members = [sva.solver_from_ic(ic, str(tmp_path / f"b{c}")) for c, ic in enumerate(cases)]
     for s in singles + members:
