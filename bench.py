@@ -61,6 +61,7 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         "k_dt_partial": c * 24,
         # fused schedule: vof read; normal (3) and curvature written
         "k_iface_fused": c * (8 + 24 + 8),
+        "k_iface_march": c * (8 + 24 + 8),
         # velocity (3), vof, normal (3), srcb read; srca and the new vof written
         "k_vof_fused": c * (24 + 8 + 24 + 8 + 8 + 8),
         # ux uy uz p t, vof, curvature, srcb of 4 fields read; srca and the
