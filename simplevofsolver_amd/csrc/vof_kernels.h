@@ -1208,20 +1208,16 @@ __global__ __launch_bounds__(kLLines) void k_lines(LineJob J) {
 constexpr int kWThreads = 256;
 constexpr int kWUnroll = 4;
 
-__device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, int lpb) {
-  extern __shared__ double S[];
-  const LineField & F = J.f[vb.y];
+// Phases of the x-line solve of `nl` lines starting at line0, their values
+// staged in S (row stride ld).  Phase 1: increments into S, by threads
+// t = 0..nt-1; element e = t + nt r, (ll, m) tracked incrementally.
+__device__ __forceinline__ void lines_fill(const LineField & F, const LineJob & J, double * S, int ld, int line0,
+                                           int nl, int t, int nt) {
   const int n = F.n;
-  const int ld = n | 1;
-  const int tid = threadIdx.x;
-  const int line0 = vb.x * lpb;
-  const int nl = min(lpb, J.nlines - line0);
-  if (nl <= 0) return;
   const int total = nl * n;
   const size_t g0 = (size_t)line0 * n;
-  // phase 1: element e = tid + kWThreads * r; (ll, m) tracked incrementally
   {
-    int e = tid, ll = tid / n, m = tid - (tid / n) * n;
+    int e = t, ll = t / n, m = t - (t / n) * n;
     while (e < total) {
       double v[kWUnroll];
       int lls[kWUnroll], ms[kWUnroll], cnt = 0;
@@ -1233,8 +1229,8 @@ __device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, i
           lls[u] = ll;
           ms[u] = m;
           cnt = u + 1;
-          e += kWThreads;
-          m += kWThreads;
+          e += nt;
+          m += nt;
           while (m >= n) { m -= n; ll++; }
         }
       }
@@ -1243,14 +1239,18 @@ __device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, i
         if (u < cnt) S[lls[u] * ld + ms[u]] = v[u];
     }
   }
-  __syncthreads();
-  if (tid < nl) {
+}
+
+// phase 2: forward elimination and back substitution of one line q[0..n-1]
+// (the reference's gtsv with the shared factors of the table)
+__device__ __forceinline__ void lines_sweep(const LineField & F, double * q) {
+  const int n = F.n;
+  {
     const double c0 = F.tab[0], rlast = F.tab[1];
     const bool flag = F.tab[2] != 0.;
     const double * __restrict__ L = F.tab + 4;
     const double * __restrict__ VAL = F.tab + 4 + n;
     const double * __restrict__ V = F.tab + 4 + 2 * n;
-    double * q = S + tid * ld;
     double qprev = q[0] / c0;
     q[0] = qprev;
     int i = 1;
@@ -1290,15 +1290,18 @@ __device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, i
       qnext = a;
     }
   }
-  __syncthreads();
-  // phase 3: field rows += solution.  Wavefront w owns lines w, w + 4, ...
-  // (two per pass), so a line's row / plane, its address and whether it has
-  // periodic halo images are wave-uniform; lanes walk x contiguously.
+}
+
+// phase 3: field rows += solution by wavefronts w = 0..nw-1 (lane = lane
+// index).  Wavefront w owns lines w, w + nw, ... (two per pass), so a
+// line's row / plane, its address and whether it has periodic halo images
+// are wave-uniform; lanes walk x contiguously.
+__device__ __forceinline__ void lines_update(const LineField & F, const LineJob & J, const double * S, int ld,
+                                             int line0, int nl, int w, int nw, int lane) {
+  const int n = F.n;
   {
-    constexpr int kWaves = kWThreads / 64;
-    const int lane = tid & 63;
-    for (int la = tid >> 6; la < nl; la += 2 * kWaves) {
-      const int lb = la + kWaves;
+    for (int la = w; la < nl; la += 2 * nw) {
+      const int lb = la + nw;
       const bool hb = lb < nl;
       const int linea = line0 + la, lineb = line0 + (hb ? lb : la);
       const int ja = linea % J.ny, ka = linea / J.ny;
@@ -1354,6 +1357,21 @@ __device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, i
       }
     }
   }
+}
+
+__device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, int lpb) {
+  extern __shared__ double S[];
+  const LineField & F = J.f[vb.y];
+  const int ld = F.n | 1;
+  const int tid = threadIdx.x;
+  const int line0 = vb.x * lpb;
+  const int nl = min(lpb, J.nlines - line0);
+  if (nl <= 0) return;
+  lines_fill(F, J, S, ld, line0, nl, tid, kWThreads);
+  __syncthreads();
+  if (tid < nl) lines_sweep(F, S + tid * ld);
+  __syncthreads();
+  lines_update(F, J, S, ld, line0, nl, tid >> 6, kWThreads / 64, tid & 63);
 }
 
 __global__ __launch_bounds__(kWThreads) void k_lines_lds(LineJob J, int lpb) {

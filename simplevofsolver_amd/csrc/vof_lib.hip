@@ -1468,9 +1468,9 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     if (sizeof(double) * (nx | 1) > 65536) return fail("x solver: nx = %d exceeds the LDS line tile", nx);
     if ((size_t)lpb * (nx | 1) * sizeof(double) > 65536) lpb = (int)(65536 / (sizeof(double) * (nx | 1)));
     const size_t shm = sizeof(double) * (size_t)lpb * (nx | 1);
-    const dim3 lg((unsigned)((lj.nlines + lpb - 1) / lpb), (unsigned)nj);
-    KLB(ctx, "k_lines_lds", vofk::kWThreads, vofk::k_lines_lds_body, vofk::k_lines_lds, lg,
-        dim3(vofk::kWThreads), shm, lj, lpb);
+    const int ngroups = (lj.nlines + lpb - 1) / lpb;
+    KLB(ctx, "k_lines_lds", vofk::kWThreads, vofk::k_lines_lds_body, vofk::k_lines_lds,
+        dim3((unsigned)ngroups, (unsigned)nj), dim3(vofk::kWThreads), shm, lj, lpb);
   }
   if (images) return 0;
   for (int f = 0; f < nj; f++) {
