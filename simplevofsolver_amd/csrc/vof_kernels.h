@@ -2212,24 +2212,29 @@ __device__ __forceinline__ void k_dt_partial_body(const VBlock & vb, Grid g, con
 #endif
   const double small = 1.e-8;
   double m = 1.;
-  for (size_t t = (size_t)vb.x * blockDim.x + threadIdx.x; t < ncell;
-       t += (size_t)vb.gx * blockDim.x) {
-    const int i = (int)(t % nx) + 1;
-    const size_t r = t / nx;
-    const int j = (int)(r % ny) + 1;
+  // 64 x 4 tiles of (i, j): thread (i, j0) of work-group (x, y, z) walks
+  // the rows j0, j0 + 4 gy, ... and planes z + 1, z + 1 + gz, ... of its
+  // column (no index division)
+  const int i = (int)(vb.x * 64 + threadIdx.x % 64) + 1;
+  (void)ncell;
+  if (i <= nx) {
+    const double dxc = i >= 2 ? g.dxc[i - 1] : 1.;
+    for (int j = (int)(vb.y * 4 + threadIdx.x / 64) + 1; j <= ny; j += 4 * (int)vb.gy) {
 #if NDIMS == 2
-    const int k = 1;
+      for (int k = 1; k <= 1; k++) {
 #else
-    const int k = (int)(r / ny) + 1;
+      for (int k = (int)vb.z + 1; k <= g.nz; k += (int)vb.gz) {
 #endif
-    if (i >= 2) m = fmin(m, g.dxc[i - 1] / (fabs(ux[I_UX(i, j, k)]) + small));
-    m = fmin(m, g.dy / (fabs(uy[I_C1(i, j, k)]) + small));
+        if (i >= 2) m = fmin(m, dxc / (fabs(ux[I_UX(i, j, k)]) + small));
+        m = fmin(m, g.dy / (fabs(uy[I_C1(i, j, k)]) + small));
 #if NDIMS == 3
-    m = fmin(m, g.dz / (fabs(uz[I_C1(i, j, k)]) + small));
+        m = fmin(m, g.dz / (fabs(uz[I_C1(i, j, k)]) + small));
 #endif
+      }
+    }
   }
   m = block_min(m);
-  if (threadIdx.x == 0) part[vb.x] = m;
+  if (threadIdx.x == 0) part[vb.x + vb.gx * (vb.y + vb.gy * vb.z)] = m;
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_dt_partial(Grid g, const double * __restrict__ ux,

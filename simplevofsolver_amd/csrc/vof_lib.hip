@@ -971,7 +971,7 @@ static int ctx_create(domain_t * domain) {
   HIPCHK(hipMalloc(&ctx->d_spec, sizeof(double2) * nx * nline));
   HIPCHK(hipMalloc(&ctx->d_vbuf, sizeof(double) * nx * nline));
   HIPCHK(hipMalloc(&ctx->d_delta, sizeof(double) * ncell * (NDIMS + 1)));
-  ctx->npart = 1024;
+  ctx->npart = 8192;
   HIPCHK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->npart * vofk::kDiagN));
   HIPCHK(hipHostMalloc(&ctx->h_scalar, sizeof(double) * 64));
   // Thomas tables: header 4 + l, val, v
@@ -1197,15 +1197,21 @@ extern "C" int fluid_update_boundaries_t(const domain_t * domain, array_t * t) {
 
 // fluid_decide_dt (decide_dt.c:104-145)
 static int decide_dt_enqueue(vof_ctx * ctx, const fluid_t * fluid) {
-  const size_t ncell = (size_t)ctx->nx * ctx->ny * ctx->nz;
-  int nb = (int)((ncell + vofk::kRedThreads - 1) / vofk::kRedThreads);
-  if (nb > ctx->npart) nb = ctx->npart;
+  // 64 x 4 (i, j) tiles, z planes split over gz work-groups so that the
+  // partial minima fit d_part
+  const unsigned gx = (unsigned)((ctx->nx + 63) / 64);
+  unsigned gy = (unsigned)((ctx->ny + 3) / 4);
+  unsigned gz = NDIMS == 3 ? (unsigned)((ctx->nz + 15) / 16) : 1u;
+  while (gz > 1 && (size_t)gx * gy * gz > (size_t)ctx->npart) gz--;
+  while (gy > 1 && (size_t)gx * gy * gz > (size_t)ctx->npart) gy--;
+  if ((size_t)gx * gy * gz > (size_t)ctx->npart) return fail("decide_dt: %u x-tiles exceed the partial buffer", gx);
+  const int nb = (int)(gx * gy * gz);
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
 #else
   const double * uz = nullptr;
 #endif
-  KLB(ctx, "k_dt_partial", vofk::kRedThreads, vofk::k_dt_partial_body, vofk::k_dt_partial, dim3(nb),
+  KLB(ctx, "k_dt_partial", vofk::kRedThreads, vofk::k_dt_partial_body, vofk::k_dt_partial, dim3(gx, gy, gz),
       dim3(vofk::kRedThreads), 0, ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data,
       uz, ctx->d_part);
   KLB(ctx, "k_min_final", vofk::kRedThreads, vofk::k_min_final_body, vofk::k_min_final, dim3(1),
