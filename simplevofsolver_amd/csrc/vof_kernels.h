@@ -1763,7 +1763,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const 
 
 // solve_linear_systems (dft.c:364-395): per (ky, kz) line the complex
 // tri-diagonal system in x with c = -l - u + lambda and the Neumann fold,
-// eliminated as gtsv (tdm.c:17-50).  v goes to scratch vbuf[x][line].  One
+// eliminated as gtsv (tdm.c:17-50).  v goes to vbuf[x][line] when store_v
+// (the factors depend only on the grid and the wavenumbers, so they are
+// written by the first solve and read back by every later one).  One
 // lane per line; the rows of the next chunk of kTsU are loaded before the
 // current chunk is eliminated, so kTsU loads per lane stay in flight.
 constexpr int kTsU = 8;
@@ -1780,7 +1782,7 @@ template <class T2>
 __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
-                              double * __restrict__ vbuf) {
+                              double * __restrict__ vbuf, int store_v) {
   const int line = (int)(vb.x * blockDim.x + threadIdx.x);
   if (line >= nlines) return;
   const int ky = line % nh;
@@ -1803,7 +1805,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
   qprev.x = qprev.x / cc;
   qprev.y = qprev.y / cc;
   q[0] = from_d2<T2>(qprev);
-  v[0] = vprev;
+  if (store_v) v[0] = vprev;
   {
     double2 cur[kTsU];
 #pragma unroll
@@ -1828,7 +1830,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
           qi.x = val * (qi.x - pl[i] * qprev.x);
           qi.y = val * (qi.y - pl[i] * qprev.y);
           q[(size_t)i * st] = from_d2<T2>(qi);
-          v[(size_t)i * st] = vi;
+          if (store_v) v[(size_t)i * st] = vi;
           qprev = qi;
           vprev = vi;
         }
@@ -1905,8 +1907,8 @@ template <class T2>
 __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
-                              double * __restrict__ vbuf) {
-  k_thomas_spec_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf);
+                              double * __restrict__ vbuf, int store_v) {
+  k_thomas_spec_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf, store_v);
 }
 
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and

@@ -252,6 +252,7 @@ struct vof_ctx {
   float2 * d_twy32;
   float2 * d_twz32;
   double * d_vbuf;
+  bool vbuf_ready;             // the Thomas back-substitution factors are in d_vbuf
   // scratch
   double * d_delta;                      // x1 pencil of increments
   double * d_part;
@@ -970,6 +971,7 @@ static int ctx_create(domain_t * domain) {
   const size_t nline = (size_t)ctx->nh * nz;
   HIPCHK(hipMalloc(&ctx->d_spec, sizeof(double2) * nx * nline));
   HIPCHK(hipMalloc(&ctx->d_vbuf, sizeof(double) * nx * nline));
+  ctx->vbuf_ready = false;
   HIPCHK(hipMalloc(&ctx->d_delta, sizeof(double) * ncell * (NDIMS + 1)));
   ctx->npart = 8192;
   HIPCHK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->npart * vofk::kDiagN));
@@ -1193,6 +1195,14 @@ extern "C" int fluid_update_boundaries_psi(const domain_t * domain, array_t * ps
 extern "C" int fluid_update_boundaries_t(const domain_t * domain, array_t * t) {
   CTX_OR_FAIL(domain);
   return bc_t(ctx, (double *)t->data);
+}
+
+// 1 for the first spectral Thomas solve (which writes the back-substitution
+// factors), 0 afterwards
+static int vbuf_store(vof_ctx * ctx) {
+  const int first = ctx->vbuf_ready ? 0 : 1;
+  ctx->vbuf_ready = true;
+  return first;
 }
 
 // fluid_decide_dt (decide_dt.c:104-145)
@@ -1529,7 +1539,7 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
         ctx->d_twz32, ctx->d_spec32, ctx->nh, -1.f);
 #endif
     KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>, dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines,
-        ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec32, ctx->d_vbuf);
+        ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
 #if NDIMS == 3
     KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z,
         ctx->d_twz32, ctx->d_spec32, ctx->nh, +1.f);
@@ -1579,7 +1589,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
     if (pass == 1) {
       const int nlines = ctx->nh * nz;
       KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
-          dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+          dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
     }
     const double sign = pass == 0 ? -1. : +1.;
     if (f256z) {
@@ -1597,7 +1607,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   {
     const int nlines = ctx->nh * nz;
     KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
-        dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+        dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
   }
 #endif
 #define IRF_ARGS ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data, images
@@ -2419,7 +2429,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
         }
         hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
                            ctx->nh, (const double *)ctx->d_pl, (const double *)ctx->d_pu,
-                           (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf);
+                           (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
         if (use1024) {
           hipLaunchKernelGGL(vofk::k_irfft_y_psi<1024>, dim3(nbx), dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y,
                              (const double2 *)ctx->d_twy, (const double2 *)ctx->d_spec, x, 1);
