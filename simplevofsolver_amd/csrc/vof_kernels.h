@@ -1359,7 +1359,7 @@ __device__ __forceinline__ void lines_update(const LineField & F, const LineJob 
   }
 }
 
-__device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, LineJob J, int lpb) {
+__device__ __forceinline__ void k_lines_lds_body(const VBlock & vb, const LineJob & J, int lpb) {
   extern __shared__ double S[];
   const LineField & F = J.f[vb.y];
   const int ld = F.n | 1;
@@ -1623,7 +1623,7 @@ constexpr int kFftElems = 2048;   // complex elements per LDS buffer
 // after the transform.  Output spec[x][k][ky].
 template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb, 
-    Grid g, FftPlan P, const double2 * __restrict__ tw, const double * __restrict__ ux,
+    Grid g, const FftPlan & P, const double2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor,
     double2 * __restrict__ spec) {
   __shared__ double2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
@@ -1708,7 +1708,7 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
 #if NDIMS == 3
 // complex transform in z of spec[x][kz][ky] for a run of ky at one x
 template <int ELEMS, class T2, bool F256 = false>
-__device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, FftPlan P, const T2 * __restrict__ tw,
+__device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, const FftPlan & P, const T2 * __restrict__ tw,
                                                       T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
   __shared__ T2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
   __shared__ T2 sb[F256 ? 1 : ELEMS];
@@ -1915,7 +1915,7 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
 // k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
 template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb, 
-    Grid g, FftPlan P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
+    Grid g, const FftPlan & P, const double2 * __restrict__ tw, const double2 * __restrict__ spec,
     double * __restrict__ psi, int images) {
   __shared__ double2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
   __shared__ double2 sb[F256 ? 1 : ELEMS];
@@ -2003,7 +2003,7 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
 // Then (float)(r * scale) is transformed in y as k_div_rfft_y does.
 template <int ELEMS>
 __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb, 
-    Grid g, FftPlan P, const float2 * __restrict__ tw, const double * __restrict__ ux,
+    Grid g, const FftPlan & P, const float2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
     double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
     const double * __restrict__ pu, int mode, float2 * __restrict__ spec) {
@@ -2085,7 +2085,7 @@ __global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
 // psi += delta (accumulate 1) in FP64, with halo images and Neumann walls
 template <int ELEMS>
 __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb, 
-    Grid g, FftPlan P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
+    Grid g, const FftPlan & P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
     double * __restrict__ psi, int accumulate) {
   __shared__ float2 sa[ELEMS], sb[ELEMS];
   const int nx = g.nx, ny = g.ny;
