@@ -233,6 +233,7 @@ struct vof_ctx {
   double * d_lapx_c;
   // predictor Thomas tables: one slot per (field, rkstep)
   size_t tab_stride;
+  size_t tab_rk;               // doubles between the tables of consecutive RK stages
   double * d_tab;
   double * h_tab;                        // pinned
   hipEvent_t tab_done[4 * 3];
@@ -978,6 +979,7 @@ static int ctx_create(domain_t * domain) {
   HIPCHK(hipHostMalloc(&ctx->h_scalar, sizeof(double) * 64));
   // Thomas tables: header 4 + l, val, v
   ctx->tab_stride = 4 + 3 * (size_t)nx;
+  ctx->tab_rk = 4 * ctx->tab_stride;
   HIPCHK(hipMalloc(&ctx->d_tab, sizeof(double) * ctx->tab_stride * 12));
   HIPCHK(hipHostMalloc(&ctx->h_tab, sizeof(double) * ctx->tab_stride * 12));
   for (int m = 0; m < 12; m++) HIPCHK(hipEventCreateWithFlags(&ctx->tab_done[m], hipEventDisableTiming));
@@ -1308,11 +1310,11 @@ static void build_tab(double * tab, int n, double prefactor, const double * lapx
 // from a pinned slot once the previous copy from that slot has completed
 static int upload_tab(vof_ctx * ctx, int field, int rkstep, double prefactor, const double ** dtab) {
   const int slot = field * 3 + rkstep;
-  double * h = ctx->h_tab + ctx->tab_stride * slot;
-  double * d = ctx->d_tab + ctx->tab_stride * slot;
+  double * h = ctx->h_tab + ctx->tab_rk * rkstep + ctx->tab_stride * field;
+  double * d = ctx->d_tab + ctx->tab_rk * rkstep + ctx->tab_stride * field;
   const int n = field == 0 ? ctx->nx - 1 : ctx->nx;
   if (ctx->rec) {
-    // batch: the tables of all cases travel in one copy before the step
+    // batch: the tables of all cases travel in one copy per stage
     build_tab(h, n, prefactor, field == 0 ? ctx->lapx_ux.data() : ctx->lapx_c.data());
     *dtab = d;
     return 0;
@@ -1805,7 +1807,7 @@ static int ensure_vof_spare(vof_ctx * ctx, const interface_t * interface) {
 }
 
 static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
-                                  const double dtv);
+                                  const double dtv, size_t rk0 = 0, size_t rk1 = 3);
 
 static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
   vof_ctx * ctx = ctx_of(domain);
@@ -1814,9 +1816,9 @@ static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t
   return integrate_fused_stages(domain, fluid, interface, *dt);
 }
 
-// the three RK stages of one step with dt known
+// RK stages rk0 .. rk1-1 (all three by default) of one step with dt known
 static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
-                                  const double dtv) {
+                                  const double dtv, size_t rk0, size_t rk1) {
   vof_ctx * ctx = ctx_of(domain);
   const int fz = ctx->fuse;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
@@ -1825,7 +1827,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
 #else
   double * uz = nullptr;
 #endif
-  for (size_t rk = 0; rk < 3; rk++) {
+  for (size_t rk = rk0; rk < rk1; rk++) {
     if (ctx->tape) {
       // the adjoint's recomputation records each stage's input state
       double ** t = ctx->tape + 3 * rk;
@@ -2101,10 +2103,13 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
     return fail("vof_batch_create: stream / event creation failed");
   }
   int rc = 0;
-  // every case's tables in one pinned block and one device block
+  // every case's tables in one pinned block and one device block, stage
+  // major: the four tables of every case for stage rk are one contiguous run
   size_t tab = 0;
-  for (auto * ctx : B->ctx) tab += ctx->tab_stride * 12;
-  B->tab_doubles = tab;
+  for (auto * ctx : B->ctx) tab += ctx->tab_stride * 4;
+  B->tab_doubles = 3 * tab;
+  const size_t tab_stage = tab;
+  tab = B->tab_doubles;
   if (hipHostMalloc(&B->h_tab, sizeof(double) * tab) != hipSuccess || hipMalloc(&B->d_tab, sizeof(double) * tab) != hipSuccess)
     rc = fail("vof_batch_create: table allocation failed");
   std::vector<const double *> src(ncases);
@@ -2121,8 +2126,9 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
       dev_free(ctx->d_tab);
       ctx->h_tab = B->h_tab + o;
       ctx->d_tab = B->d_tab + o;
+      ctx->tab_rk = tab_stage;
       ctx->tab_shared = true;
-      o += ctx->tab_stride * 12;
+      o += ctx->tab_stride * 4;
     }
     if (hipMalloc(&B->d_dt, sizeof(double) * ncases) != hipSuccess ||
         hipHostMalloc(&B->h_dt, sizeof(double) * ncases) != hipSuccess ||
@@ -2160,16 +2166,21 @@ extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
     B->ctx[c]->h_scalar[0] = B->h_dt[c];
     if (decide_dt_finish(B->ctx[c], B->domain[c], B->fluid[c], &dt[c], false)) return 1;
   }
-  // the three RK stages of every case, recorded then merged
-  for (size_t c = 0; c < n && !rc; c++) {
-    B->rec[c].recs.clear();
-    B->ctx[c]->rec = &B->rec[c];
-    rc = integrate_fused_stages(B->domain[c], B->fluid[c], B->interface[c], dt[c]);
-    B->ctx[c]->rec = nullptr;
+  // the RK stages of every case, recorded then merged stage by stage: the
+  // host records stage rk + 1 while the device runs stage rk
+  const size_t tab_stage = B->tab_doubles / 3;
+  for (size_t rk = 0; rk < 3; rk++) {
+    for (size_t c = 0; c < n && !rc; c++) {
+      B->rec[c].recs.clear();
+      B->ctx[c]->rec = &B->rec[c];
+      rc = integrate_fused_stages(B->domain[c], B->fluid[c], B->interface[c], dt[c], rk, rk + 1);
+      B->ctx[c]->rec = nullptr;
+    }
+    if (rc) return 1;
+    HIPCHK(hipMemcpyAsync(B->d_tab + rk * tab_stage, B->h_tab + rk * tab_stage, sizeof(double) * tab_stage,
+                          hipMemcpyHostToDevice, B->stream));
+    if (batch_run(B)) return 1;
   }
-  if (rc) return 1;
-  HIPCHK(hipMemcpyAsync(B->d_tab, B->h_tab, sizeof(double) * B->tab_doubles, hipMemcpyHostToDevice, B->stream));
-  if (batch_run(B)) return 1;
   if (dts)
     for (size_t c = 0; c < n; c++) dts[c] = dt[c];
   return 0;
@@ -2193,6 +2204,7 @@ extern "C" int vof_batch_destroy(vof_batch_t * B) {
         hipMalloc(&d, sizeof(double) * ctx->tab_stride * 12) == hipSuccess) {
       ctx->h_tab = h;
       ctx->d_tab = d;
+      ctx->tab_rk = 4 * ctx->tab_stride;
     } else {
       ctx->h_tab = nullptr;
       ctx->d_tab = nullptr;
