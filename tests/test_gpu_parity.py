@@ -182,3 +182,24 @@ def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
     finally:
         g0.close()
         g1.close()
+
+
+@pytest.mark.parametrize("uniformx", [True, False])
+def test_3d_step_is_symmetric_in_y_and_z(uniformx, tmp_path):
+    """The GPU step keeps a y <-> z symmetric 3D state symmetric (every z
+    term equals its y counterpart) and agrees with the oracle on it."""
+    from test_oracle_pins import yz_asymmetry, yz_symmetric_case
+    nx, n = 10, 12
+    ic = yz_symmetric_case(nx, n, seed=41, uniformx=uniformx)
+    o, g = make_pair(ic, tmp_path)
+    try:
+        assert yz_asymmetry(g.get, nx, n) == 0.0
+        for step in range(3):
+            dto = o.integrate()
+            dtg = g.integrate()
+            assert abs(dtg - dto) <= 1e-12 * dto
+            assert yz_asymmetry(g.get, nx, n) < 1e-10, step
+        for name in ("ux", "uy", "uz", "t", "vof"):
+            assert rel(g.get(name), o.get(name)) < 1e-10, name
+    finally:
+        g.close()
