@@ -62,12 +62,15 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         # fused schedule: vof read; normal (3) and curvature written
         "k_iface_fused": c * (8 + 24 + 8),
         "k_iface_march": c * (8 + 24 + 8),
-        # velocity (3), vof, normal (3), srcb read; srca and the new vof written
-        "k_vof_fused": c * (24 + 8 + 24 + 8 + 8 + 8),
-        # ux uy uz p t, vof, curvature, srcb of 4 fields read; srca and the
-        # increment of 4 fields written
-        "k_rhs_fused": c * (40 + 8 + 8 + 32 + 32 + 32),
-        "k_rhs_march": c * (40 + 8 + 8 + 32 + 32 + 32),
+        # velocity (3), vof, normal (3) read, the new vof written; srcb read in
+        # RK stages 1 and 2, srca written in stages 0 and 1 (the figures are
+        # means over the three stages of a step, whose launches alternate)
+        "k_vof_fused": c * (24 + 8 + 24 + 8) + c * (8 + 8) * 2 / 3,
+        # ux uy uz p t, vof, curvature read and the increment of 4 fields
+        # written; srcb of 4 fields read in stages 1 and 2, srca of 4 fields
+        # written in stages 0 and 1
+        "k_rhs_fused": c * (40 + 8 + 8 + 32) + c * (32 + 32) * 2 / 3,
+        "k_rhs_march": c * (40 + 8 + 8 + 32) + c * (32 + 32) * 2 / 3,
         # psi, ux uy uz p read and written back
         "k_corr_press": c * (8 + 32 + 32),
     }

@@ -245,7 +245,11 @@ int domain_finalise(domain_t * domain);
  * 0 = one launch sequence per reference function (the per-function entry
  *     points), 1 = fused kernels (default).  Both give the same fields; the
  *     fused schedule does not refresh the intermediate arrays dvof, flxx,
- *     flxy, flxz, ifrcx, ifrcy, ifrcz and srcg. */
+ *     flxy, flxz, ifrcx, ifrcy, ifrcz and srcg, does not store the last RK
+ *     stage's explicit sources (srcux[0], srcuy[0], srcuz[0], srct[0],
+ *     interface src[0]: the next step's first stage has beta = 0 and never
+ *     reads them) and skips the beta-weighted source reads of the first
+ *     stage (their weight is 0). */
 int vof_set_schedule(domain_t * domain, int schedule);
 /* stages fused by schedule 1 (bit mask, default 63 = all): 1 curvature
  * tensor, 2 VOF update, 4 sources with the surface force, 8 boundary images

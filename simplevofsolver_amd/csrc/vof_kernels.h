@@ -2872,7 +2872,7 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
                             const double * __restrict__ uz, const double * __restrict__ vof,
                             const double * __restrict__ normal, double * __restrict__ srca,
                             const double * __restrict__ srcb, double * __restrict__ vof_new,
-                            double dta, double dtb, int use_b) {
+                            double dta, double dtb, int use_b, int keep_a) {
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
@@ -2895,7 +2895,7 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
   s += g.rdz * (fzm - fzp);
 #endif
   const size_t is = I_SC(i, j, k);
-  srca[is] = s;
+  if (keep_a) srca[is] = s;
   double v = vof[I_VOF(i, j, k)];
   v += dta * s;
   if (use_b) v += dtb * srcb[is];
@@ -2906,8 +2906,8 @@ __global__ __launch_bounds__(256) void k_vof_fused(Grid g, const double * __rest
                             const double * __restrict__ uz, const double * __restrict__ vof,
                             const double * __restrict__ normal, double * __restrict__ srca,
                             const double * __restrict__ srcb, double * __restrict__ vof_new,
-                            double dta, double dtb, int use_b) {
-  k_vof_fused_body(hw_block(), g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b);
+                            double dta, double dtb, int use_b, int keep_a) {
+  k_vof_fused_body(hw_block(), g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a);
 }
 
 // fluid_predict_field sources of all fields at once, with the CSF force
@@ -2923,6 +2923,10 @@ struct FusedRhs {
   double * srca[4];
   const double * srcb[4];
   double * delta[4];
+  // use_b = 0 (first RK stage, beta = 0): srcb is not read and its term is
+  // b dt * 0.  keep_a = 0 (last RK stage): srca is not stored, as no later
+  // stage reads it (the next step's first stage has beta = 0).
+  int use_b, keep_a;
 };
 
 __device__ __forceinline__ void k_rhs_fused_body(const VBlock & vb, Grid g, FusedRhs f) {
@@ -2968,19 +2972,20 @@ __device__ __forceinline__ void k_rhs_fused_body(const VBlock & vb, Grid g, Fuse
     rhs_t_at(g, a, i, j, k, 0., sa3, sg3);
   }
   const size_t is = I_SC(i, j, k);
+  const bool ub = f.use_b, ka = f.keep_a;
   if (i >= 2) {
     const size_t ix = I_SX(i, j, k);
-    f.srca[0][ix] = sa0;
-    f.delta[0][ix] = +ca * sa0 + cb * f.srcb[0][ix] + cg * sg0;
+    if (ka) f.srca[0][ix] = sa0;
+    f.delta[0][ix] = +ca * sa0 + cb * (ub ? f.srcb[0][ix] : 0.) + cg * sg0;
   }
-  f.srca[1][is] = sa1;
-  f.delta[1][is] = +ca * sa1 + cb * f.srcb[1][is] + cg * sg1;
+  if (ka) f.srca[1][is] = sa1;
+  f.delta[1][is] = +ca * sa1 + cb * (ub ? f.srcb[1][is] : 0.) + cg * sg1;
 #if NDIMS == 3
-  f.srca[2][is] = sa2;
-  f.delta[2][is] = +ca * sa2 + cb * f.srcb[2][is] + cg * sg2;
+  if (ka) f.srca[2][is] = sa2;
+  f.delta[2][is] = +ca * sa2 + cb * (ub ? f.srcb[2][is] : 0.) + cg * sg2;
 #endif
-  f.srca[3][is] = sa3;
-  f.delta[3][is] = +ca * sa3 + cb * f.srcb[3][is] + cg * sg3;
+  if (ka) f.srca[3][is] = sa3;
+  f.delta[3][is] = +ca * sa3 + cb * (ub ? f.srcb[3][is] : 0.) + cg * sg3;
 }
 
 __global__ __launch_bounds__(256) void k_rhs_fused(Grid g, FusedRhs f) {
@@ -3052,11 +3057,16 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
     const double tzm = t[-pc], tzp = t[pc];
     const double pzm = pp[-pc], vzm = vf[-pv], kzm = cu[-pc];
 #endif
-    const double sb1 = f.srcb[1][is], sb3 = f.srcb[3][is];
-    const double sb0 = has_ux ? f.srcb[0][ix] : 0.;
+    double sb0 = 0., sb1 = 0., sb2 = 0., sb3 = 0.;
+    if (f.use_b) {
+      sb1 = f.srcb[1][is];
+      sb3 = f.srcb[3][is];
+      if (has_ux) sb0 = f.srcb[0][ix];
 #if NDIMS == 3
-    const double sb2 = f.srcb[2][is];
+      sb2 = f.srcb[2][is];
 #endif
+    }
+    (void)sb2;
     RhsArgs a = b;
     double sa0 = 0., sg0 = 0., sa1, sg1, sa3, sg3;
     if (has_ux) {
@@ -3122,17 +3132,18 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
       rhs_t_v(g, a, i, s, sa3, sg3);
     }
     const double ca = b.ca, cb = b.cb, cg = b.cg;
+    const bool ka = f.keep_a;
     if (has_ux) {
-      f.srca[0][ix] = sa0;
+      if (ka) f.srca[0][ix] = sa0;
       f.delta[0][ix] = +ca * sa0 + cb * sb0 + cg * sg0;
     }
-    f.srca[1][is] = sa1;
+    if (ka) f.srca[1][is] = sa1;
     f.delta[1][is] = +ca * sa1 + cb * sb1 + cg * sg1;
 #if NDIMS == 3
-    f.srca[2][is] = sa2;
+    if (ka) f.srca[2][is] = sa2;
     f.delta[2][is] = +ca * sa2 + cb * sb2 + cg * sg2;
 #endif
-    f.srca[3][is] = sa3;
+    if (ka) f.srca[3][is] = sa3;
     f.delta[3][is] = +ca * sa3 + cb * sb3 + cg * sg3;
     // shift the carried rows and advance one row
     a0m = a0c; a0c = a0p; axpm = axpc;

@@ -1410,6 +1410,8 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
     fr.vof = vof_old;
     fr.curv = (const double *)interface->curv.data;
     fr.tension = interface->tension;
+    fr.use_b = rkstep != 0;
+    fr.keep_a = rkstep != 2;
     for (int f = 0; f < nj; f++) {
       const int kind = jobs[f].kind;
       fr.lapx[kind] = jobs[f].lapx;
@@ -1878,7 +1880,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
           ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uz,
           (const double *)interface->vof.data, (const double *)interface->normal.data,
           (double *)interface->src[0].data, (const double *)interface->src[1].data,
-          ctx->d_vof_spare, dta, dtb, (int)(rk != 0));
+          ctx->d_vof_spare, dta, dtb, (int)(rk != 0), (int)(rk != 2));
       void * tmp = interface->vof.data;
       interface->vof.data = ctx->d_vof_spare;
       ctx->d_vof_spare = (double *)tmp;
