@@ -1807,17 +1807,19 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
   q[0] = from_d2<T2>(qprev);
   if (store_v) v[0] = vprev;
   {
-    double2 cur[kTsU];
+    // rows are kept in the storage type until used: a conversion right
+    // after a conditional load would wait for that load and serialise them
+    T2 cur[kTsU];
 #pragma unroll
     for (int u = 0; u < kTsU; u++) {
-      if (1 + u < n - 1) cur[u] = to_d2(q[(size_t)(1 + u) * st]);
+      if (1 + u < n - 1) cur[u] = q[(size_t)(1 + u) * st];
     }
     for (int i0 = 1; i0 < n - 1; i0 += kTsU) {
-      double2 nxt[kTsU];
+      T2 nxt[kTsU];
 #pragma unroll
       for (int u = 0; u < kTsU; u++) {
         const int i = i0 + kTsU + u;
-        if (i < n - 1) nxt[u] = to_d2(q[(size_t)i * st]);
+        if (i < n - 1) nxt[u] = q[(size_t)i * st];
       }
 #pragma unroll
       for (int u = 0; u < kTsU; u++) {
@@ -1826,7 +1828,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
           const double c = -pl[i] - pu[i] + eval;
           const double val = 1. / (c - pl[i] * vprev);
           const double vi = val * (pu[i]);
-          double2 qi = cur[u];
+          double2 qi = to_d2(cur[u]);
           qi.x = val * (qi.x - pl[i] * qprev.x);
           qi.y = val * (qi.y - pl[i] * qprev.y);
           q[(size_t)i * st] = from_d2<T2>(qi);
@@ -1862,24 +1864,24 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
     qprev = qi;
   }
   {
-    double2 cq[kTsU];
+    T2 cq[kTsU];
     double cv[kTsU];
 #pragma unroll
     for (int u = 0; u < kTsU; u++) {
       const int i = n - 2 - u;
       if (i >= 0) {
-        cq[u] = to_d2(q[(size_t)i * st]);
+        cq[u] = q[(size_t)i * st];
         cv[u] = v[(size_t)i * st];
       }
     }
     for (int i0 = n - 2; i0 >= 0; i0 -= kTsU) {
-      double2 nq[kTsU];
+      T2 nq[kTsU];
       double nv[kTsU];
 #pragma unroll
       for (int u = 0; u < kTsU; u++) {
         const int i = i0 - kTsU - u;
         if (i >= 0) {
-          nq[u] = to_d2(q[(size_t)i * st]);
+          nq[u] = q[(size_t)i * st];
           nv[u] = v[(size_t)i * st];
         }
       }
@@ -1887,7 +1889,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
       for (int u = 0; u < kTsU; u++) {
         const int i = i0 - u;
         if (i >= 0) {
-          double2 qi = cq[u];
+          double2 qi = to_d2(cq[u]);
           qi.x -= cv[u] * qprev.x;
           qi.y -= cv[u] * qprev.y;
           q[(size_t)i * st] = from_d2<T2>(qi);
