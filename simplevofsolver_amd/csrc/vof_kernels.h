@@ -1518,7 +1518,9 @@ constexpr int kFftLoadU = 4;      // elements per thread whose global loads are 
 // w = exp(sign 2 pi i / 16), n = a + 4 b, k = kb + 4 ka; X[k] is left in
 // v[dft16_slot(k)] = v[4 kb + ka]
 __device__ inline int dft16_slot(int k) { return 4 * (k & 3) + (k >> 2); }
-__device__ __forceinline__ void dft16(double2 * v, double sign) {
+template <class T2>
+__device__ __forceinline__ void dft16(T2 * v, typename Cx<T2>::S sign) {
+  using S = typename Cx<T2>::S;
   // cos, sin of 2 pi m / 16, m = 0..15
   constexpr double kC[16] = {1., 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
                              0., -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
@@ -1531,24 +1533,24 @@ __device__ __forceinline__ void dft16(double2 * v, double sign) {
   // DFT4 over b of v[a + 4 b], twiddled by w^(a kb), back into v[a + 4 kb]
 #pragma unroll
   for (int a = 0; a < 4; a++) {
-    double2 u[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
-    butterfly<4, double2>(u, sign);
+    T2 u[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+    butterfly<4, T2>(u, sign);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
       const int m = a * kb;
       if (m == 0) {
         v[a + 4 * kb] = u[kb];
       } else {
-        const double wr = kC[m], wi = sign * kS[m];
-        v[a + 4 * kb] = make_double2(u[kb].x * wr - u[kb].y * wi, u[kb].x * wi + u[kb].y * wr);
+        const S wr = (S)kC[m], wi = sign * (S)kS[m];
+        v[a + 4 * kb] = Cx<T2>::mk(u[kb].x * wr - u[kb].y * wi, u[kb].x * wi + u[kb].y * wr);
       }
     }
   }
   // DFT4 over a of v[a + 4 kb] -> X[kb + 4 ka], left in v[4 kb + ka]
 #pragma unroll
   for (int kb = 0; kb < 4; kb++) {
-    double2 u[4] = {v[4 * kb], v[4 * kb + 1], v[4 * kb + 2], v[4 * kb + 3]};
-    butterfly<4, double2>(u, sign);
+    T2 u[4] = {v[4 * kb], v[4 * kb + 1], v[4 * kb + 2], v[4 * kb + 3]};
+    butterfly<4, T2>(u, sign);
 #pragma unroll
     for (int ka = 0; ka < 4; ka++) v[4 * kb + ka] = u[ka];
   }
@@ -1568,32 +1570,34 @@ constexpr int kF256Stride = 272;  // complex values per column
 __device__ inline int f256_in(int n) { return (n >> 4) * 17 + (n & 15); }
 __device__ inline int f256_out(int k) { return (k & 15) * 17 + (k >> 4); }
 
-__device__ __forceinline__ void fft256(double2 * buf, int ncols, const double2 * __restrict__ tw, double sign) {
+template <class T2>
+__device__ __forceinline__ void fft256(T2 * buf, int ncols, const T2 * __restrict__ tw, typename Cx<T2>::S sign) {
+  using S = typename Cx<T2>::S;
   for (int t = threadIdx.x; t < ncols * 16; t += blockDim.x) {
-    double2 * col = buf + (t >> 4) * kF256Stride;
+    T2 * col = buf + (t >> 4) * kF256Stride;
     const int n2 = t & 15;
-    double2 v[16];
+    T2 v[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; n1++) v[n1] = col[n1 * 17 + n2];
-    dft16(v, sign);
+    dft16<T2>(v, sign);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) {
-      double2 x = v[dft16_slot(k1)];
+      T2 x = v[dft16_slot(k1)];
       if (k1 != 0 && n2 != 0) {
-        const double2 w = tw[n2 * k1];
-        const double ws = sign * w.y;
-        x = make_double2(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
+        const T2 w = tw[n2 * k1];
+        const S ws = sign * w.y;
+        x = Cx<T2>::mk(x.x * w.x - x.y * ws, x.x * ws + x.y * w.x);
       }
       col[k1 * 17 + n2] = x;
     }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < ncols * 16; t += blockDim.x) {
-    double2 * row = buf + (t >> 4) * kF256Stride + (t & 15) * 17;
-    double2 v[16];
+    T2 * row = buf + (t >> 4) * kF256Stride + (t & 15) * 17;
+    T2 v[16];
 #pragma unroll
     for (int n2 = 0; n2 < 16; n2++) v[n2] = row[n2];
-    dft16(v, sign);
+    dft16<T2>(v, sign);
 #pragma unroll
     for (int k2 = 0; k2 < 16; k2++) row[k2] = v[dft16_slot(k2)];
   }
@@ -1604,15 +1608,15 @@ __device__ __forceinline__ void fft256(double2 * buf, int ncols, const double2 *
 // path (one padded buffer), else the mixed-radix Stockham path (two buffers)
 template <bool F256> __device__ inline int fft_in(int n) { return F256 ? f256_in(n) : n; }
 template <bool F256> __device__ inline int fft_out(int k) { return F256 ? f256_out(k) : k; }
-template <bool F256>
-__device__ __forceinline__ double2 * fft_cols(double2 * a, double2 * b, int ncols, const FftPlan & P,
-                                     const double2 * __restrict__ tw, double sign) {
+template <bool F256, class T2>
+__device__ __forceinline__ T2 * fft_cols(T2 * a, T2 * b, int ncols, const FftPlan & P,
+                                     const T2 * __restrict__ tw, typename Cx<T2>::S sign) {
   if constexpr (F256) {
     (void)b; (void)P;
-    fft256(a, ncols, tw, sign);
+    fft256<T2>(a, ncols, tw, sign);
     return a;
   } else {
-    return lds_fft<double2>(a, b, ncols, P, tw, sign);
+    return lds_fft<T2>(a, b, ncols, P, tw, sign);
   }
 }
 
@@ -1682,7 +1686,7 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
     }
   }
   __syncthreads();
-  double2 * res = fft_cols<F256>(sa, sb, NC, P, tw, -1.);
+  double2 * res = fft_cols<F256, double2>(sa, sb, NC, P, tw, -1.);
   const int nh = N / 2 + 1;
   for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
     const int c = e / nh, ky = e % nh;
@@ -1738,12 +1742,12 @@ __device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, const Ff
   __syncthreads();
   T2 * res;
   if constexpr (F256) {
-    static_assert(!F256 || sizeof(T2) == sizeof(double2), "256-point path is FP64");
+    using S = typename Cx<T2>::S;
     // the direction as a constant, so the twiddle signs fold
     if (sign < 0)
-      res = fft_cols<true>(sa, sb, NC, P, tw, -1.);
+      res = fft_cols<true, T2>(sa, sb, NC, P, tw, (S)-1);
     else
-      res = fft_cols<true>(sa, sb, NC, P, tw, +1.);
+      res = fft_cols<true, T2>(sa, sb, NC, P, tw, (S)+1);
   } else {
     res = lds_fft<T2>(sa, sb, NC, P, tw, sign);
   }
@@ -1965,7 +1969,7 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
     }
   }
   __syncthreads();
-  double2 * res = fft_cols<F256>(sa, sb, NC, P, tw, +1.);
+  double2 * res = fft_cols<F256, double2>(sa, sb, NC, P, tw, +1.);
   for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
     const int xl = e % (2 * NC), j = e / (2 * NC);
     const int i = x0 + xl + 1;
@@ -2001,18 +2005,21 @@ __global__ __launch_bounds__(kFftThreads) void k_irfft_y_psi(
 // Neumann walls in x).  M includes the 1/(ny nz) of the unnormalised
 // transforms.
 
-// mode 0: r = prefactor * div u, stored in rhs; mode 1: r = rhs - A psi.
-// Then (float)(r * scale) is transformed in y as k_div_rfft_y does.
-template <int ELEMS>
+// mode 0: r = prefactor * div u, stored in rhs when store_rhs (a refinement
+// pass follows); mode 1: r = rhs - A psi.  Then (float)(r * scale) is
+// transformed in y as k_div_rfft_y does.
+template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb, 
     Grid g, const FftPlan & P, const float2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
     double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
-    const double * __restrict__ pu, int mode, float2 * __restrict__ spec) {
-  __shared__ float2 sa[ELEMS], sb[ELEMS];
+    const double * __restrict__ pu, int mode, int store_rhs, float2 * __restrict__ spec) {
+  __shared__ float2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ float2 sb[F256 ? 1 : ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = ELEMS / N;
+  const int NC = F256 ? kF256Cols : ELEMS / N;
+  const int CS = F256 ? kF256Stride : N;
   const Tile3 tile = xcd_tile(vb);
   const int x0 = tile.x * 2 * NC;
 #if NDIMS == 2
@@ -2027,45 +2034,84 @@ __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb,
 #if NDIMS == 3
   const double rdz2 = 1. / (g.dz * g.dz);
 #endif
-  for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
-    const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
-    const int i = x0 + xl + 1;
-    float r32 = 0.f;
-    if (i <= nx) {
-      const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
-      double r;
-      if (mode == 0) {
-        const double dx = g.dxf[i - 1];
-        double div = (ux[I_UX(i + 1, j, k)] - ux[I_UX(i, j, k)]) / dx +
-                     (uy[I_C1(i, j + 1, k)] - uy[I_C1(i, j, k)]) / g.dy;
+  // kFftLoadU elements per thread have their loads in flight together
+  const int total = 2 * NC * N;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    // mode 0: ux(i+1) ux(i) uy(j+1) uy(j) uz(k+1) uz(k); mode 1: psi at the
+    // centre, x-, x+, y-, y+, z-, z+ and rhs
+    double a[8][kFftLoadU];
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+      const int i = x0 + xl + 1;
+      if (e < total && i <= nx) {
+        if (mode == 0) {
+          a[0][u] = ux[I_UX(i + 1, j, k)];
+          a[1][u] = ux[I_UX(i, j, k)];
+          a[2][u] = uy[I_C1(i, j + 1, k)];
+          a[3][u] = uy[I_C1(i, j, k)];
 #if NDIMS == 3
-        div += (uz[I_C1(i, j, k + 1)] - uz[I_C1(i, j, k)]) / g.dz;
+          a[4][u] = uz[I_C1(i, j, k + 1)];
+          a[5][u] = uz[I_C1(i, j, k)];
 #endif
-        r = prefactor * div;
-        rhs[ir] = r;
-      } else {
-        const double pc = psi[I_C1(i, j, k)];
-        // psi(0) = psi(1) and psi(nx+1) = psi(nx) (Neumann images) fold the
-        // wall rows of the x operator
-        double ap = pl[i - 1] * (psi[I_C1(i - 1, j, k)] - pc) + pu[i - 1] * (psi[I_C1(i + 1, j, k)] - pc);
-        ap += rdy2 * (psi[I_C1(i, j - 1, k)] - 2. * pc + psi[I_C1(i, j + 1, k)]);
+        } else {
+          const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+          a[0][u] = psi[I_C1(i, j, k)];
+          a[1][u] = psi[I_C1(i - 1, j, k)];
+          a[2][u] = psi[I_C1(i + 1, j, k)];
+          a[3][u] = psi[I_C1(i, j - 1, k)];
+          a[4][u] = psi[I_C1(i, j + 1, k)];
 #if NDIMS == 3
-        ap += rdz2 * (psi[I_C1(i, j, k - 1)] - 2. * pc + psi[I_C1(i, j, k + 1)]);
+          a[5][u] = psi[I_C1(i, j, k - 1)];
+          a[6][u] = psi[I_C1(i, j, k + 1)];
 #endif
-        r = rhs[ir] - ap;
+          a[7][u] = rhs[ir];
+        }
       }
-      r32 = (float)(r * scale);
     }
-    float * slot = reinterpret_cast<float *>(&sa[(xl >> 1) * N + (j - 1)]);
-    slot[xl & 1] = r32;
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e >= total) break;
+      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
+      const int i = x0 + xl + 1;
+      float r32 = 0.f;
+      if (i <= nx) {
+        const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+        double r;
+        if (mode == 0) {
+          const double dx = g.dxf[i - 1];
+          double div = (a[0][u] - a[1][u]) / dx + (a[2][u] - a[3][u]) / g.dy;
+#if NDIMS == 3
+          div += (a[4][u] - a[5][u]) / g.dz;
+#endif
+          r = prefactor * div;
+          if (store_rhs) rhs[ir] = r;
+        } else {
+          const double pc = a[0][u];
+          // psi(0) = psi(1) and psi(nx+1) = psi(nx) (Neumann images) fold the
+          // wall rows of the x operator
+          double ap = pl[i - 1] * (a[1][u] - pc) + pu[i - 1] * (a[2][u] - pc);
+          ap += rdy2 * (a[3][u] - 2. * pc + a[4][u]);
+#if NDIMS == 3
+          ap += rdz2 * (a[5][u] - 2. * pc + a[6][u]);
+#endif
+          r = a[7][u] - ap;
+        }
+        r32 = (float)(r * scale);
+      }
+      float * slot = reinterpret_cast<float *>(&sa[(xl >> 1) * CS + fft_in<F256>(j - 1)]);
+      slot[xl & 1] = r32;
+    }
   }
   __syncthreads();
-  float2 * res = lds_fft<float2>(sa, sb, NC, P, tw, -1.f);
+  float2 * res = fft_cols<F256, float2>(sa, sb, NC, P, tw, -1.f);
   const int nh = N / 2 + 1;
   for (int e = threadIdx.x; e < NC * nh; e += blockDim.x) {
     const int c = e / nh, ky = e % nh;
-    const float2 z = res[c * N + ky];
-    const float2 zc = res[c * N + (N - ky) % N];
+    const float2 z = res[c * CS + fft_out<F256>(ky)];
+    const float2 zc = res[c * CS + fft_out<F256>((N - ky) % N)];
     const float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y - zc.y));
     const float2 B = make_float2(0.5f * (z.y + zc.y), -0.5f * (z.x - zc.x));
     const int xa = x0 + 2 * c, xb = xa + 1;
@@ -2074,25 +2120,28 @@ __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb,
   }
 }
 
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __global__ __launch_bounds__(kFftThreads) void k_res_rfft32_y(
     Grid g, FftPlan P, const float2 * __restrict__ tw, const double * __restrict__ ux,
     const double * __restrict__ uy, const double * __restrict__ uz, double prefactor, double scale,
     double * __restrict__ rhs, const double * __restrict__ psi, const double * __restrict__ pl,
-    const double * __restrict__ pu, int mode, float2 * __restrict__ spec) {
-  k_res_rfft32_y_body<ELEMS>(hw_block(), g, P, tw, ux, uy, uz, prefactor, scale, rhs, psi, pl, pu, mode, spec);
+    const double * __restrict__ pu, int mode, int store_rhs, float2 * __restrict__ spec) {
+  k_res_rfft32_y_body<ELEMS, F256>(hw_block(), g, P, tw, ux, uy, uz, prefactor, scale, rhs, psi, pl, pu, mode,
+                                   store_rhs, spec);
 }
 
 // inverse real transform in y in FP32; psi = delta (accumulate 0) or
 // psi += delta (accumulate 1) in FP64, with halo images and Neumann walls
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb, 
     Grid g, const FftPlan & P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
     double * __restrict__ psi, int accumulate) {
-  __shared__ float2 sa[ELEMS], sb[ELEMS];
+  __shared__ float2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ float2 sb[F256 ? 1 : ELEMS];
   const int nx = g.nx, ny = g.ny;
   const int N = ny;
-  const int NC = ELEMS / N;
+  const int NC = F256 ? kF256Cols : ELEMS / N;
+  const int CS = F256 ? kF256Stride : N;
   const Tile3 tile = xcd_tile(vb);
   const int x0 = tile.x * 2 * NC;
   const int nh = N / 2 + 1;
@@ -2103,26 +2152,41 @@ __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb,
   const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
-  for (int e = threadIdx.x; e < NC * N; e += blockDim.x) {
-    const int c = e / N, ky = e % N;
-    const int xa = x0 + 2 * c, xb = xa + 1;
-    const int kk = ky <= N / 2 ? ky : N - ky;
-    const float cs = ky <= N / 2 ? 1.f : -1.f;
-    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-    if (xa < nx) A = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
-    if (xb < nx) B = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
-    if (kk == 0 || 2 * kk == N) { A.y = 0.f; B.y = 0.f; }
-    A.y *= cs;
-    B.y *= cs;
-    sa[c * N + ky] = make_float2(A.x - B.y, A.y + B.x);
+  const int total = NC * N;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    float2 As[kFftLoadU], Bs[kFftLoadU];
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      const int c = e / N, ky = e % N;
+      const int xa = x0 + 2 * c, xb = xa + 1;
+      const int kk = ky <= N / 2 ? ky : N - ky;
+      As[u] = make_float2(0.f, 0.f);
+      Bs[u] = make_float2(0.f, 0.f);
+      if (e < total && xa < nx) As[u] = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
+      if (e < total && xb < nx) Bs[u] = spec[((size_t)xb * nz + (k - 1)) * nh + kk];
+    }
+#pragma unroll
+    for (int u = 0; u < kFftLoadU; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e >= total) break;
+      const int c = e / N, ky = e % N;
+      const int kk = ky <= N / 2 ? ky : N - ky;
+      const float cs = ky <= N / 2 ? 1.f : -1.f;
+      float2 A = As[u], B = Bs[u];
+      if (kk == 0 || 2 * kk == N) { A.y = 0.f; B.y = 0.f; }
+      A.y *= cs;
+      B.y *= cs;
+      sa[c * CS + fft_in<F256>(ky)] = make_float2(A.x - B.y, A.y + B.x);
+    }
   }
   __syncthreads();
-  float2 * res = lds_fft<float2>(sa, sb, NC, P, tw, +1.f);
+  float2 * res = fft_cols<F256, float2>(sa, sb, NC, P, tw, +1.f);
   for (int e = threadIdx.x; e < 2 * NC * N; e += blockDim.x) {
     const int xl = e % (2 * NC), j = e / (2 * NC);
     const int i = x0 + xl + 1;
     if (i <= nx) {
-      const float2 z = res[(xl >> 1) * N + j];
+      const float2 z = res[(xl >> 1) * CS + fft_out<F256>(j)];
       double v = (double)((xl & 1) ? z.y : z.x);
       if (accumulate) v = psi[I_C1(i, j + 1, k)] + v;
       const int pln = NDIMS == 3 ? k : 0;
@@ -2132,11 +2196,11 @@ __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb,
   }
 }
 
-template <int ELEMS>
+template <int ELEMS, bool F256 = false>
 __global__ __launch_bounds__(kFftThreads) void k_irfft32_y_psi(
     Grid g, FftPlan P, const float2 * __restrict__ tw, const float2 * __restrict__ spec,
     double * __restrict__ psi, int accumulate) {
-  k_irfft32_y_psi_body<ELEMS>(hw_block(), g, P, tw, spec, psi, accumulate);
+  k_irfft32_y_psi_body<ELEMS, F256>(hw_block(), g, P, tw, spec, psi, accumulate);
 }
 
 // ---------------------------------------------------------------------------

@@ -1522,11 +1522,15 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
   const double prefactor = 1. / (kRk[rkstep][2] * dt);
   const double scale = 1. / ((double)ny * (NDIMS == 3 ? (double)nz : 1.));
   const int elems = 2048;
-  const int NC = elems / ny;
+  // 256-point transforms take the register radix-16 path, as in potential()
+  const bool f256y = ny == 256;
+  const int NC = f256y ? vofk::kF256Cols : elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
+  const dim3 gy(nbx, NDIMS == 3 ? nz : 1), bf(vofk::kFftThreads);
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
-  const int NCz = elems / nz;
+  const bool f256z = nz == 256;
+  const int NCz = f256z ? vofk::kF256Cols : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
 #else
   const double * uz = nullptr;
@@ -1534,23 +1538,50 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
   double * psi = (double *)fluid->psi.data;
   const int nlines = ctx->nh * nz;
   for (int pass = 0; pass <= ctx->poisson_refine; pass++) {
-    KLB(ctx, "k_res_rfft32_y", vofk::kFftThreads, vofk::k_res_rfft32_y_body<2048>, vofk::k_res_rfft32_y<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy32, (const double *)fluid->ux.data,
-        (const double *)fluid->uy.data, uz, prefactor, scale, ctx->d_delta, (const double *)psi, ctx->d_pl,
-        ctx->d_pu, pass == 0 ? 0 : 1, ctx->d_spec32);
+    // the FP64 right-hand side is kept only when a refinement pass reads it
+    const int store_rhs = ctx->poisson_refine > 0;
+#define RES_ARGS ctx->grid, ctx->plan_y, ctx->d_twy32, (const double *)fluid->ux.data, (const double *)fluid->uy.data, uz, \
+                 prefactor, scale, ctx->d_delta, (const double *)psi, ctx->d_pl, ctx->d_pu, pass == 0 ? 0 : 1, store_rhs, \
+                 ctx->d_spec32
+    if (f256y) {
+      KLB(ctx, "k_res_rfft32_y", vofk::kFftThreads, (vofk::k_res_rfft32_y_body<2048, true>),
+          (vofk::k_res_rfft32_y<2048, true>), gy, bf, 0, RES_ARGS);
+    } else {
+      KLB(ctx, "k_res_rfft32_y", vofk::kFftThreads, vofk::k_res_rfft32_y_body<2048>, vofk::k_res_rfft32_y<2048>, gy, bf,
+          0, RES_ARGS);
+    }
+#undef RES_ARGS
 #if NDIMS == 3
-    KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z,
-        ctx->d_twz32, ctx->d_spec32, ctx->nh, -1.f);
+    for (int zp = 0; zp < 2; zp++) {
+      if (zp == 1) {
+        KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>,
+            dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez,
+            ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
+      }
+      const float sign = zp == 0 ? -1.f : +1.f;
+      if (f256z) {
+        KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, float2, true>),
+            (vofk::k_fft_z<1024, float2, true>), gz, bf, 0, ctx->grid, ctx->plan_z, ctx->d_twz32, ctx->d_spec32,
+            ctx->nh, sign);
+      } else {
+        KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz,
+            bf, 0, ctx->grid, ctx->plan_z, ctx->d_twz32, ctx->d_spec32, ctx->nh, sign);
+      }
+    }
+#else
+    KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>,
+        dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez,
+        ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
 #endif
-    KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>, dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines,
-        ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
-#if NDIMS == 3
-    KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<2048, float2>), (vofk::k_fft_z<2048, float2>), gz, dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_z,
-        ctx->d_twz32, ctx->d_spec32, ctx->nh, +1.f);
-#endif
-    KLB(ctx, "k_irfft32_y_psi", vofk::kFftThreads, vofk::k_irfft32_y_psi_body<2048>, vofk::k_irfft32_y_psi<2048>, dim3(nbx, NDIMS == 3 ? nz : 1),
-        dim3(vofk::kFftThreads), 0, ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi,
-        pass == 0 ? 0 : 1);
+#define IRF32_ARGS ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi, pass == 0 ? 0 : 1
+    if (f256y) {
+      KLB(ctx, "k_irfft32_y_psi", vofk::kFftThreads, (vofk::k_irfft32_y_psi_body<2048, true>),
+          (vofk::k_irfft32_y_psi<2048, true>), gy, bf, 0, IRF32_ARGS);
+    } else {
+      KLB(ctx, "k_irfft32_y_psi", vofk::kFftThreads, vofk::k_irfft32_y_psi_body<2048>, vofk::k_irfft32_y_psi<2048>, gy,
+          bf, 0, IRF32_ARGS);
+    }
+#undef IRF32_ARGS
   }
   return 0;
 }

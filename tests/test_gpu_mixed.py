@@ -20,6 +20,10 @@ CASES = [
     ((24, 40), False, 52),
     ((16, 12, 20), True, 53),
     ((12, 16, 8), False, 54),
+    # 256-point transforms (register radix-16 path) in y, y only, z only
+    ((16, 256), True, 55),
+    ((8, 256, 6), False, 56),
+    ((6, 8, 256), True, 57),
 ]
 
 
