@@ -108,10 +108,10 @@ def cpu_baseline(ic, steps: int):
                       f"on one host core ({el:.1f} s)"}
 
 
-def mixed_form(g, cells, steps, plain_value):
-    """The same solver with the FP32-transform Poisson solve and one FP64
-    refinement, timed over `steps` steps."""
-    g.set_poisson_mixed(1)
+def mixed_form(g, cells, steps, plain_value, refine):
+    """The same solver with the FP32-transform Poisson solve followed by
+    `refine` FP64 residual refinements, timed over `steps` steps."""
+    g.set_poisson_mixed(refine)
     g.integrate()
     g.synchronize()
     t0 = time.perf_counter()
@@ -121,11 +121,15 @@ def mixed_form(g, cells, steps, plain_value):
     el = time.perf_counter() - t0
     g.set_poisson_mixed(-1)
     v = cells * steps / el / 1e6
-    print(f"[bench] mixed-precision form {v:.1f} Mcell-steps/s ({el / steps * 1e3:.3f} ms/step)",
-          file=sys.stderr, flush=True)
+    print(f"[bench] mixed-precision form, {refine} refinement(s): {v:.1f} Mcell-steps/s "
+          f"({el / steps * 1e3:.3f} ms/step)", file=sys.stderr, flush=True)
+    # accuracy of the Poisson solution, from the GPU tests against the FP64
+    # oracle (tests/test_gpu_mixed.py bars)
+    bar = {0: "psi within 1e-4 relative (FP32 solve alone)", 1: "psi within 1e-8 relative",
+           2: "psi within 1e-10 relative"}.get(refine, "psi within 1e-10 relative")
     return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
-            "vs_plain": round(v / plain_value, 4),
-            "workload": "same 3D 256^3 case, FP32 transforms + FP64 x eliminations, 1 FP64 refinement"}
+            "vs_plain": round(v / plain_value, 4), "refinements": refine, "accuracy": bar,
+            "workload": f"same 3D 256^3 case, FP32 transforms + FP64 x eliminations, {refine} FP64 refinement(s)"}
 
 
 def batched_form(sva, ncases, steps):
@@ -338,7 +342,8 @@ def main(argv=None) -> int:
     forms = {}
     if not args.no_forms and args.poisson_mixed < 0:
         try:
-            forms["mixed_precision"] = mixed_form(g, cells, args.steps, value)
+            forms["mixed_precision"] = mixed_form(g, cells, args.steps, value, 1)
+            forms["mixed_precision_fp32_solve"] = mixed_form(g, cells, args.steps, value, 0)
         except Exception as e:  # reported, never fatal for the plain line
             forms["mixed_precision"] = {"error": repr(e)}
     if args.kernels:

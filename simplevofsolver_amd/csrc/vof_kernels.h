@@ -1649,22 +1649,29 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
   // kFftLoadU elements per thread have their loads in flight together
   const int total = 2 * NC * N;
   for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
+    // Every load of an element is issued in this loop, unconditionally
+    // (indices past the grid are clamped to it and their values unused), so
+    // all of them are in flight together: a load in the second loop, or
+    // loads under a divergent branch, made each element wait for the others.
     double xm[kFftLoadU], xp[kFftLoadU], ym[kFftLoadU], yp[kFftLoadU], zm[kFftLoadU], zp[kFftLoadU];
+    double dxv[kFftLoadU];
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
-      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
-      const int i = x0 + xl + 1;
-      if (e < total && i <= nx) {
-        xm[u] = ux[I_UX(i, j, k)];
-        xp[u] = ux[I_UX(i + 1, j, k)];
-        ym[u] = uy[I_C1(i, j, k)];
-        yp[u] = uy[I_C1(i, j + 1, k)];
+      const int xl = e % (2 * NC);
+      const int j = min(e / (2 * NC) + 1, ny);
+      const int i = min(x0 + xl + 1, nx);
+      dxv[u] = g.dxf[i - 1];
+      xm[u] = ux[I_UX(i, j, k)];
+      xp[u] = ux[I_UX(i + 1, j, k)];
+      ym[u] = uy[I_C1(i, j, k)];
+      yp[u] = uy[I_C1(i, j + 1, k)];
 #if NDIMS == 3
-        zm[u] = uz[I_C1(i, j, k)];
-        zp[u] = uz[I_C1(i, j, k + 1)];
+      zm[u] = uz[I_C1(i, j, k)];
+      zp[u] = uz[I_C1(i, j, k + 1)];
+#else
+      zm[u] = zp[u] = 0.;
 #endif
-      }
     }
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
@@ -1674,7 +1681,7 @@ __device__ __forceinline__ void k_div_rfft_y_body(const VBlock & vb,
       const int i = x0 + xl + 1;
       double r = 0.;
       if (i <= nx) {
-        const double dx = g.dxf[i - 1];
+        const double dx = dxv[u];
 #if NDIMS == 2
         r = prefactor * ((xp[u] - xm[u]) / dx + (yp[u] - ym[u]) / g.dy);
 #else
@@ -2037,37 +2044,41 @@ __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb,
   // kFftLoadU elements per thread have their loads in flight together
   const int total = 2 * NC * N;
   for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
-    // mode 0: ux(i+1) ux(i) uy(j+1) uy(j) uz(k+1) uz(k); mode 1: psi at the
-    // centre, x-, x+, y-, y+, z-, z+ and rhs
+    // mode 0: ux(i+1) ux(i) uy(j+1) uy(j) uz(k+1) uz(k), DXF(i); mode 1: psi
+    // at the centre, x-, x+, y-, y+, z-, z+ and rhs (every load of an element
+    // in the first loop, so all of them are in flight together)
     double a[8][kFftLoadU];
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
+      // indices past the grid are clamped to it (values unused), so the
+      // loads need no divergent branch
       const int e = e0 + u * (int)blockDim.x;
-      const int xl = e % (2 * NC), j = e / (2 * NC) + 1;
-      const int i = x0 + xl + 1;
-      if (e < total && i <= nx) {
-        if (mode == 0) {
-          a[0][u] = ux[I_UX(i + 1, j, k)];
-          a[1][u] = ux[I_UX(i, j, k)];
-          a[2][u] = uy[I_C1(i, j + 1, k)];
-          a[3][u] = uy[I_C1(i, j, k)];
+      const int xl = e % (2 * NC);
+      const int j = min(e / (2 * NC) + 1, ny);
+      const int i = min(x0 + xl + 1, nx);
+      a[4][u] = a[5][u] = a[6][u] = 0.;
+      if (mode == 0) {
+        a[7][u] = g.dxf[i - 1];
+        a[0][u] = ux[I_UX(i + 1, j, k)];
+        a[1][u] = ux[I_UX(i, j, k)];
+        a[2][u] = uy[I_C1(i, j + 1, k)];
+        a[3][u] = uy[I_C1(i, j, k)];
 #if NDIMS == 3
-          a[4][u] = uz[I_C1(i, j, k + 1)];
-          a[5][u] = uz[I_C1(i, j, k)];
+        a[4][u] = uz[I_C1(i, j, k + 1)];
+        a[5][u] = uz[I_C1(i, j, k)];
 #endif
-        } else {
-          const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
-          a[0][u] = psi[I_C1(i, j, k)];
-          a[1][u] = psi[I_C1(i - 1, j, k)];
-          a[2][u] = psi[I_C1(i + 1, j, k)];
-          a[3][u] = psi[I_C1(i, j - 1, k)];
-          a[4][u] = psi[I_C1(i, j + 1, k)];
+      } else {
+        const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+        a[0][u] = psi[I_C1(i, j, k)];
+        a[1][u] = psi[I_C1(i - 1, j, k)];
+        a[2][u] = psi[I_C1(i + 1, j, k)];
+        a[3][u] = psi[I_C1(i, j - 1, k)];
+        a[4][u] = psi[I_C1(i, j + 1, k)];
 #if NDIMS == 3
-          a[5][u] = psi[I_C1(i, j, k - 1)];
-          a[6][u] = psi[I_C1(i, j, k + 1)];
+        a[5][u] = psi[I_C1(i, j, k - 1)];
+        a[6][u] = psi[I_C1(i, j, k + 1)];
 #endif
-          a[7][u] = rhs[ir];
-        }
+        a[7][u] = rhs[ir];
       }
     }
 #pragma unroll
@@ -2081,7 +2092,7 @@ __device__ __forceinline__ void k_res_rfft32_y_body(const VBlock & vb,
         const size_t ir = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
         double r;
         if (mode == 0) {
-          const double dx = g.dxf[i - 1];
+          const double dx = a[7][u];
           double div = (a[0][u] - a[1][u]) / dx + (a[2][u] - a[3][u]) / g.dy;
 #if NDIMS == 3
           div += (a[4][u] - a[5][u]) / g.dz;
