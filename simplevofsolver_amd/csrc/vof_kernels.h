@@ -2959,23 +2959,39 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
   const int nz = 1;
   (void)uz;
 #endif
-  const double fxm = i == 1 ? 0. : flux_x_at(ux, vof, normal, nx, ny, i, j, k);
-  const double fxp = i == nx ? 0. : flux_x_at(ux, vof, normal, nx, ny, i + 1, j, k);
-  const double fym = flux_y_at(uy, vof, normal, nx, ny, i, j, k);
-  const double fyp = flux_y_at(uy, vof, normal, nx, ny, i, j + 1, k);
+  // the six face velocities, the seven vof values and srcb are loaded before
+  // any flux is formed, so they are in flight together (the upwind choices
+  // branch; the walls' velocities at i = 1 and the face past i = nx are loaded
+  // and unused); only interface cells read normals, inside face_flux
+  constexpr int NC = NDIMS + 1;
+  const size_t is = I_SC(i, j, k);
+  const double uxm = ux[I_UX(i, j, k)], uxp = ux[I_UX(i + 1, j, k)];
+  const double uym = uy[I_C1(i, j, k)], uyp = uy[I_C1(i, j + 1, k)];
+  const double vc = vof[I_VOF(i, j, k)];
+  const double vxm = vof[I_VOF(i - 1, j, k)], vxp = vof[I_VOF(i + 1, j, k)];
+  const double vym = vof[I_VOF(i, j - 1, k)], vyp = vof[I_VOF(i, j + 1, k)];
+  const double sb = use_b ? srcb[is] : 0.;
+#if NDIMS == 3
+  const double uzm = uz[I_C1(i, j, k)], uzp = uz[I_C1(i, j, k + 1)];
+  const double vzm = vof[I_VOF(i, j, k - 1)], vzp = vof[I_VOF(i, j, k + 1)];
+#endif
+  const double * nc = normal + NC * I_NRM(i, j, k);
+  const double fxm = i == 1 ? 0. : face_flux<0>(uxm, vxm, vc, normal + NC * I_NRM(i - 1, j, k), nc);
+  const double fxp = i == nx ? 0. : face_flux<0>(uxp, vc, vxp, nc, normal + NC * I_NRM(i + 1, j, k));
+  const double fym = face_flux<1>(uym, vym, vc, normal + NC * I_NRM(i, j - 1, k), nc);
+  const double fyp = face_flux<1>(uyp, vc, vyp, nc, normal + NC * I_NRM(i, j + 1, k));
   double s = 0.;
   s += g.xt[kXT * i + XT_RXF] * (fxm - fxp);   // 1 / DXF(i)
   s += g.rdy * (fym - fyp);
 #if NDIMS == 3
-  const double fzm = flux_z_at(uz, vof, normal, nx, ny, i, j, k);
-  const double fzp = flux_z_at(uz, vof, normal, nx, ny, i, j, k + 1);
+  const double fzm = face_flux<2>(uzm, vzm, vc, normal + NC * I_NRM(i, j, k - 1), nc);
+  const double fzp = face_flux<2>(uzp, vc, vzp, nc, normal + NC * I_NRM(i, j, k + 1));
   s += g.rdz * (fzm - fzp);
 #endif
-  const size_t is = I_SC(i, j, k);
   if (keep_a) srca[is] = s;
-  double v = vof[I_VOF(i, j, k)];
+  double v = vc;
   v += dta * s;
-  if (use_b) v += dtb * srcb[is];
+  if (use_b) v += dtb * sb;
   store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
 }
 
@@ -3259,30 +3275,40 @@ __device__ __forceinline__ void k_corr_press_body(const VBlock & vb, Grid g, dou
   const int nz = 1;
   (void)uz;
 #endif
+  // every load first (the image stores below are loops the compiler does
+  // not move loads across): one memory round trip per thread instead of four
   const double pc = psi[I_C1(i, j, k)];
   const double pm = psi[I_C1(i - 1, j, k)];
+  const double pp = psi[I_C1(i + 1, j, k)];
+  const double pym = psi[I_C1(i, j - 1, k)];
+  const double u0 = ux[I_UX(i, j, k)];   // i = 1: the wall entry, loaded and unused
+  const double v0 = uy[I_C1(i, j, k)];
+  const double p0 = p[I_C1(i, j, k)];
+  const double dxc_m = g.dxc[i - 1], dxc_p = g.dxc[i], dxf = g.dxf[i - 1];
+#if NDIMS == 3
+  const double pzm = psi[I_C1(i, j, k - 1)];
+  const double w0 = uz[I_C1(i, j, k)];
+#endif
   if (i >= 2) {
-    const double dx = g.dxc[i - 1];
-    const double v = ux[I_UX(i, j, k)] - prefactor_div(pref_c, dx) * (+pc - pm);
+    const double v = u0 - prefactor_div(pref_c, dxc_m) * (+pc - pm);
     store_periodic(ux, i - 1, nx + 1, ny + 2, j, PLANE1(k), ny, nz, 1, v);
   }
   {
-    const double v = uy[I_C1(i, j, k)] - prefactor_div(pref_c, g.dy) * (+pc - psi[I_C1(i, j - 1, k)]);
+    const double v = v0 - prefactor_div(pref_c, g.dy) * (+pc - pym);
     store_periodic(uy, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
   }
 #if NDIMS == 3
   {
-    const double v = uz[I_C1(i, j, k)] - prefactor_div(pref_c, g.dz) * (+pc - psi[I_C1(i, j, k - 1)]);
+    const double v = w0 - prefactor_div(pref_c, g.dz) * (+pc - pzm);
     store_periodic(uz, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
   }
 #endif
   {
-    const double pp = psi[I_C1(i + 1, j, k)];
-    double v = p[I_C1(i, j, k)];
+    double v = p0;
     v += pc;
-    const double dpsidx_xm = (-pm + pc) / g.dxc[i - 1];
-    const double dpsidx_xp = (-pc + pp) / g.dxc[i];
-    v -= pref_p / g.dxf[i - 1] * (-dpsidx_xm + dpsidx_xp);
+    const double dpsidx_xm = (-pm + pc) / dxc_m;
+    const double dpsidx_xp = (-pc + pp) / dxc_p;
+    v -= pref_p / dxf * (-dpsidx_xm + dpsidx_xp);
     store_periodic(p, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, v);
     neumann_walls(p, i, nx, ny + 2, j, PLANE1(k), ny, nz, v);
   }
