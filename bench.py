@@ -132,7 +132,29 @@ def mixed_form(g, cells, steps, plain_value, refine):
             "workload": f"same 3D 256^3 case, FP32 transforms + FP64 x eliminations, {refine} FP64 refinement(s)"}
 
 
-def batched_form(sva, ncases, steps):
+def autotuned_form(g, cells, steps, plain_value, k):
+    """The plain step after vof_autotune (k steps per candidate; the state is
+    restored by the tuner), timed over `steps` steps."""
+    t0 = time.perf_counter()
+    rep = g.autotune(k)
+    t_tune = time.perf_counter() - t0
+    tuned = {n[5:]: v for n, v, _ in rep if n.startswith("best:")}
+    g.integrate()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.integrate()
+    g.synchronize()
+    el = time.perf_counter() - t0
+    v = cells * steps / el / 1e6
+    print(f"[bench] autotuned ({t_tune:.1f} s, {len(rep)} trials): {tuned}: {v:.1f} Mcell-steps/s",
+          file=sys.stderr, flush=True)
+    return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+            "vs_plain": round(v / plain_value, 4), "chosen": tuned, "trials": len(rep),
+            "tune_seconds": round(t_tune, 2), "workload": "same 3D 256^3 case, FP64, autotuned tile parameters"}
+
+
+def batched_form(sva, ncases, steps, autotune=2):
     """`ncases` independent 2D cases of uneven sizes (nx in {64..256},
     ny = 2 nx) advanced together, against the same cases stepped one after
     another with integrate()."""
@@ -162,6 +184,13 @@ def batched_form(sva, ncases, steps):
         try:
             b.integrate()
             b.synchronize()
+            tuned = None
+            if autotune > 0:
+                ta = time.perf_counter()
+                rep = b.autotune(autotune)
+                tuned = {k[5:]: v for k, v, _ in rep if k.startswith("best:")}
+                print(f"[bench] batch autotune ({time.perf_counter() - ta:.1f} s): {tuned}", file=sys.stderr,
+                      flush=True)
             t0 = time.perf_counter()
             for _ in range(steps):
                 b.integrate()
@@ -177,7 +206,7 @@ def batched_form(sva, ncases, steps):
           f"one by one {seq:.1f}", file=sys.stderr, flush=True)
     return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
             "sequential_value": round(seq, 3), "vs_sequential": round(v / seq, 3), "cases": ncases,
-            "cells": cells, "steps": steps,
+            "cells": cells, "steps": steps, "autotuned": tuned,
             "workload": f"{ncases} 2D cases, nx in {{64,96,128,192,256}}, ny = 2 nx, 1-4 droplets, "
                         "Re in [50,500], We in [5,50], FP64"}
 
@@ -280,6 +309,8 @@ def main(argv=None) -> int:
                     help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb, xcd_bands, rhs_rows)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the mixed-precision and batched forms")
+    ap.add_argument("--autotune", type=int, default=3, metavar="K",
+                    help="steps per candidate of the autotuner for the autotuned and batched forms (0 = off)")
     ap.add_argument("--batch-cases", type=int, default=96)
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args(argv)
@@ -346,6 +377,11 @@ def main(argv=None) -> int:
             forms["mixed_precision_fp32_solve"] = mixed_form(g, cells, args.steps, value, 0)
         except Exception as e:  # reported, never fatal for the plain line
             forms["mixed_precision"] = {"error": repr(e)}
+    if not args.no_forms and args.poisson_mixed < 0 and args.autotune > 0:
+        try:
+            forms["autotuned"] = autotuned_form(g, cells, args.steps, value, args.autotune)
+        except Exception as e:
+            forms["autotuned"] = {"error": repr(e)}
     if args.kernels:
         print(f"[bench] kernels (profiled pass {el_prof * 1e3:.2f} ms, kernel sum {kern_total:.2f} ms; "
               f"timed pass {el * 1e3:.2f} ms):", file=sys.stderr)
@@ -358,7 +394,7 @@ def main(argv=None) -> int:
     g.close()
     if not args.no_forms:
         try:
-            forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps)
+            forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps, args.autotune)
         except Exception as e:
             forms["batched"] = {"error": repr(e)}
         try:

@@ -302,6 +302,15 @@ int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * con
                      interface_t * const interfaces[], vof_batch_t ** batch);
 /* one time step of every case; dts[c] receives case c's dt (may be NULL) */
 int vof_batch_integrate(vof_batch_t * batch, double dts[]);
+/* hipGraph replay of the batched step (on = 1, the default): each phase of
+ * the step (dt, then the three RK stages), with its argument-table and
+ * Thomas-table copies and its merged launches, is captured into a graph
+ * once and replayed while its launch layout is unchanged (re-captured when
+ * it changes, e.g. after tuning); on = 0 issues the launches directly.
+ * Results are the same either way. */
+int vof_batch_set_graphs(vof_batch_t * batch, int on);
+/* graphs captured and graph launches issued so far */
+int vof_batch_graph_counts(const vof_batch_t * batch, size_t * captures, size_t * launches);
 int vof_batch_synchronize(vof_batch_t * batch);
 int vof_batch_destroy(vof_batch_t * batch);
 
@@ -324,6 +333,20 @@ int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, interface_t *
 int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, double * loss);
 #endif
 
+/* autotuner: picks "fft_radix", "fft_elems", "iface_planes" (3D),
+ * "rhs_rows" and "lines_lpb" of vof_set_tuning by timing `steps` steps of
+ * this case per candidate (coordinate descent in that order; a candidate
+ * must be 1 % faster to replace the current value).  The case's device state
+ * is copied first and restored after every trial, so it is left as it was,
+ * with the fastest parameters set.  report (may be NULL) receives one
+ * "name value ms_per_step" line per trial and a "best name value ms" line
+ * per parameter.  Fused schedule only.  No counterpart in the reference. */
+int vof_autotune(domain_t * domain, fluid_t * fluid, interface_t * interface, int steps, char * report,
+                 size_t len);
+/* the same over a batch: every candidate is applied to all cases and timed
+ * over `steps` calls of vof_batch_integrate */
+int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len);
+
 /* tile parameters (results do not depend on them): "fft_elems" = 1024 or
  * 2048 complex values per LDS buffer of the FFT kernels, "lines_lpb" = lines
  * per work-group of the predictor's x solver (0 = automatic); "xcd_bands" =
@@ -331,6 +354,9 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
  * 8 (default) or 16 rows of one column per thread of the source kernel;
  * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
  * work-group by the 3D interface kernel;
+ * "fft_radix" = 16 (default: 256-point transforms on the register
+ * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
+ * one changes results by rounding only, about 1e-16 relative);
  * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e
  * (this one changes adjoint results) */
 int vof_set_tuning(domain_t * domain, const char name[], int value);

@@ -78,6 +78,7 @@ ABI_SYMBOLS_COMMON = [
     "vof_read_step_time", "vof_write_step_time", "vof_set_tuning", "vof_nusselt",
     "statistics", "vof_statistics_output_to", "vof_set_poisson_mixed",
     "vof_batch_create", "vof_batch_integrate", "vof_batch_synchronize", "vof_batch_destroy",
+    "vof_autotune", "vof_batch_autotune", "vof_batch_set_graphs", "vof_batch_graph_counts",
     "vof_integrate_fixed_dt",
 ]
 ABI_SYMBOLS_2D = ["vof_adjoint_gradient", "vof_adjoint_loss"]
@@ -134,6 +135,10 @@ def load(ndims: int):
     lib.vof_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, dp]
     lib.vof_set_fusion.argtypes = [vp, C.c_int]
     lib.vof_set_tuning.argtypes = [vp, C.c_char_p, C.c_int]
+    lib.vof_autotune.argtypes = [vp, vp, vp, C.c_int, C.c_char_p, C.c_size_t]
+    lib.vof_batch_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.c_size_t]
+    lib.vof_batch_set_graphs.argtypes = [vp, C.c_int]
+    lib.vof_batch_graph_counts.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     lib.vof_nusselt.argtypes = [vp, vp, dp]
     lib.vof_set_poisson_mixed.argtypes = [vp, C.c_int]
     lib.vof_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
@@ -355,6 +360,15 @@ class Solver:
     def set_tuning(self, name: str, value: int):
         self._chk(self.lib.vof_set_tuning(C.byref(self.domain), name.encode(), int(value)), "vof_set_tuning")
 
+    def autotune(self, steps: int = 3) -> list:
+        """vof_autotune: time `steps` steps per candidate tile parameter and
+        keep the fastest (the state is restored); returns the report lines
+        as (name, value, ms_per_step) tuples."""
+        buf = C.create_string_buffer(1 << 14)
+        self._chk(self.lib.vof_autotune(C.byref(self.domain), C.byref(self.fluid), C.byref(self.interface),
+                                        int(steps), buf, len(buf)), "vof_autotune")
+        return _tune_report(buf)
+
     def profile(self, on: bool = True):
         self._chk(self.lib.vof_profile(C.byref(self.domain), int(on)), "vof_profile")
 
@@ -425,6 +439,26 @@ class Batch:
             raise VofError("vof_batch_integrate: " + self.lib.vof_last_error().decode())
         return list(self._dts)
 
+    def autotune(self, steps: int = 3) -> list:
+        """vof_batch_autotune: candidates applied to every case, timed over
+        `steps` batch steps; returns (name, value, ms_per_step) tuples."""
+        buf = C.create_string_buffer(1 << 14)
+        if self.lib.vof_batch_autotune(self.h, int(steps), buf, len(buf)) != 0:
+            raise VofError("vof_batch_autotune: " + self.lib.vof_last_error().decode())
+        return _tune_report(buf)
+
+    def set_graphs(self, on: bool):
+        """hipGraph replay of the step's phases (default on)."""
+        if self.lib.vof_batch_set_graphs(self.h, int(bool(on))) != 0:
+            raise VofError("vof_batch_set_graphs: " + self.lib.vof_last_error().decode())
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) so far."""
+        a, b = C.c_size_t(0), C.c_size_t(0)
+        if self.lib.vof_batch_graph_counts(self.h, C.byref(a), C.byref(b)) != 0:
+            raise VofError("vof_batch_graph_counts: " + self.lib.vof_last_error().decode())
+        return a.value, b.value
+
     def synchronize(self):
         if self.lib.vof_batch_synchronize(self.h) != 0:
             raise VofError("vof_batch_synchronize: " + self.lib.vof_last_error().decode())
@@ -433,6 +467,19 @@ class Batch:
         if self.h:
             self.lib.vof_batch_destroy(self.h)
             self.h = None
+
+
+def _tune_report(buf) -> list:
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        if f[0] == "best":
+            out.append(("best:" + f[1], int(f[2]), float(f[3])))
+        elif f[0] == "start":
+            out.append(("start", None, float(f[2])))
+        else:
+            out.append((f[0], int(f[1]), float(f[2])))
+    return out
 
 
 def solver_from_ic(ic_obj, dirname: str, schedule: int | None = None, params: dict | None = None) -> "Solver":

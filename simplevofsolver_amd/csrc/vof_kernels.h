@@ -2278,6 +2278,8 @@ __device__ inline double block_min(double v) {
   return red[0];
 }
 
+constexpr int kDtU = 4;   // planes per pass of k_dt_partial
+
 __device__ __forceinline__ void k_dt_partial_body(const VBlock & vb, Grid g, const double * __restrict__ ux,
                                                             const double * __restrict__ uy,
                                                             const double * __restrict__ uz,
@@ -2300,16 +2302,31 @@ __device__ __forceinline__ void k_dt_partial_body(const VBlock & vb, Grid g, con
     const double dxc = i >= 2 ? g.dxc[i - 1] : 1.;
     for (int j = (int)(vb.y * 4 + threadIdx.x / 64) + 1; j <= ny; j += 4 * (int)vb.gy) {
 #if NDIMS == 2
-      for (int k = 1; k <= 1; k++) {
+      if (i >= 2) m = fmin(m, dxc / (fabs(ux[I_UX(i, j, 1)]) + small));
+      m = fmin(m, g.dy / (fabs(uy[I_C1(i, j, 1)]) + small));
 #else
-      for (int k = (int)vb.z + 1; k <= g.nz; k += (int)vb.gz) {
-#endif
-        if (i >= 2) m = fmin(m, dxc / (fabs(ux[I_UX(i, j, k)]) + small));
-        m = fmin(m, g.dy / (fabs(uy[I_C1(i, j, k)]) + small));
-#if NDIMS == 3
-        m = fmin(m, g.dz / (fabs(uz[I_C1(i, j, k)]) + small));
-#endif
+      // kDtU planes per pass with their loads issued together (planes past
+      // nz are clamped to it and skipped; the minimum does not depend on
+      // the order its terms are taken in)
+      const int gz = (int)vb.gz;
+      for (int k0 = (int)vb.z + 1; k0 <= g.nz; k0 += kDtU * gz) {
+        double a[kDtU], b[kDtU], c[kDtU];
+#pragma unroll
+        for (int u = 0; u < kDtU; u++) {
+          const int k = min(k0 + u * gz, g.nz);
+          a[u] = ux[I_UX(i, j, k)];   // i = 1: the wall entry, loaded and skipped
+          b[u] = uy[I_C1(i, j, k)];
+          c[u] = uz[I_C1(i, j, k)];
+        }
+#pragma unroll
+        for (int u = 0; u < kDtU; u++) {
+          if (k0 + u * gz > g.nz) break;
+          if (i >= 2) m = fmin(m, dxc / (fabs(a[u]) + small));
+          m = fmin(m, g.dy / (fabs(b[u]) + small));
+          m = fmin(m, g.dz / (fabs(c[u]) + small));
+        }
       }
+#endif
     }
   }
   m = block_min(m);

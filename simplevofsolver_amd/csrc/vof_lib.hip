@@ -13,6 +13,8 @@
 #include <errno.h>
 #include <sys/stat.h>
 
+#include <functional>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -223,6 +225,7 @@ struct vof_ctx {
   int lines_lpb;                         // lines per work-group of the x solver (0 = auto)
   int rhs_rows;                          // rows marched per thread by the source kernel (1 = one DOF per thread)
   int iface_planes;                      // z planes marched per work-group by the 3D interface kernel (0 = blocks)
+  int fft_radix;                         // 16: 256-point transforms on the register radix-16 path; 4: mixed radix
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
   Grid grid;
@@ -437,10 +440,18 @@ struct vof_batch {
   std::vector<interface_t *> interface;
   std::vector<Recorder> rec;
   hipStream_t stream;
-  // staging of the batched launches' argument tables (pinned + device)
-  unsigned char * h_stage;
-  unsigned char * d_stage;
-  size_t stage_bytes;
+  // staging of the merged launches' argument tables, one pinned + device
+  // pair per phase of the step (0: dt, 1..3: RK stages)
+  unsigned char * h_stage[4];
+  unsigned char * d_stage[4];
+  size_t stage_bytes[4];
+  hipEvent_t stage_done[4];
+  // hipGraph of each phase (its copies and merged launches), captured once
+  // and replayed while the phase's launch layout is unchanged
+  bool graphs;
+  hipGraphExec_t gexec[4];
+  std::vector<size_t> gsig[4];
+  size_t graph_captures, graph_launches;
   // Thomas tables of every case, contiguous (cases re-pointed into these)
   double * h_tab;
   double * d_tab;
@@ -449,7 +460,6 @@ struct vof_batch {
   double * d_dt;
   double * h_dt;
   const double ** d_dt_src;
-  hipEvent_t stage_done;
 };
 
 // build (dev == nullptr: size only / fill host) or launch (dev != nullptr)
@@ -827,6 +837,7 @@ static int ctx_create(domain_t * domain) {
   ctx->schedule = 1;
   ctx->fuse = 63;
   ctx->fft_elems = 1024;
+  ctx->fft_radix = 16;
   ctx->poisson_refine = -1;
   ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
@@ -1523,13 +1534,13 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
   const double scale = 1. / ((double)ny * (NDIMS == 3 ? (double)nz : 1.));
   const int elems = 2048;
   // 256-point transforms take the register radix-16 path, as in potential()
-  const bool f256y = ny == 256;
+  const bool f256y = ny == 256 && ctx->fft_radix == 16;
   const int NC = f256y ? vofk::kF256Cols : elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
   const dim3 gy(nbx, NDIMS == 3 ? nz : 1), bf(vofk::kFftThreads);
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
-  const bool f256z = nz == 256;
+  const bool f256z = nz == 256 && ctx->fft_radix == 16;
   const int NCz = f256z ? vofk::kF256Cols : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
 #else
@@ -1596,7 +1607,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const int elems = (ctx->fft_elems == 1024 && ny <= 1024 && (NDIMS == 2 || nz <= 1024)) ? 1024 : 2048;
   // 256-point transforms take the register radix-16 path (8 columns per
   // work-group), other lengths the mixed-radix Stockham path
-  const bool f256y = ny == 256;
+  const bool f256y = ny == 256 && ctx->fft_radix == 16;
   const int NC = f256y ? vofk::kF256Cols : elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
   const dim3 gy(nbx, NDIMS == 3 ? nz : 1), bf(vofk::kFftThreads);
@@ -1617,7 +1628,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   }
 #undef DIV_ARGS
 #if NDIMS == 3
-  const bool f256z = nz == 256;
+  const bool f256z = nz == 256 && ctx->fft_radix == 16;
   const int NCz = f256z ? vofk::kF256Cols : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
   for (int pass = 0; pass < 2; pass++) {
@@ -1976,6 +1987,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     ctx->fft_elems = value;
     return 0;
   }
+  if (!strcmp(name, "fft_radix")) {
+    if (value != 4 && value != 16) return fail("fft_radix must be 4 or 16");
+    ctx->fft_radix = value;
+    return 0;
+  }
   if (!strcmp(name, "xcd_bands")) {
     ctx->grid.bands = value ? 1 : 0;
     return 0;
@@ -2034,18 +2050,25 @@ __global__ void k_gather_dt(const double * const * __restrict__ src, int n, doub
   if (c < n) dst[c] = *src[c];
 }
 
-static int batch_stage_reserve(vof_batch * B, size_t bytes) {
-  if (bytes <= B->stage_bytes) return 0;
+static void batch_drop_graph(vof_batch * B, int ph) {
+  if (B->gexec[ph]) (void)hipGraphExecDestroy(B->gexec[ph]);
+  B->gexec[ph] = nullptr;
+  B->gsig[ph].clear();
+}
+
+static int batch_stage_reserve(vof_batch * B, int ph, size_t bytes) {
+  if (bytes <= B->stage_bytes[ph]) return 0;
   HIPCHK(hipStreamSynchronize(B->stream));
-  host_free(B->h_stage);
-  dev_free(B->d_stage);
-  B->h_stage = nullptr;
-  B->d_stage = nullptr;
-  B->stage_bytes = 0;
+  batch_drop_graph(B, ph);
+  host_free(B->h_stage[ph]);
+  dev_free(B->d_stage[ph]);
+  B->h_stage[ph] = nullptr;
+  B->d_stage[ph] = nullptr;
+  B->stage_bytes[ph] = 0;
   const size_t nb = bytes + bytes / 2 + 4096;
-  HIPCHK(hipHostMalloc(&B->h_stage, nb));
-  HIPCHK(hipMalloc(&B->d_stage, nb));
-  B->stage_bytes = nb;
+  HIPCHK(hipHostMalloc(&B->h_stage[ph], nb));
+  HIPCHK(hipMalloc(&B->d_stage[ph], nb));
+  B->stage_bytes[ph] = nb;
   return 0;
 }
 
@@ -2058,7 +2081,13 @@ struct BatchGroup {
   std::vector<int> cases;
 };
 
-static int batch_run(vof_batch * B) {
+// phase ph of the step: `pre` and `post` enqueue the phase's other copies
+// and launches (Thomas tables before a stage; the dt gather after phase 0).
+// With graphs on, the whole phase is captured once and replayed while its
+// signature (groups, kernel instantiations, grids, table offsets, staging
+// buffers) is unchanged; the argument tables themselves are refilled on the
+// host every step and copied by the graph's first node.
+static int batch_run(vof_batch * B, int ph, const std::function<int()> & pre, const std::function<int()> & post) {
   const size_t n = B->ctx.size();
   const size_t nrec = B->rec[0].recs.size();
   for (size_t c = 1; c < n; c++) {
@@ -2087,20 +2116,62 @@ static int batch_run(vof_batch * B) {
     const size_t bytes = r0.merge(B, groups[q].r, groups[q].cases, nullptr, nullptr);
     off[q + 1] = off[q] + ((bytes + 255) & ~(size_t)255);
   }
-  // the previous copy out of the staging buffer must have completed
-  HIPCHK(hipEventSynchronize(B->stage_done));
-  if (batch_stage_reserve(B, off[ng])) return 1;
+  // the previous copy out of this phase's staging buffer must have completed
+  if (B->stage_done[ph]) HIPCHK(hipEventSynchronize(B->stage_done[ph]));
+  if (batch_stage_reserve(B, ph, off[ng])) return 1;
   for (size_t q = 0; q < ng; q++) {
     const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
-    r0.merge(B, groups[q].r, groups[q].cases, B->h_stage + off[q], nullptr);
+    r0.merge(B, groups[q].r, groups[q].cases, B->h_stage[ph] + off[q], nullptr);
   }
-  HIPCHK(hipMemcpyAsync(B->d_stage, B->h_stage, off[ng], hipMemcpyHostToDevice, B->stream));
-  HIPCHK(hipEventRecord(B->stage_done, B->stream));
-  for (size_t q = 0; q < ng; q++) {
-    const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
-    if (r0.merge(B, groups[q].r, groups[q].cases, nullptr, B->d_stage + off[q]) == 0)
-      return fail("batch: launch of %s failed", r0.name);
+  auto enqueue = [&]() -> int {
+    if (pre && pre()) return 1;
+    HIPCHK(hipMemcpyAsync(B->d_stage[ph], B->h_stage[ph], off[ng], hipMemcpyHostToDevice, B->stream));
+    for (size_t q = 0; q < ng; q++) {
+      const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
+      if (r0.merge(B, groups[q].r, groups[q].cases, nullptr, B->d_stage[ph] + off[q]) == 0)
+        return fail("batch: launch of %s failed", r0.name);
+    }
+    return post ? post() : 0;
+  };
+  if (!B->graphs) {
+    if (enqueue()) return 1;
+  } else {
+    std::vector<size_t> sig = {ng, off[ng], (size_t)B->h_stage[ph], (size_t)B->d_stage[ph]};
+    for (size_t q = 0; q < ng; q++) {
+      const LaunchRec & r0 = B->rec[groups[q].cases[0]].recs[groups[q].r];
+      size_t total = 0, shm = 0;
+      for (int c : groups[q].cases) {
+        const LaunchRec & r = B->rec[c].recs[groups[q].r];
+        total += (size_t)r.grid.x * r.grid.y * r.grid.z;
+        shm = r.shm > shm ? r.shm : shm;
+      }
+      const size_t v[] = {(size_t)r0.merge, groups[q].cases.size(), off[q], total, shm, r0.block.x, r0.block.y,
+                          r0.block.z};
+      sig.insert(sig.end(), v, v + 8);
+    }
+    if (!B->gexec[ph] || B->gsig[ph] != sig) {
+      batch_drop_graph(B, ph);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      const int erc = enqueue();
+      const hipError_t ce = hipStreamEndCapture(B->stream, &g);
+      if (erc || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return erc ? 1 : fail("batch: graph capture of phase %d failed: %s", ph, hipGetErrorString(ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&B->gexec[ph], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        B->gexec[ph] = nullptr;
+        return fail("batch: graph instantiation of phase %d failed: %s", ph, hipGetErrorString(ie));
+      }
+      B->gsig[ph] = sig;
+      B->graph_captures++;
+    }
+    HIPCHK(hipGraphLaunch(B->gexec[ph], B->stream));
+    B->graph_launches++;
   }
+  HIPCHK(hipEventRecord(B->stage_done[ph], B->stream));
   return 0;
 }
 
@@ -2108,7 +2179,7 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
                                 interface_t * const interfaces[], vof_batch_t ** out) {
   if (ncases < 1 || !domains || !fluids || !interfaces || !out) return fail("vof_batch_create: bad arguments");
   vof_batch * B = new vof_batch();
-  B->stage_done = nullptr;
+  B->graphs = true;
   for (int c = 0; c < ncases; c++) {
     vof_ctx * ctx = ctx_of(domains[c]);
     if (!ctx) {
@@ -2130,9 +2201,11 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
     B->interface.push_back(interfaces[c]);
   }
   B->rec.resize(ncases);
-  if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&B->stage_done, hipEventDisableTiming) != hipSuccess) {
-    delete B;
+  bool ok = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int ph = 0; ph < 4 && ok; ph++)
+    ok = hipEventCreateWithFlags(&B->stage_done[ph], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    vof_batch_destroy(B);
     return fail("vof_batch_create: stream / event creation failed");
   }
   int rc = 0;
@@ -2188,11 +2261,15 @@ extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
     rc = decide_dt_enqueue(B->ctx[c], B->fluid[c]);
     B->ctx[c]->rec = nullptr;
   }
-  if (rc || batch_run(B)) return 1;
-  hipLaunchKernelGGL(k_gather_dt, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, B->stream,
-                     (const double * const *)B->d_dt_src, (int)n, B->d_dt);
-  LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(B->h_dt, B->d_dt, sizeof(double) * n, hipMemcpyDeviceToHost, B->stream));
+  if (rc) return 1;
+  auto gather = [&]() -> int {
+    hipLaunchKernelGGL(k_gather_dt, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, B->stream,
+                       (const double * const *)B->d_dt_src, (int)n, B->d_dt);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(B->h_dt, B->d_dt, sizeof(double) * n, hipMemcpyDeviceToHost, B->stream));
+    return 0;
+  };
+  if (batch_run(B, 0, nullptr, gather)) return 1;
   HIPCHK(hipStreamSynchronize(B->stream));
   std::vector<double> dt(n);
   for (size_t c = 0; c < n; c++) {
@@ -2210,12 +2287,31 @@ extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
       B->ctx[c]->rec = nullptr;
     }
     if (rc) return 1;
-    HIPCHK(hipMemcpyAsync(B->d_tab + rk * tab_stage, B->h_tab + rk * tab_stage, sizeof(double) * tab_stage,
-                          hipMemcpyHostToDevice, B->stream));
-    if (batch_run(B)) return 1;
+    auto tables = [&]() -> int {
+      HIPCHK(hipMemcpyAsync(B->d_tab + rk * tab_stage, B->h_tab + rk * tab_stage, sizeof(double) * tab_stage,
+                            hipMemcpyHostToDevice, B->stream));
+      return 0;
+    };
+    if (batch_run(B, (int)rk + 1, tables, nullptr)) return 1;
   }
   if (dts)
     for (size_t c = 0; c < n; c++) dts[c] = dt[c];
+  return 0;
+}
+
+extern "C" int vof_batch_set_graphs(vof_batch_t * B, int on) {
+  if (!B) return fail("vof_batch_set_graphs: batch is NULL");
+  HIPCHK(hipStreamSynchronize(B->stream));
+  B->graphs = on != 0;
+  if (!B->graphs)
+    for (int ph = 0; ph < 4; ph++) batch_drop_graph(B, ph);
+  return 0;
+}
+
+extern "C" int vof_batch_graph_counts(const vof_batch_t * B, size_t * captures, size_t * launches) {
+  if (!B || !captures || !launches) return fail("vof_batch_graph_counts: NULL argument");
+  *captures = B->graph_captures;
+  *launches = B->graph_launches;
   return 0;
 }
 
@@ -2244,17 +2340,232 @@ extern "C" int vof_batch_destroy(vof_batch_t * B) {
     }
     ctx->tab_shared = false;
   }
-  host_free(B->h_stage);
-  dev_free(B->d_stage);
+  for (int ph = 0; ph < 4; ph++) {
+    batch_drop_graph(B, ph);
+    host_free(B->h_stage[ph]);
+    dev_free(B->d_stage[ph]);
+    if (B->stage_done[ph]) (void)hipEventDestroy(B->stage_done[ph]);
+  }
   host_free(B->h_tab);
   dev_free(B->d_tab);
   dev_free(B->d_dt);
   host_free(B->h_dt);
   dev_free((void *)B->d_dt_src);
-  if (B->stage_done) (void)hipEventDestroy(B->stage_done);
   if (B->stream) (void)hipStreamDestroy(B->stream);
   delete B;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner.  The tile parameters of vof_set_tuning and the FFT radix are
+// picked by timing whole steps of the caller's own cases: coordinate descent
+// over the parameters in a fixed order, every candidate timed over `steps`
+// steps (after one untimed step that loads its code) from the same state.
+// The state of every case is copied on the device first and put back after
+// each trial and at the end, so the cases are left as they were.  A
+// candidate replaces the current value only when it is at least 1 % faster.
+
+struct TuneSnap {
+  std::vector<array_t *> arr;
+  std::vector<void *> buf;      // each array's buffer at snapshot time
+  std::vector<void *> copy;     // its contents
+  std::vector<size_t> bytes;
+  vof_ctx * ctx = nullptr;
+  double * spare = nullptr;     // the VOF ping-pong buffer and its contents
+  void * spare_copy = nullptr;
+  TuneSnap() = default;
+  TuneSnap(const TuneSnap &) = delete;
+  TuneSnap & operator=(const TuneSnap &) = delete;
+  ~TuneSnap() {
+    for (void * c : copy) dev_free(c);
+    dev_free(spare_copy);
+  }
+};
+
+static int tune_take(vof_ctx * ctx, fluid_t * f, interface_t * s, hipStream_t st, TuneSnap & S) {
+  std::vector<array_t *> all = {&f->ux, &f->uy, &f->p, &f->psi, &f->t, &s->vof, &s->ifrcx, &s->ifrcy, &s->dvof,
+                                &s->normal, &s->curv, &s->flxx, &s->flxy, &s->src[0], &s->src[1]};
+#if NDIMS == 3
+  all.push_back(&f->uz);
+  all.push_back(&s->ifrcz);
+  all.push_back(&s->flxz);
+  for (int r = 0; r < 3; r++) all.push_back(&f->srcuz[r]);
+#endif
+  for (int r = 0; r < 3; r++) {
+    all.push_back(&f->srcux[r]);
+    all.push_back(&f->srcuy[r]);
+    all.push_back(&f->srct[r]);
+  }
+  S.ctx = ctx;
+  for (array_t * a : all) {
+    if (!a->data || !a->datasize) continue;
+    void * c = nullptr;
+    HIPCHK(hipMalloc(&c, a->datasize));
+    S.copy.push_back(c);
+    S.arr.push_back(a);
+    S.buf.push_back(a->data);
+    S.bytes.push_back(a->datasize);
+    HIPCHK(hipMemcpyAsync(c, a->data, a->datasize, hipMemcpyDeviceToDevice, st));
+  }
+  S.spare = ctx->d_vof_spare;
+  if (S.spare) {
+    HIPCHK(hipMalloc(&S.spare_copy, ctx->vof_bytes));
+    HIPCHK(hipMemcpyAsync(S.spare_copy, S.spare, ctx->vof_bytes, hipMemcpyDeviceToDevice, st));
+  }
+  return 0;
+}
+
+static int tune_restore(TuneSnap & S, hipStream_t st) {
+  // buffers first (undoing the pointer swaps of the steps), then contents
+  for (size_t q = 0; q < S.arr.size(); q++) S.arr[q]->data = S.buf[q];
+  S.ctx->d_vof_spare = S.spare;
+  for (size_t q = 0; q < S.arr.size(); q++)
+    HIPCHK(hipMemcpyAsync(S.buf[q], S.copy[q], S.bytes[q], hipMemcpyDeviceToDevice, st));
+  if (S.spare) HIPCHK(hipMemcpyAsync(S.spare, S.spare_copy, S.ctx->vof_bytes, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+struct TuneKnob {
+  const char * name;
+  std::vector<int> vals;
+  int vof_ctx::*field;
+};
+
+// candidate values valid for every case (tile parameters that the cases'
+// sizes do not use are left out)
+static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
+  bool any256 = false, any_other = false, over1024 = false;
+  for (const vof_ctx * c : cs) {
+    const int lens[2] = {c->ny, NDIMS == 3 ? c->nz : c->ny};
+    for (int L : lens) {
+      if (L == 256) any256 = true;
+      else any_other = true;
+      if (L > 1024) over1024 = true;
+    }
+  }
+  std::vector<TuneKnob> k;
+  if (any256) k.push_back({"fft_radix", {16, 4}, &vof_ctx::fft_radix});
+  if ((any_other || any256) && !over1024) k.push_back({"fft_elems", {1024, 2048}, &vof_ctx::fft_elems});
+#if NDIMS == 3
+  k.push_back({"iface_planes", {8, 0, 4, 16}, &vof_ctx::iface_planes});
+#endif
+  k.push_back({"rhs_rows", {8, 1, 2, 4, 16}, &vof_ctx::rhs_rows});
+  k.push_back({"lines_lpb", {0, 2, 4, 8, 16, 32}, &vof_ctx::lines_lpb});
+  return k;
+}
+
+static int tune_run(const std::vector<vof_ctx *> & cs, std::vector<std::unique_ptr<TuneSnap>> & snaps,
+                    hipStream_t st, int steps, const std::function<int()> & step, char * report, size_t len) {
+  std::string rep;
+  char line[160];
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  auto trial = [&](double * ms) -> int {
+    for (auto & S : snaps)
+      if (tune_restore(*S, st)) return 1;
+    if (step()) return 1;
+    HIPCHK(hipEventRecord(e0, st));
+    for (int n = 0; n < steps; n++)
+      if (step()) return 1;
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = (double)t / steps;
+    return 0;
+  };
+  std::vector<TuneKnob> knobs = tune_knobs(cs);
+  double best = 0.;
+  rc = trial(&best);
+  if (!rc) {
+    snprintf(line, sizeof line, "start - %.4f\n", best);
+    rep += line;
+  }
+  for (const TuneKnob & kn : knobs) {
+    if (rc) break;
+    const int cur = cs[0]->*kn.field;
+    int pick = cur;
+    for (int v : kn.vals) {
+      if (v == cur) continue;
+      for (vof_ctx * c : cs) c->*kn.field = v;
+      double ms = 0.;
+      if ((rc = trial(&ms))) break;
+      snprintf(line, sizeof line, "%s %d %.4f\n", kn.name, v, ms);
+      rep += line;
+      if (ms < 0.99 * best) {
+        best = ms;
+        pick = v;
+      }
+    }
+    for (vof_ctx * c : cs) c->*kn.field = pick;
+    if (!rc) {
+      snprintf(line, sizeof line, "best %s %d %.4f\n", kn.name, pick, best);
+      rep += line;
+    }
+  }
+  // leave every case as it was
+  for (auto & S : snaps)
+    if (!rc && tune_restore(*S, st)) rc = 1;
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail("autotune: stream failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (report && len) {
+    strncpy(report, rep.c_str(), len - 1);
+    report[len - 1] = 0;
+  }
+  return rc;
+}
+
+extern "C" int vof_autotune(domain_t * domain, fluid_t * fluid, interface_t * interface, int steps, char * report,
+                            size_t len) {
+  CTX_OR_FAIL(domain);
+  if (!fluid || !interface) return fail("vof_autotune: fluid or interface is NULL");
+  if (steps < 1 || steps > 1000) return fail("vof_autotune: steps must be in 1..1000");
+  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_autotune: fused schedule only");
+  if (ctx->rec || ctx->tape) return fail("vof_autotune: case is recording");
+  if (ensure_vof_spare(ctx, interface)) return 1;
+  const bool prof = ctx->prof_on;
+  ctx->prof_on = false;
+  std::vector<vof_ctx *> cs = {ctx};
+  std::vector<std::unique_ptr<TuneSnap>> snaps;
+  snaps.emplace_back(new TuneSnap());
+  int rc = tune_take(ctx, fluid, interface, ctx->stream, *snaps[0]);
+  if (!rc) {
+    double dt = 0.;
+    rc = tune_run(cs, snaps, ctx->stream, steps, [&]() { return integrate(domain, fluid, interface, &dt); },
+                  report, len);
+  }
+  if (rc) (void)hipStreamSynchronize(ctx->stream);
+  ctx->prof_on = prof;
+  return rc;
+}
+
+extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, size_t len) {
+  if (!B) return fail("vof_batch_autotune: batch is NULL");
+  if (steps < 1 || steps > 1000) return fail("vof_batch_autotune: steps must be in 1..1000");
+  HIPCHK(hipStreamSynchronize(B->stream));
+  std::vector<std::unique_ptr<TuneSnap>> snaps;
+  std::vector<bool> prof(B->ctx.size());
+  int rc = 0;
+  for (size_t c = 0; c < B->ctx.size(); c++) {
+    prof[c] = B->ctx[c]->prof_on;
+    B->ctx[c]->prof_on = false;
+  }
+  for (size_t c = 0; c < B->ctx.size() && !rc; c++) {
+    if (hipStreamSynchronize(B->ctx[c]->stream) != hipSuccess) {
+      rc = fail("vof_batch_autotune: stream of case %zu failed", c);
+      break;
+    }
+    snaps.emplace_back(new TuneSnap());
+    rc = tune_take(B->ctx[c], B->fluid[c], B->interface[c], B->stream, *snaps.back());
+  }
+  if (!rc)
+    rc = tune_run(B->ctx, snaps, B->stream, steps, [&]() { return vof_batch_integrate(B, nullptr); }, report, len);
+  if (rc) (void)hipStreamSynchronize(B->stream);
+  for (size_t c = 0; c < B->ctx.size(); c++) B->ctx[c]->prof_on = prof[c];
+  return rc;
 }
 
 // one step with a given dt (the frozen dt sequence of the adjoint and its

@@ -73,3 +73,39 @@ def test_batch_rejects_cases_off_the_fused_schedule(tmp_path):
             sva.Batch([a])
     finally:
         a.close()
+
+
+@pytest.mark.parametrize("dim", [2, "2-256", "3-256"])
+def test_batch_graph_replays_match_direct_launches(dim, tmp_path):
+    """Every replay of the captured phase graphs gives the fields of the
+    single cases stepped eagerly in between (and of the same batch with
+    graphs off), step after step; the layout is captured once per phase."""
+    cases = {2: _cases_2d, "2-256": _cases_mixed_transforms, "3-256": _cases_mixed_transforms_3d}[dim]()
+    nd = len(cases[0].glsizes)
+    singles = [sva.solver_from_ic(ic, str(tmp_path / f"a{c}")) for c, ic in enumerate(cases)]
+    members = [sva.solver_from_ic(ic, str(tmp_path / f"b{c}")) for c, ic in enumerate(cases)]
+    direct = [sva.solver_from_ic(ic, str(tmp_path / f"c{c}")) for c, ic in enumerate(cases)]
+    batch = sva.Batch(members)
+    batch_d = sva.Batch(direct)
+    batch_d.set_graphs(False)
+    try:
+        for step in range(4):
+            dts = batch.integrate()
+            dts_d = batch_d.integrate()
+            ref = [s.integrate() for s in singles]
+            assert dts == ref and dts_d == ref, (step, dts, dts_d, ref)
+            batch.synchronize()
+            batch_d.synchronize()
+            for c, (a, d, s) in enumerate(zip(members, direct, singles)):
+                for n in _names(nd):
+                    want = s.get(n)
+                    assert np.array_equal(a.get(n), want), (step, c, n)
+                    assert np.array_equal(d.get(n), want), (step, c, n)
+        captures, launches = batch.graph_counts()
+        assert captures == 4 and launches == 16, (captures, launches)
+        assert batch_d.graph_counts() == (0, 0)
+    finally:
+        batch.close()
+        batch_d.close()
+        for s in singles + members + direct:
+            s.close()
