@@ -154,6 +154,112 @@ def autotuned_form(g, cells, steps, plain_value, k):
             "tune_seconds": round(t_tune, 2), "workload": "same 3D 256^3 case, FP64, autotuned tile parameters"}
 
 
+def _interior(name, a, nx, ny, nz):
+    """Interior DOFs of a padded 3D array (x fastest; vof has 2-wide y/z halos)."""
+    h = 2 if name == "vof" else 1
+    n0 = nx + 1 if name == "ux" else nx + 2
+    a = a.reshape(nz + 2 * h, ny + 2 * h, n0)
+    return a[h:-h, h:-h, 1:-1]
+
+
+def precision_form(sva, steps, refine):
+    """Precision workload: 3D 512 x 256 x 256 on 2 x 1 x 1 (x walls, y/z
+    periodic), 32 seeded droplets r in [0.03, 0.08], divergence-free velocity
+    rms 0.2, Re = 1000, We = 50.  Three solvers start from the same NPY and
+    step in lockstep: the plain FP64 step, the mixed form (FP32 transforms,
+    FP64 x eliminations, `refine` FP64 residual refinements) and an FP64
+    twin whose only difference is the rounding of its transforms (the
+    mixed-radix path for the 256-point lengths).  At steps 1, 10 and `steps`
+    each field of the mixed run and of the twin is compared with the FP64
+    run (max-norm relative; p and psi modulo their mean): the twin shows how
+    far rounding-level differences alone grow in this flow.  Before stepping,
+    one Poisson solve of each solver on the same (initial) input gives the
+    error of the mixed solver itself.  Throughput is
+    timed per solver over steps 2..`steps` (each step synchronised)."""
+    import numpy as np
+    from simplevofsolver_amd.ic import droplets_3d, write_ic
+    nx, ny, nz = 512, 256, 256
+    t0 = time.perf_counter()
+    ic, _ = droplets_3d(nx, ny, nz, lx=2.0, ly=1.0, lz=1.0, ndrops=32, rmin=0.03, rmax=0.08, urms=0.2,
+                        seed=5, Re=1000.0, We=50.0)
+    print(f"[bench] precision IC {nx}x{ny}x{nz} generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    cells = nx * ny * nz
+    names = ("ux", "uy", "uz", "vof", "p", "psi")
+    out = {"workload": f"3D {nx}x{ny}x{nz} on 2x1x1, 32 droplets r in [0.03,0.08], velocity rms 0.2, "
+                       f"Re=1000, We=50; {steps} steps; mixed = FP32 transforms + FP64 x eliminations + "
+                       f"{refine} FP64 refinement(s); fp64_twin = FP64 with mixed-radix 256-point transforms",
+           "steps": steps, "refinements": refine}
+    params = dict(ic.params)
+    checkpoints = sorted({1, min(10, steps), steps})
+
+    def fields(g):
+        res = {}
+        for n in names:
+            a = _interior(n, g.get(n), nx, ny, nz)
+            res[n] = a - a.mean() if n in ("p", "psi") else a
+        return res
+
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        write_ic(d, ic)
+        del ic
+        sol = {}
+        try:
+            for form in ("fp64", "mixed", "fp64_twin"):
+                sol[form] = sva.Solver(d, 3, params)
+            sol["mixed"].set_poisson_mixed(refine)
+            sol["fp64_twin"].set_tuning("fft_radix", 4)
+            # one Poisson solve on identical input (the initial state): the
+            # accuracy of the mixed solver itself
+            dt0 = sol["fp64"].decide_dt()
+            for g in sol.values():
+                g.potential(0, dt0)
+            ref = _interior("psi", sol["fp64"].get("psi"), nx, ny, nz)
+            ref = ref - ref.mean()
+            solve_err = {}
+            for k in ("mixed", "fp64_twin"):
+                a = _interior("psi", sol[k].get("psi"), nx, ny, nz)
+                a = a - a.mean()
+                solve_err[k] = float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+            del ref
+            out["poisson_solve_psi_error_vs_fp64"] = solve_err
+            print(f"[bench] precision workload, one Poisson solve: psi error vs FP64 {solve_err}", file=sys.stderr,
+                  flush=True)
+            elapsed = {k: 0.0 for k in sol}
+            err = {k: {} for k in sol if k != "fp64"}
+            for n in range(1, steps + 1):
+                for k, g in sol.items():
+                    t0 = time.perf_counter()
+                    g.integrate()
+                    g.synchronize()
+                    if n > 1:
+                        elapsed[k] += time.perf_counter() - t0
+                if n in checkpoints:
+                    ref = fields(sol["fp64"])
+                    for k in err:
+                        f = fields(sol[k])
+                        err[k][str(n)] = {m: float(np.max(np.abs(f[m] - ref[m])) /
+                                                   max(float(np.max(np.abs(ref[m]))), 1e-300)) for m in names}
+                        del f
+                    del ref
+                    print(f"[bench] precision workload step {n}: mixed {err['mixed'][str(n)]}, "
+                          f"twin {err['fp64_twin'][str(n)]}", file=sys.stderr, flush=True)
+        finally:
+            for g in sol.values():
+                g.close()
+    for k in sol:
+        if steps > 1:
+            v = cells * (steps - 1) / elapsed[k] / 1e6
+            out[k] = {"value": round(v, 3), "unit": "Mcell-steps/s",
+                      "ms_per_step": round(elapsed[k] / (steps - 1) * 1e3, 4)}
+    out["error_vs_fp64"] = err
+    if steps > 1:
+        out["vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 4)
+        print(f"[bench] precision workload: fp64 {out['fp64']['value']:.1f}, mixed {out['mixed']['value']:.1f} "
+              f"Mcell-steps/s", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_form(sva, ncases, steps, autotune=2):
     """`ncases` independent 2D cases of uneven sizes (nx in {64..256},
     ny = 2 nx) advanced together, against the same cases stepped one after
@@ -311,6 +417,10 @@ def main(argv=None) -> int:
                     help="skip the mixed-precision and batched forms")
     ap.add_argument("--autotune", type=int, default=3, metavar="K",
                     help="steps per candidate of the autotuner for the autotuned and batched forms (0 = off)")
+    ap.add_argument("--precision-steps", type=int, default=100,
+                    help="steps of the 512x256x256 precision workload (0 = skip)")
+    ap.add_argument("--precision-refine", type=int, default=2,
+                    help="FP64 refinements of its mixed-precision run")
     ap.add_argument("--batch-cases", type=int, default=96)
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args(argv)
@@ -397,6 +507,11 @@ def main(argv=None) -> int:
             forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps, args.autotune)
         except Exception as e:
             forms["batched"] = {"error": repr(e)}
+        if args.precision_steps > 0:
+            try:
+                forms["precision"] = precision_form(sva, args.precision_steps, args.precision_refine)
+            except Exception as e:
+                forms["precision"] = {"error": repr(e)}
         try:
             forms["adjoint"] = adjoint_form(sva)
         except Exception as e:
@@ -440,6 +555,9 @@ def main(argv=None) -> int:
             "max_div": d1["max_div"],
             "kinetic_energy_rel": ((d1["ke_x"] + d1["ke_y"] + d1["ke_z"]) /
                                    (d0["ke_x"] + d0["ke_y"] + d0["ke_z"]) - 1.0),
+            "interface_area_rel": d1["interface_area"] / d0["interface_area"] - 1.0,
+            "vof_min": d1["vof_min"],
+            "vof_max": d1["vof_max"],
             "steps": args.warmup + 2 * args.steps,
         },
     }

@@ -1566,7 +1566,8 @@ __device__ __forceinline__ void dft16(T2 * v, typename Cx<T2>::S sign) {
 // (k1, n2) and X[k] at (k % 16, k / 16) (f256_out).  Each thread owns one
 // matrix column in pass 1 and one row in pass 2, so both run in place.
 constexpr int kF256Cols = 8;      // transforms per work-group
-constexpr int kF256Stride = 272;  // complex values per column
+constexpr int kF256Stride = 273;  // complex values per column: 16 x 17 plus one, so that the
+                                  // columns of a work-group start in different LDS banks
 __device__ inline int f256_in(int n) { return (n >> 4) * 17 + (n & 15); }
 __device__ inline int f256_out(int k) { return (k & 15) * 17 + (k >> 4); }
 
@@ -2278,7 +2279,7 @@ __device__ inline double block_min(double v) {
   return red[0];
 }
 
-constexpr int kDtU = 4;   // planes per pass of k_dt_partial
+constexpr int kDtU = 16;  // planes per pass of k_dt_partial (a whole 256-plane column at 16 z groups)
 
 __device__ __forceinline__ void k_dt_partial_body(const VBlock & vb, Grid g, const double * __restrict__ ux,
                                                             const double * __restrict__ uy,

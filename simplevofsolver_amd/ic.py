@@ -112,6 +112,11 @@ def vof_spheres(xf, ny, ly, nz, lz, spheres):
     x1 = xf[1:]
     vof = np.zeros((nz, ny, nx))
     for (cx, cy, cz, r) in spheres:
+        # columns that meet [cx - r, cx + r] (the others receive exactly 0:
+        # both of their x bounds clip to the same side of the disc)
+        i0 = max(0, int(np.searchsorted(x1, cx - r, side="left")) - 1)
+        i1 = min(nx, int(np.searchsorted(x0, cx + r, side="right")) + 1)
+        xa, xb = x0[i0:i1], x1[i0:i1]
         for sy in (-ly, 0.0, ly):
             for sz in (-lz, 0.0, lz):
                 ccy, ccz = cy + sy, cz + sz
@@ -130,13 +135,13 @@ def vof_spheres(xf, ny, ly, nz, lz, spheres):
                     wq = 0.5 * (zb - za) * _GL_W
                     y0 = (js * dy)[:, None]
                     y1 = ((js + 1) * dy)[:, None]
-                    acc = np.zeros((len(js), nx))
+                    acc = np.zeros((len(js), i1 - i0))
                     for zz, ww in zip(zq, wq):
                         rz = np.sqrt(max(r * r - (zz - ccz) ** 2, 0.0))
                         if rz <= 0.0:
                             continue
-                        acc += ww * disc_rect_area(x0[None, :], x1[None, :], y0, y1, cx, ccy, rz)
-                    vof[k, js, :] += acc / ((x1 - x0)[None, :] * dy * dz)
+                        acc += ww * disc_rect_area(xa[None, :], xb[None, :], y0, y1, cx, ccy, rz)
+                    vof[k, js, i0:i1] += acc / ((xb - xa)[None, :] * dy * dz)
     return np.clip(vof, 0.0, 1.0)
 
 
