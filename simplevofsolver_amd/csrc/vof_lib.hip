@@ -2505,10 +2505,12 @@ static int tune_run(const std::vector<vof_ctx *> & cs, std::vector<std::unique_p
       rep += line;
     }
   }
-  // leave every case as it was
+  // leave every case as it was (also after a failed trial: the snapshot is
+  // put back whatever the trial left)
+  if (rc) (void)hipStreamSynchronize(st);
   for (auto & S : snaps)
-    if (!rc && tune_restore(*S, st)) rc = 1;
-  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail("autotune: stream failed");
+    if (tune_restore(*S, st)) rc = 1;
+  if (hipStreamSynchronize(st) != hipSuccess) rc = fail("autotune: stream failed");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (report && len) {
