@@ -334,7 +334,7 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
 #endif
 
 /* autotuner: picks "fft_radix", "fft_elems", "iface_planes" (3D),
- * "rhs_rows" and "lines_lpb" of vof_set_tuning by timing `steps` steps of
+ * "rhs_rows", "iface_overlap" and "lines_lpb" of vof_set_tuning by timing `steps` steps of
  * this case per candidate (coordinate descent in that order; a candidate
  * must be 1 % faster to replace the current value).  The case's device state
  * is copied first and restored after every trial, so it is left as it was,
@@ -354,6 +354,9 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * 8 (default) or 16 rows of one column per thread of the source kernel;
  * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
  * work-group by the 3D interface kernel;
+ * "iface_overlap" = 1 (default) runs the interface kernel of RK stages 2
+ * and 3 on a second stream, beside the previous stage's x solve, Poisson
+ * solve and correction, or 0 (one stream);
  * "fft_radix" = 16 (default: 256-point transforms on the register
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);

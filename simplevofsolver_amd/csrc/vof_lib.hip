@@ -226,6 +226,11 @@ struct vof_ctx {
   int rhs_rows;                          // rows marched per thread by the source kernel (1 = one DOF per thread)
   int iface_planes;                      // z planes marched per work-group by the 3D interface kernel (0 = blocks)
   int fft_radix;                         // 16: 256-point transforms on the register radix-16 path; 4: mixed radix
+  int iface_overlap;                     // 1: the next stage's interface kernel runs on `side` (see integrate)
+  hipStream_t side;                      // second stream of the fused schedule (default priority)
+  hipEvent_t ev_rhs, ev_iface, ev_main;  // source kernel done (main); interface kernel done (side); main so far
+  bool mark_rhs;                         // record ev_rhs after the next source kernel
+  bool iface_ready;                      // the first stage's interface kernel is already on the main stream
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
   Grid grid;
@@ -838,6 +843,7 @@ static int ctx_create(domain_t * domain) {
   ctx->fuse = 63;
   ctx->fft_elems = 1024;
   ctx->fft_radix = 16;
+  ctx->iface_overlap = 1;
   ctx->poisson_refine = -1;
   ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
@@ -860,6 +866,10 @@ static int ctx_create(domain_t * domain) {
   ctx->prm[P_T_XM] = +0.5; ctx->prm_set[P_T_XM] = true;
   ctx->prm[P_T_XP] = -0.5; ctx->prm_set[P_T_XP] = true;
   HIPCHK(hipStreamCreate(&ctx->stream));
+  HIPCHK(hipStreamCreate(&ctx->side));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_rhs, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_iface, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_main, hipEventDisableTiming));
   // metrics
   HIPCHK(hipMalloc(&ctx->d_dxf, sizeof(double) * nx));
   HIPCHK(hipMalloc(&ctx->d_dxc, sizeof(double) * (nx + 1)));
@@ -1073,6 +1083,11 @@ extern "C" int domain_finalise(domain_t * domain) {
     host_free(ctx->h_scalar);
     prof_flush(ctx);
     for (auto & e : ctx->ev_pool) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipEventDestroy(ctx->ev_rhs);
+    (void)hipEventDestroy(ctx->ev_iface);
+    (void)hipEventDestroy(ctx->ev_main);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     domain->info = nullptr;
@@ -1452,6 +1467,11 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
       default:
         KLB(ctx, "k_rhs_fused", 256, vofk::k_rhs_fused_body, vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
             ctx->grid, fr);
+    }
+    if (ctx->mark_rhs) {
+      // the last reader of the curvature and normals of this stage is done
+      HIPCHK(hipEventRecord(ctx->ev_rhs, ctx->stream));
+      ctx->mark_rhs = false;
     }
   }
   for (int f = 0; f < nj; f++) {
@@ -1853,14 +1873,74 @@ static int ensure_vof_spare(vof_ctx * ctx, const interface_t * interface) {
 static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
                                   const double dtv, size_t rk0 = 0, size_t rk1 = 3);
 
+static int launch_iface(vof_ctx * ctx, interface_t * interface);
+
+// the fused schedule may use the side stream (not while launches are
+// recorded for a batch, the adjoint records stage states, or per-kernel
+// timing is on)
+static bool overlap_ok(const vof_ctx * ctx) {
+  const int fz = ctx->fuse;
+  return ctx->iface_overlap && (fz & F_IFACE) && (fz & F_VOF) && (fz & F_RHS) && !ctx->rec && !ctx->tape &&
+         !ctx->prof_on;
+}
+
 static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
   vof_ctx * ctx = ctx_of(domain);
   if ((ctx->fuse & F_VOF) && ensure_vof_spare(ctx, interface)) return 1;
+  if (overlap_ok(ctx)) {
+    // the first stage's interface kernel needs no dt: it runs on the main
+    // stream while dt is reduced on the side stream (after everything
+    // issued before on the main stream, which may still write the velocity)
+    HIPCHK(hipEventRecord(ctx->ev_main, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_main, 0));
+    if (launch_iface(ctx, interface)) return 1;
+    hipStream_t main_stream = ctx->stream;
+    ctx->stream = ctx->side;
+    const int rc = fluid_decide_dt(domain, fluid, dt);
+    ctx->stream = main_stream;
+    if (rc) return 1;
+    ctx->iface_ready = true;
+    const int rs = integrate_fused_stages(domain, fluid, interface, *dt);
+    ctx->iface_ready = false;
+    return rs;
+  }
   if (fluid_decide_dt(domain, fluid, dt)) return 1;
   return integrate_fused_stages(domain, fluid, interface, *dt);
 }
 
 // RK stages rk0 .. rk1-1 (all three by default) of one step with dt known
+// fused curvature tensor (curvature_tensor.c:178-186) of the current vof
+// on ctx->stream
+static int launch_iface(vof_ctx * ctx, interface_t * interface) {
+  const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  const double * v = (const double *)interface->vof.data;
+  double * nrm = (double *)interface->normal.data;
+  double * cu = (double *)interface->curv.data;
+#if NDIMS == 3
+  const int kz = ctx->iface_planes;
+  const dim3 gm((unsigned)((nx + vofk::kMX - 1) / vofk::kMX), (unsigned)((ny + vofk::kMY - 1) / vofk::kMY),
+                (unsigned)((nz + (kz > 0 ? kz : 1) - 1) / (kz > 0 ? kz : 1)));
+  const dim3 bm(vofk::kMThreads);
+  if (kz == 4) {
+    KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<4>, vofk::k_iface_march<4>, gm, bm, 0,
+        ctx->grid, v, nrm, cu);
+  } else if (kz == 8) {
+    KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<8>, vofk::k_iface_march<8>, gm, bm, 0,
+        ctx->grid, v, nrm, cu);
+  } else if (kz == 16) {
+    KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<16>, vofk::k_iface_march<16>, gm, bm, 0,
+        ctx->grid, v, nrm, cu);
+  } else
+#endif
+  {
+    const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
+                  (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
+    KLB(ctx, "k_iface_fused", vofk::kIThreads, vofk::k_iface_fused_body, vofk::k_iface_fused, gi,
+        dim3(vofk::kIThreads), 0, ctx->grid, v, nrm, cu);
+  }
+  return 0;
+}
+
 static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, interface_t * interface,
                                   const double dtv, size_t rk0, size_t rk1) {
   vof_ctx * ctx = ctx_of(domain);
@@ -1871,6 +1951,13 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
 #else
   double * uz = nullptr;
 #endif
+  // the interface kernel of stage rk + 1 reads only the vof that stage rk's
+  // VOF update wrote, and writes the normals and curvature whose last reader
+  // in stage rk is the source kernel: it runs on the side stream from the end
+  // of that kernel, beside the x solve, the Poisson solve and the
+  // correction (not while launches are recorded for a batch, the adjoint
+  // records stage states, or per-kernel timing is on)
+  const bool overlap = overlap_ok(ctx);
   for (size_t rk = rk0; rk < rk1; rk++) {
     if (ctx->tape) {
       // the adjoint's recomputation records each stage's input state
@@ -1882,30 +1969,14 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
     }
     // curvature tensor (curvature_tensor.c:178-186)
     if (fz & F_IFACE) {
-      const double * v = (const double *)interface->vof.data;
-      double * nrm = (double *)interface->normal.data;
-      double * cu = (double *)interface->curv.data;
-#if NDIMS == 3
-      const int kz = ctx->iface_planes;
-      const dim3 gm((unsigned)((nx + vofk::kMX - 1) / vofk::kMX), (unsigned)((ny + vofk::kMY - 1) / vofk::kMY),
-                    (unsigned)((nz + (kz > 0 ? kz : 1) - 1) / (kz > 0 ? kz : 1)));
-      const dim3 bm(vofk::kMThreads);
-      if (kz == 4) {
-        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<4>, vofk::k_iface_march<4>, gm, bm, 0,
-            ctx->grid, v, nrm, cu);
-      } else if (kz == 8) {
-        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<8>, vofk::k_iface_march<8>, gm, bm, 0,
-            ctx->grid, v, nrm, cu);
-      } else if (kz == 16) {
-        KLB(ctx, "k_iface_march", vofk::kMThreads, vofk::k_iface_march_body<16>, vofk::k_iface_march<16>, gm, bm, 0,
-            ctx->grid, v, nrm, cu);
-      } else
-#endif
-      {
-        const dim3 gi((unsigned)((nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ny + vofk::kIY - 1) / vofk::kIY),
-                      (unsigned)((nz + vofk::kIZ - 1) / vofk::kIZ));
-        KLB(ctx, "k_iface_fused", vofk::kIThreads, vofk::k_iface_fused_body, vofk::k_iface_fused, gi,
-            dim3(vofk::kIThreads), 0, ctx->grid, v, nrm, cu);
+      if (rk != rk0 && overlap) {
+        // launched on the side stream during the previous stage
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_iface, 0));
+      } else if (rk == rk0 && ctx->iface_ready) {
+        // launched by integrate before dt was reduced
+        ctx->iface_ready = false;
+      } else if (launch_iface(ctx, interface)) {
+        return 1;
       }
     } else {
       if (interface_compute_curvature_tensor(domain, interface)) return 1;
@@ -1931,9 +2002,21 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
       if (interface_update_vof(domain, rk, dtv, fluid, interface)) return 1;
     }
     // predictor (predict/main.c:62-79)
+    const bool next_side = overlap && rk + 1 < rk1;
+    ctx->mark_rhs = next_side;
     if (predict_impl(domain, rk, dtv, fluid, interface, 1, rhs_fused ? vof_old : nullptr,
                      (fz & F_LINES) ? 1 : 0))
       return 1;
+    if (next_side) {
+      if (ctx->mark_rhs) return fail("integrate: the source kernel did not mark its end");
+      HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_rhs, 0));
+      hipStream_t main_stream = ctx->stream;
+      ctx->stream = ctx->side;
+      const int rc = launch_iface(ctx, interface);
+      ctx->stream = main_stream;
+      if (rc) return 1;
+      HIPCHK(hipEventRecord(ctx->ev_iface, ctx->side));
+    }
     // Poisson (both reference solvers are the same discrete problem)
     if (potential(domain, rk, dtv, fluid, (fz & F_PSI) ? 1 : 0)) return 1;
     // correction and pressure (correct_velocity/main.c:15-27, update_pressure.c:87-109)
@@ -1985,6 +2068,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     if (value != 1024 && value != 2048) return fail("fft_elems must be 1024 or 2048");
     if (value < ctx->ny || (NDIMS == 3 && value < ctx->nz)) return fail("fft_elems below the FFT length");
     ctx->fft_elems = value;
+    return 0;
+  }
+  if (!strcmp(name, "iface_overlap")) {
+    if (value != 0 && value != 1) return fail("iface_overlap must be 0 or 1");
+    ctx->iface_overlap = value;
     return 0;
   }
   if (!strcmp(name, "fft_radix")) {
@@ -2450,6 +2538,7 @@ static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
   k.push_back({"iface_planes", {8, 0, 4, 16}, &vof_ctx::iface_planes});
 #endif
   k.push_back({"rhs_rows", {8, 1, 2, 4, 16}, &vof_ctx::rhs_rows});
+  k.push_back({"iface_overlap", {1, 0}, &vof_ctx::iface_overlap});
   k.push_back({"lines_lpb", {0, 2, 4, 8, 16, 32}, &vof_ctx::lines_lpb});
   return k;
 }
