@@ -355,8 +355,9 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
  * work-group by the 3D interface kernel;
  * "iface_overlap" = 1 (default) runs the interface kernel of RK stages 2
- * and 3 on a second stream, beside the previous stage's x solve, Poisson
- * solve and correction, or 0 (one stream);
+ * and 3 on a second stream from the end of the previous stage's source
+ * kernel (beside its x solve, Poisson solve and correction), 2 from the
+ * start of its Thomas pass, or 0 (one stream);
  * "fft_radix" = 16 (default: 256-point transforms on the register
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);

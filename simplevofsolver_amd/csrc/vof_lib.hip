@@ -228,8 +228,8 @@ struct vof_ctx {
   int fft_radix;                         // 16: 256-point transforms on the register radix-16 path; 4: mixed radix
   int iface_overlap;                     // 1: the next stage's interface kernel runs on `side` (see integrate)
   hipStream_t side;                      // second stream of the fused schedule (default priority)
-  hipEvent_t ev_rhs, ev_iface, ev_main;  // source kernel done (main); interface kernel done (side); main so far
-  bool mark_rhs;                         // record ev_rhs after the next source kernel
+  hipEvent_t ev_rhs, ev_iface, ev_main;  // side launch point (main); interface kernel done (side); main so far
+  const interface_t * pending_iface;     // the next stage's interface kernel, to go on the side stream
   bool iface_ready;                      // the first stage's interface kernel is already on the main stream
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
@@ -1367,6 +1367,24 @@ static void swap_data(array_t * a, array_t * b) {
 // vof_old / curv non-null: the sources of all fields come from one fused
 // kernel that forms the CSF force from them (fused schedule); images: the
 // line solver writes the halo images itself instead of boundary launches.
+static int launch_iface(vof_ctx * ctx, interface_t * interface);
+
+// the next stage's interface kernel on the side stream, from this point of
+// the main stream on (its readers of the normals and curvature are done)
+static int side_iface(vof_ctx * ctx) {
+  interface_t * itf = const_cast<interface_t *>(ctx->pending_iface);
+  ctx->pending_iface = nullptr;
+  HIPCHK(hipEventRecord(ctx->ev_rhs, ctx->stream));
+  HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_rhs, 0));
+  hipStream_t main_stream = ctx->stream;
+  ctx->stream = ctx->side;
+  const int rc = launch_iface(ctx, itf);
+  ctx->stream = main_stream;
+  if (rc) return 1;
+  HIPCHK(hipEventRecord(ctx->ev_iface, ctx->side));
+  return 0;
+}
+
 static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid,
                         const interface_t * interface, int fused, const double * vof_old = nullptr,
                         int images = 0) {
@@ -1468,11 +1486,8 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
         KLB(ctx, "k_rhs_fused", 256, vofk::k_rhs_fused_body, vofk::k_rhs_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
             ctx->grid, fr);
     }
-    if (ctx->mark_rhs) {
-      // the last reader of the curvature and normals of this stage is done
-      HIPCHK(hipEventRecord(ctx->ev_rhs, ctx->stream));
-      ctx->mark_rhs = false;
-    }
+    // the last reader of the curvature of this stage is done
+    if (ctx->pending_iface && ctx->iface_overlap == 1 && side_iface(ctx)) return 1;
   }
   for (int f = 0; f < nj; f++) {
     const int kind = jobs[f].kind;
@@ -1653,6 +1668,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) {
+      if (ctx->pending_iface && ctx->iface_overlap == 2 && side_iface(ctx)) return 1;
       const int nlines = ctx->nh * nz;
       KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
           dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
@@ -1671,6 +1687,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   }
 #else
   {
+    if (ctx->pending_iface && ctx->iface_overlap == 2 && side_iface(ctx)) return 1;
     const int nlines = ctx->nh * nz;
     KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
         dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
@@ -2001,24 +2018,21 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
     } else {
       if (interface_update_vof(domain, rk, dtv, fluid, interface)) return 1;
     }
-    // predictor (predict/main.c:62-79)
-    const bool next_side = overlap && rk + 1 < rk1;
-    ctx->mark_rhs = next_side;
+    // predictor (predict/main.c:62-79); the next stage's interface kernel
+    // goes to the side stream after the source kernel (iface_overlap 1) or
+    // before the Thomas pass of the Poisson solve (2)
+    ctx->pending_iface = (overlap && rk + 1 < rk1) ? interface : nullptr;
     if (predict_impl(domain, rk, dtv, fluid, interface, 1, rhs_fused ? vof_old : nullptr,
-                     (fz & F_LINES) ? 1 : 0))
+                     (fz & F_LINES) ? 1 : 0)) {
+      ctx->pending_iface = nullptr;
       return 1;
-    if (next_side) {
-      if (ctx->mark_rhs) return fail("integrate: the source kernel did not mark its end");
-      HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_rhs, 0));
-      hipStream_t main_stream = ctx->stream;
-      ctx->stream = ctx->side;
-      const int rc = launch_iface(ctx, interface);
-      ctx->stream = main_stream;
-      if (rc) return 1;
-      HIPCHK(hipEventRecord(ctx->ev_iface, ctx->side));
     }
     // Poisson (both reference solvers are the same discrete problem)
-    if (potential(domain, rk, dtv, fluid, (fz & F_PSI) ? 1 : 0)) return 1;
+    if (potential(domain, rk, dtv, fluid, (fz & F_PSI) ? 1 : 0)) {
+      ctx->pending_iface = nullptr;
+      return 1;
+    }
+    if (ctx->pending_iface && side_iface(ctx)) return 1;
     // correction and pressure (correct_velocity/main.c:15-27, update_pressure.c:87-109)
     if (fz & F_CORR) {
       const double pref_c = kRk[rk][2] * dtv;
@@ -2071,7 +2085,7 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
     return 0;
   }
   if (!strcmp(name, "iface_overlap")) {
-    if (value != 0 && value != 1) return fail("iface_overlap must be 0 or 1");
+    if (value < 0 || value > 2) return fail("iface_overlap must be 0, 1 or 2");
     ctx->iface_overlap = value;
     return 0;
   }
@@ -2538,7 +2552,7 @@ static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
   k.push_back({"iface_planes", {8, 0, 4, 16}, &vof_ctx::iface_planes});
 #endif
   k.push_back({"rhs_rows", {8, 1, 2, 4, 16}, &vof_ctx::rhs_rows});
-  k.push_back({"iface_overlap", {1, 0}, &vof_ctx::iface_overlap});
+  k.push_back({"iface_overlap", {1, 2, 0}, &vof_ctx::iface_overlap});
   k.push_back({"lines_lpb", {0, 2, 4, 8, 16, 32}, &vof_ctx::lines_lpb});
   return k;
 }
