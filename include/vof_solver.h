@@ -354,10 +354,10 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * 8 (default) or 16 rows of one column per thread of the source kernel;
  * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
  * work-group by the 3D interface kernel;
- * "iface_overlap" = 1 (default) runs the interface kernel of RK stages 2
- * and 3 on a second stream from the end of the previous stage's source
- * kernel (beside its x solve, Poisson solve and correction), 2 from the
- * start of its Thomas pass, or 0 (one stream);
+ * "iface_overlap" = 1 runs the interface kernel of RK stages 2 and 3 on a
+ * second stream from the end of the previous stage's source kernel (beside
+ * its x solve, Poisson solve and correction), 2 from the start of its
+ * Thomas pass, 0 on one stream (default: 1 from 2^20 cells on, else 0);
  * "fft_radix" = 16 (default: 256-point transforms on the register
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);

@@ -843,7 +843,9 @@ static int ctx_create(domain_t * domain) {
   ctx->fuse = 63;
   ctx->fft_elems = 1024;
   ctx->fft_radix = 16;
-  ctx->iface_overlap = 1;
+  // two streams pay from about 2^20 cells; below, the launches of a step
+  // are short and the cross-stream events cost more than the overlap gains
+  ctx->iface_overlap = (size_t)ctx->nx * (size_t)ctx->ny * (size_t)ctx->nz >= ((size_t)1 << 20) ? 1 : 0;
   ctx->poisson_refine = -1;
   ctx->adj_floor = 1e-9;
   ctx->lines_lpb = 0;
@@ -866,10 +868,6 @@ static int ctx_create(domain_t * domain) {
   ctx->prm[P_T_XM] = +0.5; ctx->prm_set[P_T_XM] = true;
   ctx->prm[P_T_XP] = -0.5; ctx->prm_set[P_T_XP] = true;
   HIPCHK(hipStreamCreate(&ctx->stream));
-  HIPCHK(hipStreamCreate(&ctx->side));
-  HIPCHK(hipEventCreateWithFlags(&ctx->ev_rhs, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ctx->ev_iface, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ctx->ev_main, hipEventDisableTiming));
   // metrics
   HIPCHK(hipMalloc(&ctx->d_dxf, sizeof(double) * nx));
   HIPCHK(hipMalloc(&ctx->d_dxc, sizeof(double) * (nx + 1)));
@@ -1083,11 +1081,13 @@ extern "C" int domain_finalise(domain_t * domain) {
     host_free(ctx->h_scalar);
     prof_flush(ctx);
     for (auto & e : ctx->ev_pool) (void)hipEventDestroy(e);
-    (void)hipStreamSynchronize(ctx->side);
-    (void)hipEventDestroy(ctx->ev_rhs);
-    (void)hipEventDestroy(ctx->ev_iface);
-    (void)hipEventDestroy(ctx->ev_main);
-    (void)hipStreamDestroy(ctx->side);
+    if (ctx->side) {
+      (void)hipStreamSynchronize(ctx->side);
+      (void)hipEventDestroy(ctx->ev_rhs);
+      (void)hipEventDestroy(ctx->ev_iface);
+      (void)hipEventDestroy(ctx->ev_main);
+      (void)hipStreamDestroy(ctx->side);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     domain->info = nullptr;
@@ -1901,10 +1901,23 @@ static bool overlap_ok(const vof_ctx * ctx) {
          !ctx->prof_on;
 }
 
+// the side stream and its events, created on first use (a context that
+// never overlaps keeps one stream: with few hardware queues, idle extra
+// streams still take queue slots from the main streams of other contexts)
+static int ensure_side(vof_ctx * ctx) {
+  if (ctx->side) return 0;
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_rhs, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_iface, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_main, hipEventDisableTiming));
+  HIPCHK(hipStreamCreate(&ctx->side));
+  return 0;
+}
+
 static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
   vof_ctx * ctx = ctx_of(domain);
   if ((ctx->fuse & F_VOF) && ensure_vof_spare(ctx, interface)) return 1;
   if (overlap_ok(ctx)) {
+    if (ensure_side(ctx)) return 1;
     // the first stage's interface kernel needs no dt: it runs on the main
     // stream while dt is reduced on the side stream (after everything
     // issued before on the main stream, which may still write the velocity)
@@ -1975,6 +1988,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
   // correction (not while launches are recorded for a batch, the adjoint
   // records stage states, or per-kernel timing is on)
   const bool overlap = overlap_ok(ctx);
+  if (overlap && ensure_side(ctx)) return 1;
   for (size_t rk = rk0; rk < rk1; rk++) {
     if (ctx->tape) {
       // the adjoint's recomputation records each stage's input state
