@@ -13,9 +13,13 @@
 #include <errno.h>
 #include <sys/stat.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -438,7 +442,71 @@ struct Recorder {
   std::vector<LaunchRec> recs;
 };
 
+// Host worker threads of a batch: run(n, f) calls f(0 .. n-1) spread over
+// the workers and the calling thread and returns when all calls are done.
+// Recording a case's launches touches only that case's context, recorder
+// and table slots, so the cases of a phase are recorded in parallel.
+struct HostPool {
+  std::vector<std::thread> th;
+  std::mutex m;
+  std::condition_variable cv, done;
+  const std::function<void(int)> * job = nullptr;
+  int njobs = 0;
+  std::atomic<int> next{0};
+  int active = 0;
+  unsigned gen = 0;
+  bool stop = false;
+
+  void work() {
+    for (int i; (i = next.fetch_add(1)) < njobs;) (*job)(i);
+  }
+  void loop() {
+    unsigned seen = 0;
+    std::unique_lock<std::mutex> l(m);
+    for (;;) {
+      cv.wait(l, [&] { return stop || gen != seen; });
+      if (stop) return;
+      seen = gen;
+      l.unlock();
+      work();
+      l.lock();
+      if (--active == 0) done.notify_all();
+    }
+  }
+  void start(int nthreads) {
+    for (int t = 0; t < nthreads; t++) th.emplace_back([this] { loop(); });
+  }
+  void run(int n, const std::function<void(int)> & f) {
+    if (th.empty() || n < 2) {
+      for (int i = 0; i < n; i++) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> l(m);
+      job = &f;
+      njobs = n;
+      next = 0;
+      active = (int)th.size();
+      gen++;
+    }
+    cv.notify_all();
+    work();
+    std::unique_lock<std::mutex> l(m);
+    done.wait(l, [&] { return active == 0; });
+    job = nullptr;
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> l(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto & t : th) t.join();
+  }
+};
+
 struct vof_batch {
+  HostPool pool;
   std::vector<vof_ctx *> ctx;
   std::vector<const domain_t *> domain;
   std::vector<fluid_t *> fluid;
@@ -2362,6 +2430,14 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
     vof_batch_destroy(B);
     return 1;
   }
+  {
+    // up to 8 recording workers (the calling thread records too)
+    const unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)(hw > 1 ? hw / 2 : 1);
+    nt = nt > 8 ? 8 : nt;
+    nt = nt > ncases - 1 ? ncases - 1 : nt;
+    if (nt > 0) B->pool.start(nt);
+  }
   *out = B;
   return 0;
 }
@@ -2369,15 +2445,27 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
 extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
   if (!B) return fail("vof_batch_integrate: batch is NULL");
   const size_t n = B->ctx.size();
-  int rc = 0;
+  // each case's launches are recorded on a worker thread; an error message
+  // (thread-local) is carried back to this thread
+  std::vector<int> rcs(n, 0);
+  std::vector<std::string> errs(n);
+  auto gather_rc = [&]() -> int {
+    for (size_t c = 0; c < n; c++)
+      if (rcs[c]) {
+        g_err = errs[c];
+        return 1;
+      }
+    return 0;
+  };
   // dt of every case (decide_dt.c:104-145): batched reductions, one copy back
-  for (size_t c = 0; c < n && !rc; c++) {
+  B->pool.run((int)n, [&](int c) {
     B->rec[c].recs.clear();
     B->ctx[c]->rec = &B->rec[c];
-    rc = decide_dt_enqueue(B->ctx[c], B->fluid[c]);
+    rcs[c] = decide_dt_enqueue(B->ctx[c], B->fluid[c]);
     B->ctx[c]->rec = nullptr;
-  }
-  if (rc) return 1;
+    if (rcs[c]) errs[c] = g_err;
+  });
+  if (gather_rc()) return 1;
   auto gather = [&]() -> int {
     hipLaunchKernelGGL(k_gather_dt, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, B->stream,
                        (const double * const *)B->d_dt_src, (int)n, B->d_dt);
@@ -2396,13 +2484,14 @@ extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
   // host records stage rk + 1 while the device runs stage rk
   const size_t tab_stage = B->tab_doubles / 3;
   for (size_t rk = 0; rk < 3; rk++) {
-    for (size_t c = 0; c < n && !rc; c++) {
+    B->pool.run((int)n, [&](int c) {
       B->rec[c].recs.clear();
       B->ctx[c]->rec = &B->rec[c];
-      rc = integrate_fused_stages(B->domain[c], B->fluid[c], B->interface[c], dt[c], rk, rk + 1);
+      rcs[c] = integrate_fused_stages(B->domain[c], B->fluid[c], B->interface[c], dt[c], rk, rk + 1);
       B->ctx[c]->rec = nullptr;
-    }
-    if (rc) return 1;
+      if (rcs[c]) errs[c] = g_err;
+    });
+    if (gather_rc()) return 1;
     auto tables = [&]() -> int {
       HIPCHK(hipMemcpyAsync(B->d_tab + rk * tab_stage, B->h_tab + rk * tab_stage, sizeof(double) * tab_stage,
                             hipMemcpyHostToDevice, B->stream));
