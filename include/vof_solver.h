@@ -334,7 +334,8 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
 #endif
 
 /* autotuner: picks "fft_radix", "fft_elems", "iface_planes" (3D),
- * "rhs_rows", "iface_overlap" and "lines_lpb" of vof_set_tuning by timing `steps` steps of
+ * "rhs_rows", "iface_overlap", "thomas_rows" and "lines_lpb" of
+ * vof_set_tuning by timing `steps` steps of
  * this case per candidate (coordinate descent in that order; a candidate
  * must be 1 % faster to replace the current value).  The case's device state
  * is copied first and restored after every trial, so it is left as it was,
@@ -358,6 +359,8 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * second stream from the end of the previous stage's source kernel (beside
  * its x solve, Poisson solve and correction), 2 from the start of its
  * Thomas pass, 0 on one stream (default: 1 from 2^20 cells on, else 0);
+ * "thomas_rows" = 8 (default), 16 or 32 rows of the spectrum prefetched per
+ * lane by the Poisson solver's x eliminations;
  * "fft_radix" = 16 (default: 256-point transforms on the register
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);

@@ -1778,9 +1778,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_z(Grid g, FftPlan P, const 
 // eliminated as gtsv (tdm.c:17-50).  v goes to vbuf[x][line] when store_v
 // (the factors depend only on the grid and the wavenumbers, so they are
 // written by the first solve and read back by every later one).  One
-// lane per line; the rows of the next chunk of kTsU are loaded before the
-// current chunk is eliminated, so kTsU loads per lane stay in flight.
-constexpr int kTsU = 8;
+// lane per line; the rows of the next chunk of U rows are loaded before the
+// current chunk is eliminated, so U loads per lane stay in flight.
+constexpr int kTsU = 8;   // default rows per chunk (the U of the templates below)
 
 __device__ inline double2 to_d2(double2 a) { return a; }
 __device__ inline double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
@@ -1790,7 +1790,7 @@ template <> __device__ inline float2 from_d2<float2>(double2 a) { return make_fl
 
 // T2 is the storage type of the spectrum (double2, or float2 in the
 // mixed-precision solver); the elimination itself is FP64 either way
-template <class T2>
+template <class T2, int U = kTsU>
 __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
@@ -1821,20 +1821,20 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
   {
     // rows are kept in the storage type until used: a conversion right
     // after a conditional load would wait for that load and serialise them
-    T2 cur[kTsU];
+    T2 cur[U];
 #pragma unroll
-    for (int u = 0; u < kTsU; u++) {
+    for (int u = 0; u < U; u++) {
       if (1 + u < n - 1) cur[u] = q[(size_t)(1 + u) * st];
     }
-    for (int i0 = 1; i0 < n - 1; i0 += kTsU) {
-      T2 nxt[kTsU];
+    for (int i0 = 1; i0 < n - 1; i0 += U) {
+      T2 nxt[U];
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) {
-        const int i = i0 + kTsU + u;
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + U + u;
         if (i < n - 1) nxt[u] = q[(size_t)i * st];
       }
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) {
+      for (int u = 0; u < U; u++) {
         const int i = i0 + u;
         if (i < n - 1) {
           const double c = -pl[i] - pu[i] + eval;
@@ -1850,7 +1850,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
         }
       }
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) cur[u] = nxt[u];
+      for (int u = 0; u < U; u++) cur[u] = nxt[u];
     }
   }
   {
@@ -1876,29 +1876,29 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
     qprev = qi;
   }
   {
-    T2 cq[kTsU];
-    double cv[kTsU];
+    T2 cq[U];
+    double cv[U];
 #pragma unroll
-    for (int u = 0; u < kTsU; u++) {
+    for (int u = 0; u < U; u++) {
       const int i = n - 2 - u;
       if (i >= 0) {
         cq[u] = q[(size_t)i * st];
         cv[u] = v[(size_t)i * st];
       }
     }
-    for (int i0 = n - 2; i0 >= 0; i0 -= kTsU) {
-      T2 nq[kTsU];
-      double nv[kTsU];
+    for (int i0 = n - 2; i0 >= 0; i0 -= U) {
+      T2 nq[U];
+      double nv[U];
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) {
-        const int i = i0 - kTsU - u;
+      for (int u = 0; u < U; u++) {
+        const int i = i0 - U - u;
         if (i >= 0) {
           nq[u] = q[(size_t)i * st];
           nv[u] = v[(size_t)i * st];
         }
       }
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) {
+      for (int u = 0; u < U; u++) {
         const int i = i0 - u;
         if (i >= 0) {
           double2 qi = to_d2(cq[u]);
@@ -1909,7 +1909,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
         }
       }
 #pragma unroll
-      for (int u = 0; u < kTsU; u++) {
+      for (int u = 0; u < U; u++) {
         cq[u] = nq[u];
         cv[u] = nv[u];
       }
@@ -1917,12 +1917,12 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
   }
 }
 
-template <class T2>
+template <class T2, int U = kTsU>
 __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
                               double * __restrict__ vbuf, int store_v) {
-  k_thomas_spec_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf, store_v);
+  k_thomas_spec_body<T2, U>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf, store_v);
 }
 
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and

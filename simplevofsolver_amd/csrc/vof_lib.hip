@@ -230,6 +230,7 @@ struct vof_ctx {
   int rhs_rows;                          // rows marched per thread by the source kernel (1 = one DOF per thread)
   int iface_planes;                      // z planes marched per work-group by the 3D interface kernel (0 = blocks)
   int fft_radix;                         // 16: 256-point transforms on the register radix-16 path; 4: mixed radix
+  int thomas_rows;                       // rows per prefetched chunk of the spectral Thomas kernel (8, 16, 32)
   int iface_overlap;                     // 1: the next stage's interface kernel runs on `side` (see integrate)
   hipStream_t side;                      // second stream of the fused schedule (default priority)
   hipEvent_t ev_rhs, ev_iface, ev_main;  // side launch point (main); interface kernel done (side); main so far
@@ -911,6 +912,7 @@ static int ctx_create(domain_t * domain) {
   ctx->fuse = 63;
   ctx->fft_elems = 1024;
   ctx->fft_radix = 16;
+  ctx->thomas_rows = 8;
   // two streams pay from about 2^20 cells; below, the launches of a step
   // are short and the cross-stream events cost more than the overlap gains
   ctx->iface_overlap = (size_t)ctx->nx * (size_t)ctx->ny * (size_t)ctx->nz >= ((size_t)1 << 20) ? 1 : 0;
@@ -1631,6 +1633,26 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 // mixed-precision solve: FP32 transforms, FP64 eliminations, and
 // `poisson_refine` FP64 residual corrections; psi always carries its halo
 // images and Neumann walls (the residual reads them)
+// x eliminations of the spectrum (k_thomas_spec with ctx->thomas_rows rows
+// per prefetched chunk; the arithmetic does not depend on it)
+template <class T2>
+static int launch_thomas(vof_ctx * ctx, const char * name, T2 * spec, int nlines) {
+  const dim3 g((unsigned)((nlines + 63) / 64)), b(64);
+#define TS_ARGS ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, spec, ctx->d_vbuf, vbuf_store(ctx)
+  switch (ctx->thomas_rows) {
+    case 16:
+      KLB(ctx, name, 64, (vofk::k_thomas_spec_body<T2, 16>), (vofk::k_thomas_spec<T2, 16>), g, b, 0, TS_ARGS);
+      break;
+    case 32:
+      KLB(ctx, name, 64, (vofk::k_thomas_spec_body<T2, 32>), (vofk::k_thomas_spec<T2, 32>), g, b, 0, TS_ARGS);
+      break;
+    default:
+      KLB(ctx, name, 64, (vofk::k_thomas_spec_body<T2, 8>), (vofk::k_thomas_spec<T2, 8>), g, b, 0, TS_ARGS);
+  }
+#undef TS_ARGS
+  return 0;
+}
+
 static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fluid) {
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
   const double prefactor = 1. / (kRk[rkstep][2] * dt);
@@ -1667,11 +1689,7 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
 #undef RES_ARGS
 #if NDIMS == 3
     for (int zp = 0; zp < 2; zp++) {
-      if (zp == 1) {
-        KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>,
-            dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez,
-            ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
-      }
+      if (zp == 1 && launch_thomas<float2>(ctx, "k_thomas_spec32", ctx->d_spec32, nlines)) return 1;
       const float sign = zp == 0 ? -1.f : +1.f;
       if (f256z) {
         KLB(ctx, "k_fft32_z", vofk::kFftThreads, (vofk::k_fft_z_body<1024, float2, true>),
@@ -1683,9 +1701,7 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
       }
     }
 #else
-    KLB(ctx, "k_thomas_spec32", 64, vofk::k_thomas_spec_body<float2>, vofk::k_thomas_spec<float2>,
-        dim3((nlines + 63) / 64), dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez,
-        ctx->d_spec32, ctx->d_vbuf, vbuf_store(ctx));
+    if (launch_thomas<float2>(ctx, "k_thomas_spec32", ctx->d_spec32, nlines)) return 1;
 #endif
 #define IRF32_ARGS ctx->grid, ctx->plan_y, ctx->d_twy32, (const float2 *)ctx->d_spec32, psi, pass == 0 ? 0 : 1
     if (f256y) {
@@ -1737,9 +1753,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) {
       if (ctx->pending_iface && ctx->iface_overlap == 2 && side_iface(ctx)) return 1;
-      const int nlines = ctx->nh * nz;
-      KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
-          dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
+      if (launch_thomas<double2>(ctx, "k_thomas_spec", ctx->d_spec, ctx->nh * nz)) return 1;
     }
     const double sign = pass == 0 ? -1. : +1.;
     if (f256z) {
@@ -1756,9 +1770,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
 #else
   {
     if (ctx->pending_iface && ctx->iface_overlap == 2 && side_iface(ctx)) return 1;
-    const int nlines = ctx->nh * nz;
-    KLB(ctx, "k_thomas_spec", 64, vofk::k_thomas_spec_body<double2>, vofk::k_thomas_spec<double2>, dim3((nlines + 63) / 64),
-        dim3(64), 0, nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
+    if (launch_thomas<double2>(ctx, "k_thomas_spec", ctx->d_spec, ctx->nh * nz)) return 1;
   }
 #endif
 #define IRF_ARGS ctx->grid, ctx->plan_y, ctx->d_twy, ctx->d_spec, (double *)fluid->psi.data, images
@@ -2169,6 +2181,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   if (!strcmp(name, "iface_overlap")) {
     if (value < 0 || value > 2) return fail("iface_overlap must be 0, 1 or 2");
     ctx->iface_overlap = value;
+    return 0;
+  }
+  if (!strcmp(name, "thomas_rows")) {
+    if (value != 8 && value != 16 && value != 32) return fail("thomas_rows must be 8, 16 or 32");
+    ctx->thomas_rows = value;
     return 0;
   }
   if (!strcmp(name, "fft_radix")) {
@@ -2656,6 +2673,7 @@ static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
 #endif
   k.push_back({"rhs_rows", {8, 1, 2, 4, 16}, &vof_ctx::rhs_rows});
   k.push_back({"iface_overlap", {1, 2, 0}, &vof_ctx::iface_overlap});
+  k.push_back({"thomas_rows", {8, 16, 32}, &vof_ctx::thomas_rows});
   k.push_back({"lines_lpb", {0, 2, 4, 8, 16, 32}, &vof_ctx::lines_lpb});
   return k;
 }

@@ -21,7 +21,8 @@ from simplevofsolver_amd.ic import random_fields
 pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"fft_radix": {4, 16}, "fft_elems": {1024, 2048}, "iface_planes": {0, 4, 8, 16},
-              "rhs_rows": {1, 2, 4, 8, 16}, "lines_lpb": {0, 2, 4, 8, 16, 32}, "iface_overlap": {0, 1, 2}}
+              "rhs_rows": {1, 2, 4, 8, 16}, "lines_lpb": {0, 2, 4, 8, 16, 32}, "iface_overlap": {0, 1, 2},
+              "thomas_rows": {8, 16, 32}}
 
 
 def _names(nd):

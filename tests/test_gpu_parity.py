@@ -183,7 +183,8 @@ def test_fused_schedule_is_bit_identical(case, mask, tmp_path):
 @pytest.mark.parametrize("tuning", [("fft_elems", 2048), ("lines_lpb", 1), ("lines_lpb", 7), ("lines_lpb", 15), ("xcd_bands", 1),
                                     ("rhs_rows", 1), ("rhs_rows", 2), ("rhs_rows", 4), ("rhs_rows", 16),
                                     ("iface_planes", 0), ("iface_planes", 4), ("iface_planes", 16),
-                                    ("iface_overlap", 0), ("iface_overlap", 1), ("iface_overlap", 2)])
+                                    ("iface_overlap", 0), ("iface_overlap", 1), ("iface_overlap", 2),
+                                    ("thomas_rows", 16), ("thomas_rows", 32)])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
     """Tile sizes change the work split only: every line and every FFT column
