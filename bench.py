@@ -302,17 +302,31 @@ def batched_form(sva, ncases, steps, autotune=2):
                 b.integrate()
             b.synchronize()
             el = time.perf_counter() - t0
+            # the FP32-Poisson variant of the same batch (FP32 transforms, FP64
+            # x eliminations, one FP64 refinement): the phases are captured anew
+            for s in solvers:
+                s.set_poisson_mixed(1)
+            b.integrate()
+            b.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.integrate()
+            b.synchronize()
+            el32 = time.perf_counter() - t0
         finally:
             b.close()
     finally:
         for s in solvers:
             s.close()
     v = cells * steps / el / 1e6
+    v32 = cells * steps / el32 / 1e6
     print(f"[bench] batched form {v:.1f} Mcell-steps/s ({el / steps * 1e3:.3f} ms/step), "
-          f"one by one {seq:.1f}", file=sys.stderr, flush=True)
+          f"one by one {seq:.1f}; FP32-Poisson variant {v32:.1f}", file=sys.stderr, flush=True)
     return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
             "sequential_value": round(seq, 3), "vs_sequential": round(v / seq, 3), "cases": ncases,
             "cells": cells, "steps": steps, "autotuned": tuned,
+            "fp32_poisson": {"value": round(v32, 3), "ms_per_step": round(el32 / steps * 1e3, 4),
+                             "vs_fp64_batch": round(v32 / v, 4), "refinements": 1},
             "workload": f"{ncases} 2D cases, nx in {{64,96,128,192,256}}, ny = 2 nx, 1-4 droplets, "
                         "Re in [50,500], We in [5,50], FP64"}
 
