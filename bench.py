@@ -331,7 +331,7 @@ def batched_form(sva, ncases, steps, autotune=2):
                         "Re in [50,500], We in [5,50], FP64"}
 
 
-def adjoint_form(sva, nsteps=50, every=10, ndirs=16, n=256):
+def adjoint_form(sva, nsteps=50, every=10, ndirs=32, n=256):
     """Differentiation workload: 2D n x n, one droplet r = 0.2 in a
     divergence-free velocity (rms 0.05); loss = centroid x + interface
     length after nsteps; gradient w.r.t. the initial velocity by the adjoint
