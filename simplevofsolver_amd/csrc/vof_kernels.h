@@ -1565,7 +1565,8 @@ __device__ __forceinline__ void dft16(T2 * v, typename Cx<T2>::S sign) {
 // free); x[n] sits at row n / 16, column n % 16 (f256_in), Y[n2][k1] at
 // (k1, n2) and X[k] at (k % 16, k / 16) (f256_out).  Each thread owns one
 // matrix column in pass 1 and one row in pass 2, so both run in place.
-constexpr int kF256Cols = 8;      // transforms per work-group
+constexpr int kF256Cols = 8;      // transforms per work-group of the y kernels (16 x columns = 128-byte rows)
+constexpr int kF256ColsZ = 16;    // transforms per work-group of the z kernel (every thread of the FFT passes busy)
 constexpr int kF256Stride = 273;  // complex values per column: 16 x 17 plus one, so that the
                                   // columns of a work-group start in different LDS banks
 __device__ inline int f256_in(int n) { return (n >> 4) * 17 + (n & 15); }
@@ -1722,10 +1723,10 @@ __global__ __launch_bounds__(kFftThreads) void k_div_rfft_y(
 template <int ELEMS, class T2, bool F256 = false>
 __device__ __forceinline__ void k_fft_z_body(const VBlock & vb, Grid g, const FftPlan & P, const T2 * __restrict__ tw,
                                                       T2 * __restrict__ spec, int nh, typename Cx<T2>::S sign) {
-  __shared__ T2 sa[F256 ? kF256Cols * kF256Stride : ELEMS];
+  __shared__ T2 sa[F256 ? kF256ColsZ * kF256Stride : ELEMS];
   __shared__ T2 sb[F256 ? 1 : ELEMS];
   const int N = g.nz;
-  const int NC = F256 ? kF256Cols : ELEMS / N;
+  const int NC = F256 ? kF256ColsZ : ELEMS / N;
   const int CS = F256 ? kF256Stride : N;
   const Tile3 tile = xcd_tile(vb);
   const int ky0 = tile.x * NC;

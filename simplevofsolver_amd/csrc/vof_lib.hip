@@ -1666,7 +1666,7 @@ static int potential_mixed(vof_ctx * ctx, size_t rkstep, double dt, fluid_t * fl
 #if NDIMS == 3
   const double * uz = (const double *)fluid->uz.data;
   const bool f256z = nz == 256 && ctx->fft_radix == 16;
-  const int NCz = f256z ? vofk::kF256Cols : elems / nz;
+  const int NCz = f256z ? vofk::kF256ColsZ : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
 #else
   const double * uz = nullptr;
@@ -1748,7 +1748,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
 #undef DIV_ARGS
 #if NDIMS == 3
   const bool f256z = nz == 256 && ctx->fft_radix == 16;
-  const int NCz = f256z ? vofk::kF256Cols : elems / nz;
+  const int NCz = f256z ? vofk::kF256ColsZ : elems / nz;
   const dim3 gz((ctx->nh + NCz - 1) / NCz, nx);
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) {
