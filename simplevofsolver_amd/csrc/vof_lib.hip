@@ -1725,7 +1725,7 @@ static int potential(const domain_t * domain, size_t rkstep, double dt, fluid_t 
   const double prefactor = 1. / (kRk[rkstep][2] * dt) / norm;
   const int elems = (ctx->fft_elems == 1024 && ny <= 1024 && (NDIMS == 2 || nz <= 1024)) ? 1024 : 2048;
   // 256-point transforms take the register radix-16 path (8 columns per
-  // work-group), other lengths the mixed-radix Stockham path
+  // work-group in y, 16 in z), other lengths the mixed-radix Stockham path
   const bool f256y = ny == 256 && ctx->fft_radix == 16;
   const int NC = f256y ? vofk::kF256Cols : elems / ny;
   const int nbx = (nx + 2 * NC - 1) / (2 * NC);
