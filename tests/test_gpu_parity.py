@@ -1,7 +1,7 @@
 """GPU parity: every entry point of the HIP library against the CPU oracle
 on the same seeded inputs, through the C ABI.
 
-Bars (BASELINE north star): FP64 fields within 1e-10 relative after a full
+Bars: FP64 fields within 1e-10 relative after a full
 step; per entry point, on identical inputs, results that are bit-identical
 where the arithmetic is the same sequence of IEEE operations (stencils,
 Thomas with shared factors, reductions by min) and within 1e-13 relative where
