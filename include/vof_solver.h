@@ -179,11 +179,14 @@ int integrate(const domain_t * domain, fluid_t * fluid, interface_t * interface,
 
 /* ---- tri-diagonal solver (replaces include/tdm.h:6-45) ------------------- */
 
-/* l, c, u are HOST arrays owned by the info object (filled by the caller
- * through get_l/get_c/get_u, as in the reference); solve() takes a DEVICE
- * pointer to nrhs contiguous lines of `size` values (double, or interleaved
- * (re, im) pairs when is_complex) and runs the batched GPU Thomas /
- * Sherman-Morrison kernels on the context-free default stream. */
+/* src/tdm.c:147-324.  l, c, u are HOST arrays owned by the info object
+ * (filled by the caller through get_l/get_c/get_u, as in the reference);
+ * solve() takes a DEVICE pointer to nrhs contiguous lines of `size` values
+ * (double, or interleaved (re, im) pairs when is_complex), runs the GPU
+ * Thomas / Sherman-Morrison kernels on the object's own stream (blocking
+ * w.r.t. the null stream) and returns when the answers are in place, bit for
+ * bit the reference's.  construct() also rejects sizes the reference's loops
+ * would read out of bounds with: size < 2, or size < 3 when periodic. */
 typedef struct tdm_info_t_ tdm_info_t;
 typedef struct {
   int (* const construct)(const int size, const int nrhs, const bool is_periodic,

@@ -149,6 +149,8 @@ def load(ndims: int):
     if ndims == 2:
         lib.vof_adjoint_gradient.argtypes = [vp, vp, vp, C.c_int, dp, C.c_int, dp, dp, dp]
         lib.vof_adjoint_loss.argtypes = [vp, vp, dp]
+    lib.linear_system_init.argtypes = [vp, C.POINTER(C.c_bool), C.POINTER(C.c_size_t), vp]
+    lib.linear_system_finalise.argtypes = [vp]
     lib.vof_statistics_output_to.argtypes = [vp, C.c_char_p]
     lib.vof_read_step_time.argtypes = [C.c_char_p, C.POINTER(C.c_size_t), dp]
     lib.vof_write_step_time.argtypes = [C.c_char_p, C.c_size_t, C.c_double]
@@ -467,6 +469,75 @@ class Batch:
         if self.h:
             self.lib.vof_batch_destroy(self.h)
             self.h = None
+
+
+class _TdmVT(C.Structure):
+    """tdm_t (include/tdm.h:8-45 of the reference): eight function pointers."""
+    _fields_ = [
+        ("construct", C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.c_bool, C.c_bool, C.POINTER(C.c_void_p))),
+        ("get_l", C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.POINTER(C.c_double)))),
+        ("get_c", C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.POINTER(C.c_double)))),
+        ("get_u", C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.POINTER(C.c_double)))),
+        ("get_size", C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))),
+        ("get_nrhs", C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))),
+        ("solve", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)),
+        ("destruct", C.CFUNCTYPE(C.c_int, C.c_void_p)),
+    ]
+
+
+def tdm_vtable(ndims: int = 2) -> _TdmVT:
+    """The exported ``tdm`` object of the library of that dimensionality."""
+    return _TdmVT.in_dll(load(ndims), "tdm")
+
+
+class Tdm:
+    """One tri-diagonal system, used as the reference uses its ``tdm``
+    object (src/tdm.c:147-313): construct, fill l / c / u through the host
+    arrays that get_l / get_c / get_u return, solve right-hand sides in
+    place, destruct.  ``solve`` takes a DEVICE pointer to ``nrhs`` contiguous
+    lines of ``size`` doubles (or complex values as (re, im) pairs)."""
+
+    def __init__(self, size: int, nrhs: int, is_periodic: bool, is_complex: bool, ndims: int = 2):
+        self.lib = load(ndims)
+        self.vt = tdm_vtable(ndims)
+        self.info = C.c_void_p()
+        rc = self.vt.construct(int(size), int(nrhs), bool(is_periodic), bool(is_complex), C.byref(self.info))
+        if rc != 0:
+            raise VofError("tdm.construct: " + self.lib.vof_last_error().decode())
+        self.size, self.nrhs = int(size), int(nrhs)
+        self.l, self.c, self.u = (self._host(getattr(self.vt, f)) for f in ("get_l", "get_c", "get_u"))
+
+    def _host(self, getter) -> np.ndarray:
+        p = C.POINTER(C.c_double)()
+        if getter(self.info, C.byref(p)) != 0:
+            raise VofError(self.lib.vof_last_error().decode())
+        return np.ctypeslib.as_array(p, shape=(self.size,))
+
+    def solve(self, data_ptr) -> None:
+        if self.vt.solve(self.info, data_ptr) != 0:
+            raise VofError("tdm.solve: " + self.lib.vof_last_error().decode())
+
+    def destruct(self) -> None:
+        if self.info:
+            self.vt.destruct(self.info)
+            self.info = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destruct()
+        except Exception:
+            pass
+
+
+def _linear_system_struct(ndims: int):
+    class LinearSystem(C.Structure):
+        """linear_system_t (include/linear_system.h of the reference)."""
+        _fields_ = [("is_initialised", C.c_bool), ("implicit", C.c_bool * ndims),
+                    ("x1pncl", C.c_void_p), ("y1pncl", C.c_void_p),
+                    ("x1pncl_mysizes", C.c_size_t * ndims), ("y1pncl_mysizes", C.c_size_t * ndims),
+                    ("tdm_x", C.c_void_p), ("tdm_y", C.c_void_p),
+                    ("transposer_x1_to_y1", C.c_void_p), ("transposer_y1_to_x1", C.c_void_p)]
+    return LinearSystem
 
 
 def _tune_report(buf) -> list:

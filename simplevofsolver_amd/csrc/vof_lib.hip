@@ -3391,78 +3391,202 @@ extern "C" int vof_copy_probe(domain_t * domain, size_t n, int reps, double * gb
 
 // ---------------------------------------------------------------------------
 // tri-diagonal solver vtable (tdm.c:110-324) over device right-hand sides
+//
+// The reference's gtsv (tdm.c:17-50) recomputes, for every right-hand side,
+// the same factors v[i] = u[i] / (c[i] - l[i] v[i-1]) and the reciprocals
+// 1 / (c[i] - l[i] v[i-1]); for a periodic system it solves the correction
+// vector q1 and the denominator of the Sherman-Morrison update once per call
+// (tdm.c:78-86,93).  Here k_tdm_table computes all of these once per call on
+// the device, with the reference's operations in the reference's order, and
+// k_tdm_lines runs only the per-line recurrences, so every line gets the
+// reference's values bit for bit.
 
 struct tdm_info_t_ {
   int size, nrhs;
   bool is_periodic, is_complex;
-  double * l, * c, * u;        // host
+  double * l, * c, * u;        // host, filled by the caller through get_l / get_c / get_u
   double * d_luc;              // device copy of l, c, u
-  double * d_v;                // per-rhs v scratch (size * nrhs)
-  double * d_q1;               // periodic correction vector (size)
+  double * d_tab;              // factor table (tdm_table_len(size) doubles)
+  hipStream_t stream;          // default priority, blocking w.r.t. the null stream
 };
 
 namespace vofk {
-// gtsv on one line of length n (stride st elements) for nrhs interleaved
-// components (1 real, 2 complex); v per line in vbuf
-__device__ inline void gtsv_line(int n, const double * l, const double * c, const double * u,
-                                 double * v, double * q, int ncomp) {
-  v[0] = u[0] / c[0];
-  for (int r = 0; r < ncomp; r++) q[r] = q[r] / c[0];
-  for (int i = 1; i < n - 1; i++) {
-    const double val = 1. / (c[i] - l[i] * v[i - 1]);
-    v[i] = val * (u[i]);
-    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] = val * (q[i * ncomp + r] - l[i] * q[(i - 1) * ncomp + r]);
+// factor table of one system of `n` rows:
+//   [0, n)        v[i]   (upper factors, rows 0 .. m-2 of the gtsv part)
+//   [n, 2n)       rec[i] (reciprocal pivots of rows 1 .. m-1; rec[0] = c[0], a divisor)
+//   [2n, 3n)      q1[i]  (periodic: the solved correction vector, rows 0 .. n-2)
+//   3n            1 if the last gtsv pivot is singular (|pivot| <= DBL_EPSILON, tdm.c:39-44)
+//   3n + 1        den of the Sherman-Morrison update (tdm.c:93)
+//   3n + 2        1 if |den| < DBL_EPSILON (tdm.c:94)
+// m = n for a non-periodic system, n - 1 for a periodic one
+__host__ __device__ constexpr size_t tdm_table_len(int n) { return 3 * (size_t)n + 3; }
+
+// forward and backward sweeps of gtsv (tdm.c:26-48) over one strided line with the
+// table's factors: q[k * st] is row k
+__device__ __forceinline__ void tdm_sweeps(int m, const double * __restrict__ tab, int n,
+                                           const double * __restrict__ l, double * q, size_t st) {
+  const double * v = tab;
+  const double * rec = tab + n;
+  double prev = q[0] / rec[0];
+  q[0] = prev;
+  for (int i = 1; i < m - 1; i++) {
+    prev = rec[i] * (q[i * st] - l[i] * prev);
+    q[i * st] = prev;
   }
-  const double val = c[n - 1] - l[n - 1] * v[n - 2];
-  if (fabs(val) > DBL_EPSILON) {
-    const double rr = 1. / val;
-    for (int r = 0; r < ncomp; r++)
-      q[(n - 1) * ncomp + r] = rr * (q[(n - 1) * ncomp + r] - l[n - 1] * q[(n - 2) * ncomp + r]);
+  if (tab[3 * n] != 0.) {
+    prev = 0.;
   } else {
-    for (int r = 0; r < ncomp; r++) q[(n - 1) * ncomp + r] = 0.;
+    prev = rec[m - 1] * (q[(m - 1) * st] - l[m - 1] * prev);
   }
-  for (int i = n - 2; i >= 0; i--)
-    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] -= v[i] * q[(i + 1) * ncomp + r];
+  q[(m - 1) * st] = prev;
+  for (int i = m - 2; i >= 0; i--) {
+    prev = q[i * st] - v[i] * prev;
+    q[i * st] = prev;
+  }
 }
 
-// tdm_solve (tdm.c:66-106): one lane per right-hand side; the periodic
-// correction vector q1 is solved by the first lane of every block into its
-// own copy (all copies are identical)
-__global__ void k_tdm_solve(int n, int nrhs, int periodic, int ncomp, const double * __restrict__ luc,
-                            double * __restrict__ vbuf, double * __restrict__ q1buf,
-                            double * __restrict__ data) {
-  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= nrhs) return;
-  const double * l = luc;
-  const double * c = luc + n;
-  const double * u = luc + 2 * n;
-  double * v = vbuf + (size_t)j * n;
-  double * q = data + (size_t)j * n * ncomp;
+// one thread: the factors (tdm.c:27,33-34,38), the periodic correction q1
+// (tdm.c:80-86) and the Sherman-Morrison denominator (tdm.c:93)
+__global__ void k_tdm_table(int n, int periodic, const double * __restrict__ l, const double * __restrict__ c,
+                            const double * __restrict__ u, double * __restrict__ tab) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int m = periodic ? n - 1 : n;
+  double * v = tab;
+  double * rec = tab + n;
+  v[0] = u[0] / c[0];
+  rec[0] = c[0];
+  for (int i = 1; i < m - 1; i++) {
+    const double val = 1. / (c[i] - l[i] * v[i - 1]);
+    rec[i] = val;
+    v[i] = val * (u[i]);
+  }
+  const double val = c[m - 1] - l[m - 1] * v[m - 2];
+  const bool singular = !(fabs(val) > DBL_EPSILON);
+  rec[m - 1] = singular ? 0. : 1. / val;
+  tab[3 * n] = singular ? 1. : 0.;
   if (!periodic) {
-    gtsv_line(n, l, c, u, v, q, ncomp);
+    tab[3 * n + 1] = 0.;
+    tab[3 * n + 2] = 0.;
     return;
   }
-  double * q1 = q1buf + (size_t)j * n;
+  double * q1 = tab + 2 * n;
   for (int i = 0; i < n - 1; i++) q1[i] = i == 0 ? -l[i] : i == n - 2 ? -u[i] : 0.;
-  gtsv_line(n - 1, l, c, u, v, q1, 1);
-  gtsv_line(n - 1, l, c, u, v, q, ncomp);
+  tdm_sweeps(m, tab, n, l, q1, 1);
   const double den = c[n - 1] + u[n - 1] * q1[0] + l[n - 1] * q1[n - 2];
-  for (int r = 0; r < ncomp; r++) {
-    const double num = q[(n - 1) * ncomp + r] - u[n - 1] * q[r] - l[n - 1] * q[(n - 2) * ncomp + r];
-    q[(n - 1) * ncomp + r] = fabs(den) < DBL_EPSILON ? 0. : num / den;
+  tab[3 * n + 1] = den;
+  tab[3 * n + 2] = fabs(den) < DBL_EPSILON ? 1. : 0.;
+}
+
+// the periodic update of one line (tdm.c:92-98)
+__device__ __forceinline__ void tdm_periodic_fix(int n, const double * __restrict__ tab,
+                                                 const double * __restrict__ l, const double * __restrict__ u,
+                                                 double * q, size_t st) {
+  const double * q1 = tab + 2 * n;
+  const double num = q[(n - 1) * st] - u[n - 1] * q[0] - l[n - 1] * q[(n - 2) * st];
+  const double last = tab[3 * n + 2] != 0. ? 0. : num / tab[3 * n + 1];
+  q[(n - 1) * st] = last;
+  for (int i = 0; i < n - 1; i++) q[i * st] = q[i * st] + last * q1[i];
+}
+
+// tdm_solve (tdm.c:66-106) over nrhs contiguous lines of n rows with ncomp
+// interleaved components (1 real, 2 = the real and imaginary parts of a
+// complex value, which the reference's real-by-complex arithmetic treats as
+// two independent systems with the same coefficients).
+//
+// lpb > 0: a work-group loads lpb consecutive lines (one contiguous run of
+// lpb * n * ncomp doubles, coalesced) into LDS with one system per LDS row of
+// odd stride (lanes of one wavefront hit distinct banks), one lane per
+// system runs the sweeps there, and the work-group stores the run back.
+// lpb == 0: lines too long for LDS; one lane per system in global memory.
+__global__ __launch_bounds__(256) void k_tdm_lines(int n, int nrhs, int periodic, int ncomp, int lpb,
+                                                   const double * __restrict__ l, const double * __restrict__ u,
+                                                   const double * __restrict__ tab, double * __restrict__ data) {
+  const int m = periodic ? n - 1 : n;
+  if (lpb == 0) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (long)nrhs * ncomp) return;
+    double * q = data + (s / ncomp) * (size_t)n * ncomp + (s % ncomp);
+    tdm_sweeps(m, tab, n, l, q, (size_t)ncomp);
+    if (periodic) tdm_periodic_fix(n, tab, l, u, q, (size_t)ncomp);
+    return;
   }
-  for (int i = 0; i < n - 1; i++)
-    for (int r = 0; r < ncomp; r++) q[i * ncomp + r] = q[i * ncomp + r] + q[(n - 1) * ncomp + r] * q1[i];
+  extern __shared__ double lds[];
+  const int stride = n | 1;
+  const long j0 = (long)blockIdx.x * lpb;
+  const int lines = (int)min((long)lpb, (long)nrhs - j0);
+  const int run = lines * n * ncomp;
+  double * g = data + j0 * (size_t)n * ncomp;
+  for (int e = threadIdx.x; e < run; e += blockDim.x) {
+    const int jl = e / (n * ncomp);
+    const int rem = e - jl * n * ncomp;
+    const int i = rem / ncomp;
+    const int r = rem - i * ncomp;
+    lds[(jl * ncomp + r) * stride + i] = g[e];
+  }
+  __syncthreads();
+  const int s = threadIdx.x;
+  if (s < lines * ncomp) {
+    double * q = lds + s * stride;
+    tdm_sweeps(m, tab, n, l, q, 1);
+    if (periodic) tdm_periodic_fix(n, tab, l, u, q, 1);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < run; e += blockDim.x) {
+    const int jl = e / (n * ncomp);
+    const int rem = e - jl * n * ncomp;
+    const int i = rem / ncomp;
+    const int r = rem - i * ncomp;
+    g[e] = lds[(jl * ncomp + r) * stride + i];
+  }
 }
 }  // namespace vofk
 
 // host-only: the vtable object must not be emitted by the device pass
 #if !defined(__HIP_DEVICE_COMPILE__)
+// lines per work-group of k_tdm_lines within 64 KiB of LDS and 256 systems
+// (0: the line does not fit, one lane per system in global memory)
+static int tdm_lines_per_group(int n, int ncomp) {
+  const size_t row = sizeof(double) * (size_t)(n | 1);
+  const size_t by_lds = (size_t)65536 / (row * ncomp);
+  const size_t by_lanes = 256 / ncomp;
+  return (int)(by_lds < by_lanes ? by_lds : by_lanes);
+}
+
+// the table and line kernels of one solve on `st`
+static int tdm_enqueue(int n, int nrhs, bool periodic, bool cplx, const double * l, const double * c,
+                       const double * u, double * tab, void * data, hipStream_t st) {
+  const int ncomp = cplx ? 2 : 1;
+  hipLaunchKernelGGL(vofk::k_tdm_table, dim3(1), dim3(64), 0, st, n, (int)periodic, l, c, u, tab);
+  LAUNCHCHK();
+  const int lpb = tdm_lines_per_group(n, ncomp);
+  if (lpb > 0) {
+    const long nb = ((long)nrhs + lpb - 1) / lpb;
+    const size_t shm = sizeof(double) * (size_t)(n | 1) * ncomp * lpb;
+    hipLaunchKernelGGL(vofk::k_tdm_lines, dim3((unsigned)nb), dim3(256), shm, st, n, nrhs, (int)periodic, ncomp,
+                       lpb, l, u, (const double *)tab, (double *)data);
+  } else {
+    const long nb = ((long)nrhs * ncomp + 255) / 256;
+    hipLaunchKernelGGL(vofk::k_tdm_lines, dim3((unsigned)nb), dim3(256), 0, st, n, nrhs, (int)periodic, ncomp, 0,
+                       l, u, (const double *)tab, (double *)data);
+  }
+  LAUNCHCHK();
+  return 0;
+}
+
+// argument checks shared by every solve: the reference's gtsv reads v[n-2]
+// (n >= 2), and its periodic path runs gtsv on n - 1 rows (n >= 3)
+static int tdm_check_sizes(const char * who, int size, int nrhs, bool is_periodic) {
+  if (size <= 0) return fail("ERROR(%s): size should be positive: %d", who, size);
+  if (nrhs <= 0) return fail("ERROR(%s): nrhs should be positive: %d", who, nrhs);
+  if (size < (is_periodic ? 3 : 2))
+    return fail("ERROR(%s): size %d is too small for a%s system", who, size, is_periodic ? " periodic" : "");
+  return 0;
+}
+
 static int tdm_construct(const int size, const int nrhs, const bool is_periodic, const bool is_complex,
                          tdm_info_t ** info) {
-  if (size <= 0) { *info = nullptr; return fail("ERROR(construct): size should be positive: %d", size); }
-  if (nrhs <= 0) { *info = nullptr; return fail("ERROR(construct): nrhs should be positive: %d", nrhs); }
-  if (size < 3) { *info = nullptr; return fail("ERROR(construct): size should be at least 3: %d", size); }
+  *info = nullptr;
+  if (tdm_check_sizes("construct", size, nrhs, is_periodic)) return 1;
   tdm_info_t * t = new tdm_info_t_();
   t->size = size;
   t->nrhs = nrhs;
@@ -3471,9 +3595,19 @@ static int tdm_construct(const int size, const int nrhs, const bool is_periodic,
   t->l = (double *)calloc(size, sizeof(double));
   t->c = (double *)calloc(size, sizeof(double));
   t->u = (double *)calloc(size, sizeof(double));
-  HIPCHK(hipMalloc(&t->d_luc, sizeof(double) * 3 * size));
-  HIPCHK(hipMalloc(&t->d_v, sizeof(double) * (size_t)size * nrhs));
-  HIPCHK(hipMalloc(&t->d_q1, sizeof(double) * (size_t)size * nrhs));
+  t->d_luc = nullptr;
+  t->d_tab = nullptr;
+  t->stream = nullptr;
+  hipError_t e = hipMalloc(&t->d_luc, sizeof(double) * 3 * size);
+  if (e == hipSuccess) e = hipMalloc(&t->d_tab, sizeof(double) * vofk::tdm_table_len(size));
+  if (e == hipSuccess) e = hipStreamCreate(&t->stream);
+  if (e != hipSuccess || !t->l || !t->c || !t->u) {
+    free(t->l); free(t->c); free(t->u);
+    dev_free(t->d_luc); dev_free(t->d_tab);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return fail("ERROR(construct): allocation failed: %s", hipGetErrorString(e));
+  }
   *info = t;
   return 0;
 }
@@ -3502,25 +3636,26 @@ static int tdm_get_nrhs(const tdm_info_t * info, int * nrhs) {
   *nrhs = info->nrhs;
   return 0;
 }
+// tdm.c:269-292: synchronous, as the reference's solve returns with the answers in data
 static int tdm_solve(const tdm_info_t * info, void * data) {
   if (!info) return fail("ERROR(solve): info is NULL");
+  if (!data) return fail("ERROR(solve): data is NULL");
   const int n = info->size;
-  std::vector<double> luc(3 * (size_t)n);
-  memcpy(&luc[0], info->l, sizeof(double) * n);
-  memcpy(&luc[n], info->c, sizeof(double) * n);
-  memcpy(&luc[2 * n], info->u, sizeof(double) * n);
-  HIPCHK(hipMemcpy(info->d_luc, luc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  const int nb = (info->nrhs + 63) / 64;
-  hipLaunchKernelGGL(vofk::k_tdm_solve, dim3(nb), dim3(64), 0, 0, n, info->nrhs, (int)info->is_periodic,
-                     info->is_complex ? 2 : 1, info->d_luc, info->d_v, info->d_q1, (double *)data);
-  LAUNCHCHK();
-  HIPCHK(hipDeviceSynchronize());
+  hipStream_t st = info->stream;
+  HIPCHK(hipMemcpyAsync(info->d_luc, info->l, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(info->d_luc + n, info->c, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(info->d_luc + 2 * n, info->u, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  if (tdm_enqueue(n, info->nrhs, info->is_periodic, info->is_complex, info->d_luc, info->d_luc + n,
+                  info->d_luc + 2 * n, info->d_tab, data, st))
+    return 1;
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 static int tdm_destruct(tdm_info_t * info) {
   if (!info) return fail("ERROR(destruct): info is NULL");
   free(info->l); free(info->c); free(info->u);
-  dev_free(info->d_luc); dev_free(info->d_v); dev_free(info->d_q1);
+  dev_free(info->d_luc); dev_free(info->d_tab);
+  if (info->stream) (void)hipStreamDestroy(info->stream);
   delete info;
   return 0;
 }

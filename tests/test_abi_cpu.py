@@ -113,3 +113,22 @@ def test_droplets_3d_small():
     assert abs(rms - 0.1) < 1e-12
     vol = ic.fields["vof"][:, :, 1:-1].sum() * dx ** 3
     assert abs(vol - sum(4 / 3 * np.pi * r ** 3 for *_, r in spheres)) < 1e-6
+
+
+@pytest.mark.parametrize("ndims", [2, 3])
+def test_tdm_construct_rejects_sizes_the_reference_cannot_solve(ndims):
+    """tdm.construct (src/tdm.c:147-183) rejects non-positive sizes and
+    counts; sizes the reference's loops would read out of bounds with (gtsv
+    reads v[n-2], the periodic path runs gtsv on n - 1 rows) are rejected
+    here too.  The checks run before any device call."""
+    vt = sva.tdm_vtable(ndims)
+    lib = sva.load(ndims)
+    info = C.c_void_p(123)
+    for size, nrhs, periodic, msg in ((0, 4, False, b"size should be positive"),
+                                      (8, 0, False, b"nrhs should be positive"),
+                                      (1, 4, False, b"too small"), (2, 4, True, b"too small")):
+        assert vt.construct(size, nrhs, periodic, False, C.byref(info)) != 0
+        assert not info.value
+        assert msg in lib.vof_last_error()
+    assert vt.solve(None, None) != 0 and b"info is NULL" in lib.vof_last_error()
+    assert vt.destruct(None) != 0
