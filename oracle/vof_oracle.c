@@ -566,11 +566,23 @@ int orc_tdm_solve_complex(int n, int nrhs, int periodic, const double * l,
  * twiddle table. */
 typedef struct { int n; double * c, * s; } twiddle_t;
 
+/* Twiddle table: 0 = octant-reduced, exactly symmetric (default); 1 =
+ * cos / sin of 2 pi k / n directly.  Both give the same DFT up to rounding
+ * (as two FFTW plans do); the tests use 1 to measure how a step amplifies
+ * rounding. */
+static int g_fft_variant = 0;
+void orc_set_fft_variant(int v) { g_fft_variant = v ? 1 : 0; }
+
 static void twiddle_make(twiddle_t * tw, int n) {
   tw->n = n;
   tw->c = malloc(sizeof(double) * n);
   tw->s = malloc(sizeof(double) * n);
   for (int k = 0; k < n; k++) {
+    if (g_fft_variant) {
+      tw->c[k] = cos(2. * g_pi * (double)k / n);
+      tw->s[k] = sin(2. * g_pi * (double)k / n);
+      continue;
+    }
     /* reduce to the first octant so that symmetric angles give
      * bit-identical magnitudes */
     const long m8 = 8L * k;  /* angle = 2 pi k / n = (pi/4) (m8 / n) */

@@ -88,6 +88,10 @@ void orc_momentum(const orc_t * o, double * out /* NDIMS */);
 double orc_interface_area(const orc_t * o);
 void orc_nusselt(const orc_t * o, double * out /* 4 */);
 
+/* twiddle table of the transforms (0 octant-reduced, the default; 1 direct
+ * cos / sin): the same DFT up to rounding */
+void orc_set_fft_variant(int v);
+
 /* statistics accumulators (statistics.c) */
 typedef struct orc_stats orc_stats_t;
 orc_stats_t * orc_stats_create(const orc_t * o);

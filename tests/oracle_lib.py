@@ -67,6 +67,7 @@ def load(ndims: int):
     lib.orc_tdm_solve.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     lib.orc_tdm_solve_complex.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     lib.orc_fft.argtypes = [C.c_int, C.c_int, _dp]
+    lib.orc_set_fft_variant.argtypes = [C.c_int]
     lib.orc_rfft.argtypes = [C.c_int, _dp, _dp]
     lib.orc_irfft.argtypes = [C.c_int, _dp, _dp]
     lib.orc_redft10.argtypes = [C.c_int, _dp, _dp]
@@ -272,6 +273,12 @@ def tdm_solve(l, c, u, q, periodic=False, lib_ndims=2):
     nrhs = qq.size // n
     assert lib.orc_tdm_solve(n, nrhs, int(periodic), _ptr(l), _ptr(c), _ptr(u), _ptr(qq)) == 0
     return qq
+
+
+def set_fft_variant(v, ndims):
+    """Twiddle table of the oracle's transforms (0 octant-reduced, the
+    default; 1 direct cos / sin): the same DFT up to rounding."""
+    load(ndims).orc_set_fft_variant(int(v))
 
 
 def fft(x, sign=-1):

@@ -859,3 +859,46 @@ def test_static_droplet_laplace_pressure_jump(uniformx, R):
     assert abs(jump / laplace - 1.0) < 0.03, (jump, laplace)
     # the pressure is uniform inside and outside, away from the interface
     assert p[r < R - 0.1].std() < 0.01 * laplace and p[r > R + 0.1].std() < 0.01 * laplace
+
+
+# --- sensitivity of whole steps on 3D droplet workloads ------------------------------
+
+def test_step_amplifies_transform_rounding_on_droplets():
+    """Two oracles that differ only in the rounding of their transforms'
+    twiddle tables (octant-reduced vs direct cos / sin; the same DFT, as two
+    FFTW plans would be) on the 3D droplet case 64x32x32 (We = 50, velocity
+    rms 0.2).  On identical input one Poisson solve agrees to rounding
+    (psi within 1e-14 modulo its mean; measured 7.6e-16), yet after one full
+    step the velocities differ by 1e-4 relative and the curvature by 0.2:
+    where vof varies only at rounding level the corner normals are
+    normalised rounding noise (gradient / max(|gradient|, DBL_EPSILON),
+    curvature_tensor.c:14-48), and a 1e-16 change turns them around.  So a
+    whole-step comparison of two correct implementations of this step on
+    droplet workloads is bounded by this amplification, not by 1e-10."""
+    from parity_util import demean, interior, rel
+    from simplevofsolver_amd.ic import droplets_3d
+    ic, _ = droplets_3d(64, 32, 32, lx=2.0, ndrops=2, rmin=0.15, rmax=0.25, urms=0.2, seed=5, Re=1000.0,
+                        We=50.0, uniformx=True)
+    gl = ic.glsizes
+    dxf = np.diff(ic.xf)
+
+    def run(variant, fn):
+        OL.set_fft_variant(variant, 3)
+        try:
+            fn()
+        finally:
+            OL.set_fft_variant(0, 3)
+
+    a, b = Oracle(ic), Oracle(ic)
+    dt = a.decide_dt()
+    run(0, lambda: a.potential(0, dt))
+    run(1, lambda: b.potential(0, dt))
+    e_psi = rel(interior("psi", demean("psi", b.get("psi"), gl, dxf), gl),
+                interior("psi", demean("psi", a.get("psi"), gl, dxf), gl))
+    assert 0.0 < e_psi < 1e-14, e_psi
+    a, b = Oracle(ic), Oracle(ic)
+    run(0, a.integrate)
+    run(1, b.integrate)
+    e_u = max(rel(b.get(n), a.get(n)) for n in ("ux", "uy", "uz"))
+    assert e_u > 1e-8, e_u
+    assert rel(b.get("curv"), a.get("curv")) > 1e-3
