@@ -31,12 +31,18 @@ METRIC = "Mcell-steps/s at 3D 256³ FP64 plain vs batched/mixed-prec; adjoint/fo
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
+def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int, iface_cells: int | None = None) -> float | None:
     """Compulsory FP64 traffic of one launch: every array element the kernel
-    must read once plus every element it must write once (3D layouts)."""
+    must read once plus every element it must write once (3D layouts).
+
+    Cell normals (nx, ny, nz, d: 32 B) exist only at interface cells
+    (1e-8 <= vof <= 1 - 1e-8; curvature_tensor.c:94-145 skips the others), so
+    the interface and VOF-update kernels charge them per interface cell:
+    `iface_cells` of them (every cell when it is None, an upper bound)."""
     nh = ny // 2 + 1
     c = nx * ny * nz
     s = nx * nh * nz
+    nrm = 32 * (c if iface_cells is None else iface_cells)
     table = {
         "k_gradient": (nx + 1) * (ny + 3) * (nz + 3) * (8 + 24),
         "k_normal_curv": nx * (ny + 2) * (nz + 2) * (24 + 8 + 8),
@@ -59,18 +65,23 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int) -> float | None:
         "k_correct": c * (8 + 24 + 24),
         "k_pressure": c * (8 + 8 + 8),
         "k_dt_partial": c * 24,
-        # fused schedule: vof read; normal (3) and curvature written
-        "k_iface_fused": c * (8 + 24 + 8),
-        "k_iface_march": c * (8 + 24 + 8),
-        # velocity (3), vof, normal (3) read, the new vof written; srcb read in
-        # RK stages 1 and 2, srca written in stages 0 and 1 (the figures are
-        # means over the three stages of a step, whose launches alternate)
-        "k_vof_fused": c * (24 + 8 + 24 + 8) + c * (8 + 8) * 2 / 3,
+        # fused schedule: vof read and curvature written per cell, the normal
+        # written at interface cells
+        "k_iface_fused": c * (8 + 8) + nrm,
+        "k_iface_march": c * (8 + 8) + nrm,
+        # velocity (3) and vof read per cell, the normal of interface cells,
+        # the new vof written; srcb read in RK stages 1 and 2, srca written in
+        # stages 0 and 1 (the figures are means over the three stages of a
+        # step, whose launches alternate)
+        "k_vof_fused": c * (24 + 8 + 8) + nrm + c * (8 + 8) * 2 / 3,
         # ux uy uz p t, vof, curvature read and the increment of 4 fields
         # written; srcb of 4 fields read in stages 1 and 2, srca of 4 fields
         # written in stages 0 and 1
         "k_rhs_fused": c * (40 + 8 + 8 + 32) + c * (32 + 32) * 2 / 3,
         "k_rhs_march": c * (40 + 8 + 8 + 32) + c * (32 + 32) * 2 / 3,
+        # the same reads and source traffic; the new ux uy uz t written
+        # instead of their increments (the x solve runs in LDS)
+        "k_pred_lines": c * (40 + 8 + 8 + 32) + c * (32 + 32) * 2 / 3,
         # psi, ux uy uz p read and written back
         "k_corr_press": c * (8 + 32 + 32),
     }
@@ -484,13 +495,18 @@ def main(argv=None) -> int:
     rep = g.profile_report()
     g.profile(False)
     d1 = g.diagnostics()
+    # interface cells of the last state (their count moves by well under 1 %
+    # over these steps): the cells whose normals the interface kernels write
+    vint = _interior("vof", g.get("vof"), n, n, n)
+    n_iface = int(((vint >= 1e-8) & (vint <= 1.0 - 1e-8)).sum())
+    del vint
     cells = n * n * n
     value = cells * args.steps / el / 1e6
     kern_total = sum(ms for _, ms in rep.values())
     dom = max(rep.items(), key=lambda kv: kv[1][1])
     kname, (kcnt, kms) = dom
     kavg_s = kms / kcnt * 1e-3
-    kbytes = algorithmic_bytes(kname, n, n, n)
+    kbytes = algorithmic_bytes(kname, n, n, n, n_iface)
     achieved = kbytes / kavg_s / 1e9 if kbytes else None
     traffic = read_traffic(kname, f"3d{n}")
     copy_gbs = g.copy_probe(1 << 27, 10)
@@ -510,7 +526,7 @@ def main(argv=None) -> int:
         print(f"[bench] kernels (profiled pass {el_prof * 1e3:.2f} ms, kernel sum {kern_total:.2f} ms; "
               f"timed pass {el * 1e3:.2f} ms):", file=sys.stderr)
         for k, (cnt, ms) in sorted(rep.items(), key=lambda kv: -kv[1][1]):
-            b = algorithmic_bytes(k, n, n, n)
+            b = algorithmic_bytes(k, n, n, n, n_iface)
             bw = f"{b / (ms / cnt * 1e-3) / 1e9:8.1f} GB/s" if b else ""
             print(f"  {k:18s} {cnt:5d} x {ms / cnt * 1e3:9.1f} us  total {ms:9.2f} ms {bw}",
                   file=sys.stderr)
@@ -558,6 +574,7 @@ def main(argv=None) -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": kbytes,
+            "interface_cells": n_iface,
             "mean_launch_us": round(kavg_s * 1e6, 2),
             "profiled_ms_per_step": round(el_prof / args.steps * 1e3, 4),
             "copy_probe_gbs": round(copy_gbs, 1),

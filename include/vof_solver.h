@@ -240,6 +240,13 @@ int vof_array_to_host(const domain_t * domain, const array_t * array, void * hos
 int vof_array_from_host(const domain_t * domain, array_t * array, const void * host);
 /* wait for all queued work of this domain */
 int vof_synchronize(const domain_t * domain);
+/* device buffers for callers of tdm.solve and linear_system_init (whose
+ * data are device pointers here) that hold no HIP runtime of their own.
+ * vof_device_copy is synchronous; kind 1 = host to device, 2 = device to
+ * host, 3 = device to device. */
+int vof_device_alloc(void ** ptr, size_t bytes);
+int vof_device_free(void * ptr);
+int vof_device_copy(void * dst, const void * src, size_t bytes, int kind);
 /* free device memory (the reference never frees) */
 int interface_finalise(const domain_t * domain, interface_t * interface);
 int fluid_finalise(const domain_t * domain, fluid_t * fluid);
@@ -254,10 +261,14 @@ int domain_finalise(domain_t * domain);
  *     reads them) and skips the beta-weighted source reads of the first
  *     stage (their weight is 0). */
 int vof_set_schedule(domain_t * domain, int schedule);
-/* stages fused by schedule 1 (bit mask, default 63 = all): 1 curvature
+/* stages fused by schedule 1 (bit mask, default 127 = all): 1 curvature
  * tensor, 2 VOF update, 4 sources with the surface force, 8 boundary images
  * of the predictor, 16 boundary images of the potential, 32 correction with
- * pressure */
+ * pressure, 64 the whole predictor in one kernel (sources, x solve and
+ * update with images; needs 2, 4 and 8, and x lines that fit the kernel's
+ * LDS: nx <= 2048).  With 64 the predictor writes the new ux, uy, uz, t into
+ * second buffers and swaps them in: like the src arrays, the data pointers
+ * of these fields change between calls. */
 int vof_set_fusion(domain_t * domain, int mask);
 /* device diagnostics (logging/divergence.c:37-55, logging/interface.c:29-51,
  * logging/energy.c:43-65, logging/momentum.c:39-53): out = {max|div u|,
@@ -337,7 +348,7 @@ int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, dou
 #endif
 
 /* autotuner: picks "fft_radix", "fft_elems", "iface_planes" (3D),
- * "rhs_rows", "iface_overlap", "thomas_rows" and "lines_lpb" of
+ * "pred_rows", "rhs_rows", "iface_overlap", "thomas_rows" and "lines_lpb" of
  * vof_set_tuning by timing `steps` steps of
  * this case per candidate (coordinate descent in that order; a candidate
  * must be 1 % faster to replace the current value).  The case's device state
@@ -356,6 +367,8 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * per work-group of the predictor's x solver (0 = automatic); "xcd_bands" =
  * 1 walks the 3D stencil tiles band by band per XCD; "rhs_rows" = 1, 2, 4,
  * 8 (default) or 16 rows of one column per thread of the source kernel;
+ * "pred_rows" = 1, 2, 4 or 8 (default) rows per thread of the one-kernel
+ * predictor (fewer where its x lines would not fit 64 KiB of LDS);
  * "iface_planes" = 0 (blocks), 4, 8 (default) or 16 z planes marched per
  * work-group by the 3D interface kernel;
  * "iface_overlap" = 1 runs the interface kernel of RK stages 2 and 3 on a

@@ -79,7 +79,7 @@ ABI_SYMBOLS_COMMON = [
     "statistics", "vof_statistics_output_to", "vof_set_poisson_mixed",
     "vof_batch_create", "vof_batch_integrate", "vof_batch_synchronize", "vof_batch_destroy",
     "vof_autotune", "vof_batch_autotune", "vof_batch_set_graphs", "vof_batch_graph_counts",
-    "vof_integrate_fixed_dt",
+    "vof_integrate_fixed_dt", "vof_device_alloc", "vof_device_free", "vof_device_copy",
 ]
 ABI_SYMBOLS_2D = ["vof_adjoint_gradient", "vof_adjoint_loss"]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
@@ -563,3 +563,5 @@ def solver_from_ic(ic_obj, dirname: str, schedule: int | None = None, params: di
     if params:
         prm.update(params)
     return Solver(dirname, ic_obj.ndims, prm, schedule)
+
+from . import hip  # noqa: E402,F401  (device buffers through the library's runtime)

@@ -3110,20 +3110,15 @@ __global__ __launch_bounds__(256) void k_rhs_fused(Grid g, FusedRhs f) {
 // so each DOF loads one new row per column (35 loads instead of ~70) and
 // the next row's loads are independent of the current row's arithmetic.
 // Values and arithmetic are those of k_rhs_fused (the same rhs_*_v).
-template <int R>
-__device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, FusedRhs f) {
+//
+// rhs_rows marches rows jb .. jb+nr-1 of column (i, k): it stores the
+// explicit sources (keep_a) and hands each row's increments
+// ca sa + cb sb + cg sg of ux, uy, uz, t to sink(r, d0, d1, d2, d3) (d0 is
+// meaningful only for i >= 2, the ux DOFs; d2 only in 3D).
+template <int R, class Sink>
+__device__ __forceinline__ void rhs_rows(const Grid & g, const FusedRhs & f, int i, int jb, int k, int nr,
+                                         Sink && sink) {
   const int nx = g.nx, ny = g.ny;
-#if NDIMS == 3
-  const Tile3 tl = g.bands ? band_tile(vb) : plain_tile(vb);
-  const int k = (int)tl.z + 1;
-#else
-  const Tile3 tl = plain_tile(vb);
-  const int k = 1;
-#endif
-  const int i = (int)(tl.x * blockDim.x + threadIdx.x) + 1;
-  const int jb = (int)(tl.y * blockDim.y + threadIdx.y) * R + 1;
-  if (i > nx || jb > ny) return;
-  const int nr = min(R, ny + 1 - jb);
   const ptrdiff_t sx = nx + 1, sc = nx + 2;   // row strides of ux and cell arrays
 #if NDIMS == 3
   const ptrdiff_t pc = sc * (ny + 2), pv = sc * (ny + 4);
@@ -3215,8 +3210,8 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
       a.dif = f.dif[1];
       rhs_uy_v(g, a, i, s, f.tension * grad * kappa, sa1, sg1);
     }
+    double sa2 = 0., sg2 = 0.;
 #if NDIMS == 3
-    double sa2, sg2;
     {
       UzSt s;
       s.c = c0c; s.xm = cxm; s.xp = cxp; s.ym = c0m; s.yp = c0p; s.zm = czm; s.zp = czpc;
@@ -3245,18 +3240,16 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
     }
     const double ca = b.ca, cb = b.cb, cg = b.cg;
     const bool ka = f.keep_a;
-    if (has_ux) {
-      if (ka) f.srca[0][ix] = sa0;
-      f.delta[0][ix] = +ca * sa0 + cb * sb0 + cg * sg0;
-    }
-    if (ka) f.srca[1][is] = sa1;
-    f.delta[1][is] = +ca * sa1 + cb * sb1 + cg * sg1;
+    if (ka) {
+      if (has_ux) f.srca[0][ix] = sa0;
+      f.srca[1][is] = sa1;
 #if NDIMS == 3
-    if (ka) f.srca[2][is] = sa2;
-    f.delta[2][is] = +ca * sa2 + cb * sb2 + cg * sg2;
+      f.srca[2][is] = sa2;
 #endif
-    if (ka) f.srca[3][is] = sa3;
-    f.delta[3][is] = +ca * sa3 + cb * sb3 + cg * sg3;
+      f.srca[3][is] = sa3;
+    }
+    sink(r, is, ix, +ca * sa0 + cb * sb0 + cg * sg0, +ca * sa1 + cb * sb1 + cg * sg1,
+         +ca * sa2 + cb * sb2 + cg * sg2, +ca * sa3 + cb * sb3 + cg * sg3);
     // shift the carried rows and advance one row
     a0m = a0c; a0c = a0p; axpm = axpc;
     b0m = b0c; b0c = b0p; bxmc = bxmp;
@@ -3274,8 +3267,186 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
 }
 
 template <int R>
+__device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, FusedRhs f) {
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 3
+  const Tile3 tl = g.bands ? band_tile(vb) : plain_tile(vb);
+  const int k = (int)tl.z + 1;
+#else
+  const Tile3 tl = plain_tile(vb);
+  const int k = 1;
+#endif
+  const int i = (int)(tl.x * blockDim.x + threadIdx.x) + 1;
+  const int jb = (int)(tl.y * blockDim.y + threadIdx.y) * R + 1;
+  if (i > nx || jb > ny) return;
+  const int nr = min(R, ny + 1 - jb);
+  const bool has_ux = i >= 2;
+  rhs_rows<R>(g, f, i, jb, k, nr, [&](int, size_t is, size_t ix, double d0, double d1, double d2, double d3) {
+    if (has_ux) f.delta[0][ix] = d0;
+    f.delta[1][is] = d1;
+#if NDIMS == 3
+    f.delta[2][is] = d2;
+#else
+    (void)d2;
+#endif
+    f.delta[3][is] = d3;
+  });
+}
+
+template <int R>
 __global__ __launch_bounds__(256) void k_rhs_march(Grid g, FusedRhs f) {
   k_rhs_march_body<R>(hw_block(), g, f);
+}
+
+// ---------------------------------------------------------------------------
+// The whole predictor of one RK stage in one kernel (predict/main.c:62-79;
+// ux.c:306-388 and its uy, uz, t counterparts): a work-group owns complete x
+// lines -- R consecutive rows j of one plane k, every x -- and
+//   1. marches the sources of ux, uy, uz, t over its rows (rhs_rows, the same
+//      arithmetic as k_rhs_march) and keeps the increments in LDS,
+//   2. solves (I - prefactor Lx) x = increment along each line there (one
+//      wavefront per field, one lane per row: the recurrences of gtsv with
+//      the shared factors, as k_lines_lds),
+//   3. writes old field + solution into a second buffer of each field (with
+//      the x walls and the periodic halo images), because the stencils of the
+//      neighbouring rows, run by other work-groups, still read the old field.
+// The increments never travel through HBM.  Values are those of
+// k_rhs_march followed by k_lines_lds, bit for bit.
+constexpr int kPNF = NDIMS + 1;   // fields solved in x: ux, uy, (uz,) t
+
+struct PredLines {
+  FusedRhs rhs;              // sources; old fields in rhs.base
+  const double * tab[4];     // Thomas table per field kind (0 ux, 1 uy, 2 uz, 3 t)
+  double * out[4];           // new buffer per field kind
+  int xw;                    // lanes along x per row group: 64, 128 or 256
+};
+
+// LDS slot of a field kind: ux 0, uy 1, uz 2 (3D), t NDIMS
+__device__ __forceinline__ int pred_kind(int slot) { return slot == NDIMS ? 3 : slot; }
+// element `kind` of a four-entry argument array, selected without indexing
+// (an indexed kernel argument is copied to scratch memory)
+template <class T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int kind) {
+  return kind == 0 ? a[0] : kind == 1 ? a[1] : kind == 2 ? a[2] : a[3];
+}
+
+template <int RT>
+__device__ __forceinline__ void k_pred_lines_body(const VBlock & vb, Grid g, const PredLines & P) {
+  extern __shared__ double S[];
+  const int nx = g.nx, ny = g.ny;
+#if NDIMS == 3
+  const Tile3 tl = g.bands ? band_tile(vb) : plain_tile(vb);
+  const int k = (int)tl.z + 1;
+  const int nz = g.nz;
+#else
+  const Tile3 tl = plain_tile(vb);
+  const int k = 1;
+  const int nz = 1;
+#endif
+  const int XW = P.xw;
+  const int G = 256 / XW;
+  const int R = G * RT;
+  const int ld = nx;                    // row stride of the LDS lines
+  const size_t fs = (size_t)R * ld;     // LDS stride of one field's rows
+  const int j0 = (int)tl.y * R + 1;     // first row of this work-group
+  if (j0 > ny) return;                  // uniform over the work-group
+  const int nrows = min(R, ny + 1 - j0);
+  const FusedRhs & f = P.rhs;
+  // 1. increments of the rows into LDS
+  {
+    const int tx = (int)threadIdx.x % XW, ty = (int)threadIdx.x / XW;
+    const int jb = j0 + ty * RT;
+    if (jb <= ny) {
+      const int nr = min(RT, ny + 1 - jb);
+      double * srow = S + (size_t)(ty * RT) * ld;
+      for (int c = 0; c < nx; c += XW) {
+        const int i = c + tx + 1;
+        if (i > nx) break;
+        rhs_rows<RT>(g, f, i, jb, k, nr, [&](int r, size_t, size_t, double d0, double d1, double d2, double d3) {
+          double * row = srow + (size_t)r * ld;
+          if (i >= 2) row[i - 2] = d0;
+          row[fs + i - 1] = d1;
+#if NDIMS == 3
+          row[2 * fs + i - 1] = d2;
+          row[3 * fs + i - 1] = d3;
+#else
+          (void)d2;
+          row[2 * fs + i - 1] = d3;
+#endif
+          // keep the next row's loads behind these stores (as the global
+          // stores of k_rhs_march do): hoisting them spills registers
+          __asm__ __volatile__("" ::: "memory");
+        });
+      }
+    }
+  }
+  __syncthreads();
+  // 2. one wavefront per field, one lane per row
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
+  if (w < kPNF && lane < nrows) {
+    const int kind = pred_kind(w);
+    LineField F{};
+    F.tab = pick4(P.tab, kind);
+    F.n = kind == 0 ? nx - 1 : nx;
+    lines_sweep(F, S + (size_t)w * fs + (size_t)lane * ld);
+  }
+  __syncthreads();
+  // 3. new field rows: wavefront w takes lines w, w + 4, ... (field-major);
+  // a line's field, row and halo decision are wave-uniform, lanes walk x
+  const int n1 = ny + 2;
+  for (int l = w; l < kPNF * nrows; l += 4) {
+    const int slot = l / nrows, r = l - slot * nrows;
+    const int kind = pred_kind(slot);
+    const int j = j0 + r;
+    const int n = kind == 0 ? nx - 1 : nx;
+    const int n0 = kind == 0 ? nx + 1 : nx + 2;
+    const double * old = kind == 0 ? f.base.ux : kind == 1 ? f.base.uy : kind == 2 ? f.base.uz : f.base.t;
+    double * const out = pick4(P.out, kind);
+#if NDIMS == 3
+    const size_t roff = (size_t)n0 * ((size_t)j + (size_t)n1 * (size_t)k);
+#else
+    const size_t roff = (size_t)n0 * (size_t)j;
+#endif
+    const double * __restrict__ orow = old + roff;
+    double * __restrict__ nrow = out + roff;
+    const double * srow = S + (size_t)slot * fs + (size_t)r * ld;
+    // rows next to the periodic halo also write their images (walls
+    // included): the reference's halo copy of the updated field
+    const bool edge = j < 2 || j >= ny
+#if NDIMS == 3
+                      || k < 2 || k >= nz
+#endif
+        ;
+    for (int m0 = 0; m0 < n; m0 += 64 * kWUnroll) {
+      double v[kWUnroll];
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++) {
+        const int m = m0 + 64 * u + lane;
+        if (m < n) v[u] = orow[1 + m] + srow[m];
+      }
+#pragma unroll
+      for (int u = 0; u < kWUnroll; u++) {
+        const int m = m0 + 64 * u + lane;
+        if (m < n) {
+          nrow[1 + m] = v[u];
+          if (edge) store_images(out, 1 + m, n0, n1, j, PLANE1(k), ny, nz, 1, v[u]);
+        }
+      }
+    }
+    // the x walls (constants the predictor never changes)
+    if (lane < 2) {
+      const int xw = lane == 0 ? 0 : n + 1;
+      const double wv = orow[xw];
+      nrow[xw] = wv;
+      if (edge) store_images(out, xw, n0, n1, j, PLANE1(k), ny, nz, 1, wv);
+    }
+  }
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void k_pred_lines(Grid g, PredLines P) {
+  k_pred_lines_body<RT>(hw_block(), g, P);
 }
 
 __device__ inline double prefactor_div(double a, double b) { return a / b; }

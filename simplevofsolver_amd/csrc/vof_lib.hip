@@ -238,6 +238,8 @@ struct vof_ctx {
   bool iface_ready;                      // the first stage's interface kernel is already on the main stream
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
+  double * d_fspare[4];                  // second buffer of ux, uy, uz, t (one-kernel predictor)
+  int pred_rows;                         // rows marched per thread by the one-kernel predictor (1, 2, 4, 8)
   Grid grid;
   double * d_dxf;
   double * d_dxc;
@@ -830,6 +832,33 @@ extern "C" int vof_synchronize(const domain_t * domain) {
   return 0;
 }
 
+// device memory for callers of the tdm vtable and linear_system that hold no
+// HIP runtime of their own: the buffers come from this library's runtime
+extern "C" int vof_device_alloc(void ** ptr, size_t bytes) {
+  if (!ptr) return fail("vof_device_alloc: ptr is NULL");
+  *ptr = nullptr;
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  return 0;
+}
+
+extern "C" int vof_device_free(void * ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return 0;
+}
+
+extern "C" int vof_device_copy(void * dst, const void * src, size_t bytes, int kind) {
+  if (!dst || !src) return fail("vof_device_copy: null pointer");
+  hipMemcpyKind k;
+  switch (kind) {
+    case 1: k = hipMemcpyHostToDevice; break;
+    case 2: k = hipMemcpyDeviceToHost; break;
+    case 3: k = hipMemcpyDeviceToDevice; break;
+    default: return fail("vof_device_copy: kind %d is not 1 (to device), 2 (to host) or 3 (device to device)", kind);
+  }
+  HIPCHK(hipMemcpy(dst, src, bytes, k));
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // FFT plans and twiddles
 
@@ -909,7 +938,7 @@ static int ctx_create(domain_t * domain) {
 #endif
   ctx->dy = domain->dy;
   ctx->schedule = 1;
-  ctx->fuse = 63;
+  ctx->fuse = 127;
   ctx->fft_elems = 1024;
   ctx->fft_radix = 16;
   ctx->thomas_rows = 8;
@@ -923,6 +952,8 @@ static int ctx_create(domain_t * domain) {
   ctx->iface_planes = 8;
   ctx->d_vof_spare = nullptr;
   ctx->vof_bytes = 0;
+  for (int m = 0; m < 4; m++) ctx->d_fspare[m] = nullptr;
+  ctx->pred_rows = 8;
   ctx->prof_on = false;
   ctx->ev_next = 0;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
@@ -1147,6 +1178,7 @@ extern "C" int domain_finalise(domain_t * domain) {
     dev_free(ctx->d_spec32); dev_free(ctx->d_twy32); dev_free(ctx->d_twz32);
     dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
     if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
+    for (int m = 0; m < 4; m++) dev_free(ctx->d_fspare[m]);
     for (int m = 0; m < 9; m++) dev_free(ctx->d_stat[m]);
     host_free(ctx->h_scalar);
     prof_flush(ctx);
@@ -1405,6 +1437,8 @@ static void build_tab(double * tab, int n, double prefactor, const double * lapx
 // enqueue: tables for `field` (0 ux, 1 uy, 2 uz, 3 t) at `rkstep`, copied
 // from a pinned slot once the previous copy from that slot has completed
 static int upload_tab(vof_ctx * ctx, int field, int rkstep, double prefactor, const double ** dtab) {
+  if (!ctx->h_tab || !ctx->d_tab)
+    return fail("this case has no Thomas tables (their allocation failed when its batch was destroyed)");
   const int slot = field * 3 + rkstep;
   double * h = ctx->h_tab + ctx->tab_rk * rkstep + ctx->tab_stride * field;
   double * d = ctx->d_tab + ctx->tab_rk * rkstep + ctx->tab_stride * field;
@@ -1455,9 +1489,39 @@ static int side_iface(vof_ctx * ctx) {
   return 0;
 }
 
+// the second buffers of the one-kernel predictor, allocated on first use
+// with the fields' contents (the kernel rewrites every entry of the buffer it
+// writes, walls and halo images included)
+static int ensure_field_spares(vof_ctx * ctx, fluid_t * fluid) {
+  array_t * arr[4] = {&fluid->ux, &fluid->uy, nullptr, &fluid->t};
+#if NDIMS == 3
+  arr[2] = &fluid->uz;
+#endif
+  for (int m = 0; m < 4; m++) {
+    if (!arr[m] || ctx->d_fspare[m]) continue;
+    HIPCHK(hipMalloc(&ctx->d_fspare[m], arr[m]->datasize));
+    HIPCHK(hipMemcpyAsync(ctx->d_fspare[m], arr[m]->data, arr[m]->datasize, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return 0;
+}
+
+// rows per thread of the one-kernel predictor at this grid (0: the x lines
+// of a work-group do not fit its 64 KiB of LDS; the two-kernel path runs)
+static int pred_rows_for(const vof_ctx * ctx, int * xw) {
+  const int nx = ctx->nx;
+  int w = ((min(nx, 256) + 63) / 64) * 64;   // lanes along x: 64, 128 or 256
+  if (w == 192) w = 256;
+  const int groups = 256 / w;                // row groups of a work-group
+  *xw = w;
+  const size_t line = sizeof(double) * (size_t)nx * (NDIMS + 1);
+  int rt = ctx->pred_rows;
+  while (rt > 0 && line * (size_t)(groups * rt) > 65536) rt /= 2;
+  return rt;
+}
+
 static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid_t * fluid,
                         const interface_t * interface, int fused, const double * vof_old = nullptr,
-                        int images = 0) {
+                        int images = 0, bool one_kernel = false) {
   CTX_OR_FAIL(domain);
   if (rkstep > 2) return fail("rkstep %zu out of range", rkstep);
   // reset_srcs (predict/main.c:9-29): the a <-> b swap; a (and g) are fully
@@ -1533,6 +1597,55 @@ static int predict_impl(const domain_t * domain, size_t rkstep, double dt, fluid
       fr.srca[kind] = (double *)jobs[f].src[0].data;
       fr.srcb[kind] = (const double *)jobs[f].src[1].data;
       fr.delta[kind] = ctx->d_delta + (size_t)f * ncell;
+    }
+    int xw = 0;
+    const int rt = (one_kernel && images) ? pred_rows_for(ctx, &xw) : 0;
+    if (rt > 0) {
+      // sources, x solve and update in one kernel; the new fields go to the
+      // second buffers, which then become the fields.  The buffers are
+      // allocated before any batch records launches (vof_batch_create)
+      if (ctx->rec && (!ctx->d_fspare[0] || !ctx->d_fspare[3]))
+        return fail("one-kernel predictor: second field buffers missing in a batch");
+      if (ensure_field_spares(ctx, fluid)) return 1;
+      vofk::PredLines P{};
+      P.rhs = fr;
+      P.xw = xw;
+      for (int f = 0; f < nj; f++) {
+        const int kind = jobs[f].kind;
+        const double * dtab = nullptr;
+        if (upload_tab(ctx, kind, (int)rkstep, kind == 3 ? pt : pm, &dtab)) return 1;
+        P.tab[kind] = dtab;
+        P.out[kind] = ctx->d_fspare[kind];
+      }
+      const int R = (256 / xw) * rt;
+      const size_t shm = sizeof(double) * (size_t)(NDIMS + 1) * (size_t)R * (size_t)nx;
+      const dim3 gp(1u, (unsigned)((ny + R - 1) / R), (unsigned)nz);
+      switch (rt) {
+        case 1:
+          KLB(ctx, "k_pred_lines", 256, vofk::k_pred_lines_body<1>, vofk::k_pred_lines<1>, gp, dim3(256), shm,
+              ctx->grid, P);
+          break;
+        case 2:
+          KLB(ctx, "k_pred_lines", 256, vofk::k_pred_lines_body<2>, vofk::k_pred_lines<2>, gp, dim3(256), shm,
+              ctx->grid, P);
+          break;
+        case 4:
+          KLB(ctx, "k_pred_lines", 256, vofk::k_pred_lines_body<4>, vofk::k_pred_lines<4>, gp, dim3(256), shm,
+              ctx->grid, P);
+          break;
+        default:
+          KLB(ctx, "k_pred_lines", 256, vofk::k_pred_lines_body<8>, vofk::k_pred_lines<8>, gp, dim3(256), shm,
+              ctx->grid, P);
+      }
+      for (int f = 0; f < nj; f++) {
+        const int kind = jobs[f].kind;
+        void * tmp = jobs[f].field->data;
+        jobs[f].field->data = ctx->d_fspare[kind];
+        ctx->d_fspare[kind] = (double *)tmp;
+      }
+      // the last reader of the curvature of this stage is done
+      if (ctx->pending_iface && ctx->iface_overlap == 1 && side_iface(ctx)) return 1;
+      return 0;
     }
     const int rows = ctx->rhs_rows;
     switch (rows) {
@@ -1956,7 +2069,7 @@ extern "C" int interface_update_vof(const domain_t * domain, const size_t rkstep
 // ---------------------------------------------------------------------------
 // integrate (src/integrate.c:11-68)
 
-enum { F_IFACE = 1, F_VOF = 2, F_RHS = 4, F_LINES = 8, F_PSI = 16, F_CORR = 32 };
+enum { F_IFACE = 1, F_VOF = 2, F_RHS = 4, F_LINES = 8, F_PSI = 16, F_CORR = 32, F_PRED = 64 };
 
 // the VOF ping-pong buffer of the fused VOF update
 static int ensure_vof_spare(vof_ctx * ctx, const interface_t * interface) {
@@ -1996,6 +2109,7 @@ static int ensure_side(vof_ctx * ctx) {
 static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t * interface, double * dt) {
   vof_ctx * ctx = ctx_of(domain);
   if ((ctx->fuse & F_VOF) && ensure_vof_spare(ctx, interface)) return 1;
+  if ((ctx->fuse & F_PRED) && ensure_field_spares(ctx, fluid)) return 1;
   if (overlap_ok(ctx)) {
     if (ensure_side(ctx)) return 1;
     // the first stage's interface kernel needs no dt: it runs on the main
@@ -2056,10 +2170,12 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
   vof_ctx * ctx = ctx_of(domain);
   const int fz = ctx->fuse;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+  // the field buffers are read at each use: the one-kernel predictor swaps
+  // ux, uy, uz, t with their second buffers every stage
 #if NDIMS == 3
-  double * uz = (double *)fluid->uz.data;
+  auto uzp = [&]() { return (double *)fluid->uz.data; };
 #else
-  double * uz = nullptr;
+  auto uzp = [&]() { return (double *)nullptr; };
 #endif
   // the interface kernel of stage rk + 1 reads only the vof that stage rk's
   // VOF update wrote, and writes the normals and curvature whose last reader
@@ -2101,7 +2217,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
       const double dta = dtv * kRk[rk][0];
       const double dtb = dtv * kRk[rk][1];
       KLB(ctx, "k_vof_fused", 256, vofk::k_vof_fused_body, vofk::k_vof_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
-          ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uz,
+          ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uzp(),
           (const double *)interface->vof.data, (const double *)interface->normal.data,
           (double *)interface->src[0].data, (const double *)interface->src[1].data,
           ctx->d_vof_spare, dta, dtb, (int)(rk != 0), (int)(rk != 2));
@@ -2117,7 +2233,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
     // before the Thomas pass of the Poisson solve (2)
     ctx->pending_iface = (overlap && rk + 1 < rk1) ? interface : nullptr;
     if (predict_impl(domain, rk, dtv, fluid, interface, 1, rhs_fused ? vof_old : nullptr,
-                     (fz & F_LINES) ? 1 : 0)) {
+                     (fz & F_LINES) ? 1 : 0, (fz & F_PRED) != 0)) {
       ctx->pending_iface = nullptr;
       return 1;
     }
@@ -2133,7 +2249,7 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
       const double pref_p = 0.5 * kRk[rk][2] * dtv * fluid->m_dif;
       KLB(ctx, "k_corr_press", 256, vofk::k_corr_press_body, vofk::k_corr_press, grid3(nx, ny, nz, kBlk), kBlk, 0,
           ctx->grid, pref_c, pref_p, (const double *)fluid->psi.data, (double *)fluid->ux.data,
-          (double *)fluid->uy.data, uz, (double *)fluid->p.data);
+          (double *)fluid->uy.data, uzp(), (double *)fluid->p.data);
     } else {
       if (fluid_correct_velocity(domain, rk, dtv, fluid)) return 1;
       if (fluid_update_pressure(domain, rk, dtv, fluid)) return 1;
@@ -2168,8 +2284,10 @@ extern "C" int integrate(const domain_t * domain, fluid_t * fluid, interface_t *
 // "lines_lpb" (lines per work-group of the x solver, 0 = automatic),
 // "xcd_bands" (band tile order of the 3D stencil kernels), "rhs_rows"
 // (rows of one column marched per thread by the source kernel: 1, 2, 4, 8,
-// 16) and "iface_planes" (z planes marched per work-group by the 3D
-// interface kernel: 0 = 32 x 4 x 4 blocks, 4, 8, 16)
+// 16), "pred_rows" (rows marched per thread by the one-kernel predictor: 1,
+// 2, 4, 8; fewer where the work-group's x lines would not fit 64 KiB of LDS)
+// and "iface_planes" (z planes marched per work-group by the 3D interface
+// kernel: 0 = 32 x 4 x 4 blocks, 4, 8, 16)
 extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   CTX_OR_FAIL(domain);
   if (!strcmp(name, "fft_elems")) {
@@ -2215,6 +2333,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   if (!strcmp(name, "rhs_rows")) {
     if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return fail("rhs_rows must be 1, 2, 4, 8 or 16");
     ctx->rhs_rows = value;
+    return 0;
+  }
+  if (!strcmp(name, "pred_rows")) {
+    if (value != 1 && value != 2 && value != 4 && value != 8) return fail("pred_rows must be 1, 2, 4 or 8");
+    ctx->pred_rows = value;
     return 0;
   }
   return fail("vof_set_tuning: unknown parameter %s", name);
@@ -2387,7 +2510,7 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
       delete B;
       return fail("vof_batch_create: case %d has no initialised domain", c);
     }
-    if (ctx->schedule != 1 || ctx->fuse != 63) {
+    if (ctx->schedule != 1 || (ctx->fuse & 63) != 63) {
       delete B;
       return fail("vof_batch_create: case %d is not on the fully fused schedule", c);
     }
@@ -2423,6 +2546,7 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
   for (int c = 0; c < ncases && !rc; c++) {
     vof_ctx * ctx = B->ctx[c];
     rc = ensure_vof_spare(ctx, B->interface[c]);
+    if (!rc && (ctx->fuse & F_PRED)) rc = ensure_field_spares(ctx, B->fluid[c]);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail("vof_batch_create: case stream failed");
     src[c] = ctx->d_part + ctx->npart;
   }
@@ -2546,7 +2670,10 @@ extern "C" int vof_batch_synchronize(vof_batch_t * B) {
 extern "C" int vof_batch_destroy(vof_batch_t * B) {
   if (!B) return 0;
   if (B->stream) (void)hipStreamSynchronize(B->stream);
-  // cases keep working on their own after the batch: give them private tables
+  // cases keep working on their own after the batch: give them private
+  // tables.  A case whose tables cannot be allocated is left without any
+  // (upload_tab then refuses to step it) and the call reports the failure.
+  int rc = 0;
   for (auto * ctx : B->ctx) {
     if (!ctx->tab_shared) continue;
     double * h = nullptr;
@@ -2557,8 +2684,10 @@ extern "C" int vof_batch_destroy(vof_batch_t * B) {
       ctx->d_tab = d;
       ctx->tab_rk = 4 * ctx->tab_stride;
     } else {
+      host_free(h);
       ctx->h_tab = nullptr;
       ctx->d_tab = nullptr;
+      rc = fail("vof_batch_destroy: no memory for a case's own Thomas tables; that case can no longer be stepped");
     }
     ctx->tab_shared = false;
   }
@@ -2575,7 +2704,7 @@ extern "C" int vof_batch_destroy(vof_batch_t * B) {
   dev_free((void *)B->d_dt_src);
   if (B->stream) (void)hipStreamDestroy(B->stream);
   delete B;
-  return 0;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -2595,6 +2724,8 @@ struct TuneSnap {
   vof_ctx * ctx = nullptr;
   double * spare = nullptr;     // the VOF ping-pong buffer and its contents
   void * spare_copy = nullptr;
+  array_t * fld[4] = {nullptr, nullptr, nullptr, nullptr};   // ux, uy, uz, t (ping-pong with d_fspare)
+  void * fld_buf[4] = {nullptr, nullptr, nullptr, nullptr};  // their buffers at snapshot time
   TuneSnap() = default;
   TuneSnap(const TuneSnap &) = delete;
   TuneSnap & operator=(const TuneSnap &) = delete;
@@ -2629,6 +2760,13 @@ static int tune_take(vof_ctx * ctx, fluid_t * f, interface_t * s, hipStream_t st
     S.bytes.push_back(a->datasize);
     HIPCHK(hipMemcpyAsync(c, a->data, a->datasize, hipMemcpyDeviceToDevice, st));
   }
+  S.fld[0] = &f->ux;
+  S.fld[1] = &f->uy;
+#if NDIMS == 3
+  S.fld[2] = &f->uz;
+#endif
+  S.fld[3] = &f->t;
+  for (int m = 0; m < 4; m++) S.fld_buf[m] = S.fld[m] ? S.fld[m]->data : nullptr;
   S.spare = ctx->d_vof_spare;
   if (S.spare) {
     HIPCHK(hipMalloc(&S.spare_copy, ctx->vof_bytes));
@@ -2638,7 +2776,16 @@ static int tune_take(vof_ctx * ctx, fluid_t * f, interface_t * s, hipStream_t st
 }
 
 static int tune_restore(TuneSnap & S, hipStream_t st) {
-  // buffers first (undoing the pointer swaps of the steps), then contents
+  // buffers first (undoing the pointer swaps of the steps), then contents.
+  // A field and its second buffer are a pair whose roles the one-kernel
+  // predictor swaps: the field gets its snapshot buffer back and the second
+  // buffer is the pair's other member (whose contents the predictor rewrites
+  // in full before reading them).
+  for (int m = 0; m < 4; m++) {
+    if (!S.fld[m]) continue;
+    void * cur = S.fld[m]->data;
+    if (cur != S.fld_buf[m]) S.ctx->d_fspare[m] = (double *)cur;
+  }
   for (size_t q = 0; q < S.arr.size(); q++) S.arr[q]->data = S.buf[q];
   S.ctx->d_vof_spare = S.spare;
   for (size_t q = 0; q < S.arr.size(); q++)
@@ -2671,6 +2818,7 @@ static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
 #if NDIMS == 3
   k.push_back({"iface_planes", {8, 0, 4, 16}, &vof_ctx::iface_planes});
 #endif
+  k.push_back({"pred_rows", {8, 4, 2}, &vof_ctx::pred_rows});
   k.push_back({"rhs_rows", {8, 1, 2, 4, 16}, &vof_ctx::rhs_rows});
   k.push_back({"iface_overlap", {1, 2, 0}, &vof_ctx::iface_overlap});
   k.push_back({"thomas_rows", {8, 16, 32}, &vof_ctx::thomas_rows});
@@ -2749,9 +2897,10 @@ extern "C" int vof_autotune(domain_t * domain, fluid_t * fluid, interface_t * in
   CTX_OR_FAIL(domain);
   if (!fluid || !interface) return fail("vof_autotune: fluid or interface is NULL");
   if (steps < 1 || steps > 1000) return fail("vof_autotune: steps must be in 1..1000");
-  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_autotune: fused schedule only");
+  if (ctx->schedule != 1 || (ctx->fuse & 63) != 63) return fail("vof_autotune: fused schedule only");
   if (ctx->rec || ctx->tape) return fail("vof_autotune: case is recording");
   if (ensure_vof_spare(ctx, interface)) return 1;
+  if ((ctx->fuse & F_PRED) && ensure_field_spares(ctx, fluid)) return 1;
   const bool prof = ctx->prof_on;
   ctx->prof_on = false;
   std::vector<vof_ctx *> cs = {ctx};
@@ -2799,8 +2948,9 @@ extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, siz
 extern "C" int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, interface_t * interface,
                                       double dt) {
   CTX_OR_FAIL(domain);
-  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_integrate_fixed_dt: fused schedule only");
+  if (ctx->schedule != 1 || (ctx->fuse & 63) != 63) return fail("vof_integrate_fixed_dt: fused schedule only");
   if (ensure_vof_spare(ctx, interface)) return 1;
+  if ((ctx->fuse & F_PRED) && ensure_field_spares(ctx, fluid)) return 1;
   return integrate_fused_stages(domain, fluid, interface, dt);
 }
 
@@ -2890,7 +3040,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
                                     double * grad_uy) {
   CTX_OR_FAIL(domain);
   if (nsteps < 1 || !dts || every < 1) return fail("vof_adjoint_gradient: bad arguments");
-  if (ctx->schedule != 1 || ctx->fuse != 63) return fail("vof_adjoint_gradient: fused schedule only");
+  if (ctx->schedule != 1 || (ctx->fuse & 63) != 63) return fail("vof_adjoint_gradient: fused schedule only");
   {
     double buoy = 0.;
     if (param_get(ctx, P_BUOY, &buoy)) return 1;
@@ -2898,6 +3048,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
   }
   if (ctx->poisson_refine >= 0) return fail("vof_adjoint_gradient: plain FP64 Poisson only");
   if (ensure_vof_spare(ctx, interface)) return 1;
+  if ((ctx->fuse & F_PRED) && ensure_field_spares(ctx, fluid)) return 1;
   const int nx = ctx->nx, ny = ctx->ny;
   const size_t bvof = interface->vof.datasize, bux = fluid->ux.datasize, buy = fluid->uy.datasize;
   const size_t bp = fluid->p.datasize;
@@ -3102,7 +3253,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
 // stages of the fused schedule (bit mask; the default is all of them)
 extern "C" int vof_set_fusion(domain_t * domain, int mask) {
   CTX_OR_FAIL(domain);
-  if (mask < 0 || mask > 63) return fail("vof_set_fusion: mask %d out of range", mask);
+  if (mask < 0 || mask > 127) return fail("vof_set_fusion: mask %d out of range", mask);
   ctx->fuse = mask;
   return 0;
 }

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"fft_radix": {4, 16}, "fft_elems": {1024, 2048}, "iface_planes": {0, 4, 8, 16},
               "rhs_rows": {1, 2, 4, 8, 16}, "lines_lpb": {0, 2, 4, 8, 16, 32}, "iface_overlap": {0, 1, 2},
-              "thomas_rows": {8, 16, 32}}
+              "thomas_rows": {8, 16, 32}, "pred_rows": {2, 4, 8}}
 
 
 def _names(nd):
@@ -100,7 +100,7 @@ def test_autotune_restores_state_and_keeps_results(glsizes, seed, tmp_path):
         before = {n: g1.get(n) for n in _names(nd)}
         report = g1.autotune(steps=1)
         best = _chosen(report)
-        expect = {"rhs_rows", "lines_lpb"} | ({"iface_planes"} if nd == 3 else set())
+        expect = {"pred_rows", "rhs_rows", "lines_lpb"} | ({"iface_planes"} if nd == 3 else set())
         assert expect <= set(best), best
         for n in _names(nd):
             assert np.array_equal(g1.get(n), before[n]), n

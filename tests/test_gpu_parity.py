@@ -153,7 +153,9 @@ def test_droplet_2d_workload(tmp_path):
         g.close()
 
 
-FUSION_MASKS = [1, 2, 6, 8, 16, 32, 63]
+# 63: every stage fused, the predictor as source kernel + x-line kernel;
+# 79 and 127: the predictor as one kernel (sources, x solve, update)
+FUSION_MASKS = [1, 2, 6, 8, 16, 32, 63, 79, 127]
 
 
 @pytest.mark.parametrize("mask", FUSION_MASKS)
@@ -184,7 +186,8 @@ def test_fused_schedule_is_bit_identical(case, mask, tmp_path):
                                     ("rhs_rows", 1), ("rhs_rows", 2), ("rhs_rows", 4), ("rhs_rows", 16),
                                     ("iface_planes", 0), ("iface_planes", 4), ("iface_planes", 16),
                                     ("iface_overlap", 0), ("iface_overlap", 1), ("iface_overlap", 2),
-                                    ("thomas_rows", 16), ("thomas_rows", 32)])
+                                    ("thomas_rows", 16), ("thomas_rows", 32),
+                                    ("pred_rows", 1), ("pred_rows", 2), ("pred_rows", 4)])
 @pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
 def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
     """Tile sizes change the work split only: every line and every FFT column
@@ -194,6 +197,10 @@ def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
     ic = random_fields(glsizes, seed=seed, uniformx=uniformx)
     g0 = sva.solver_from_ic(ic, str(tmp_path / "a"))
     g1 = sva.solver_from_ic(ic, str(tmp_path / "b"))
+    if tuning[0] in ("rhs_rows", "lines_lpb"):
+        # parameters of the two-kernel predictor
+        g0.set_fusion(63)
+        g1.set_fusion(63)
     g1.set_tuning(*tuning)
     nd = len(glsizes)
     try:

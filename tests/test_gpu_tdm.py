@@ -51,7 +51,7 @@ def _gpu_solve(l, c, u, q, periodic, ndims=2):
     try:
         t.l[:], t.c[:], t.u[:] = l, c, u
         host = np.ascontiguousarray(q, dtype=np.complex128 if cplx else np.float64)
-        buf = DeviceBuffer.from_array(host)
+        buf = DeviceBuffer.from_array(host, ndims)
         t.solve(buf.ptr)
         out = buf.download(host.dtype, host.shape)
         buf.free()
@@ -106,7 +106,7 @@ def test_tdm_reuse_and_3d_library():
             l, c, u = _system(n, rng, False)
             t.l[:], t.c[:], t.u[:] = l, c, u
             q = rng.standard_normal((nrhs, n))
-            buf = DeviceBuffer.from_array(q)
+            buf = DeviceBuffer.from_array(q, 3)
             t.solve(buf.ptr)
             got = buf.download(np.float64, q.shape)
             ref = OL.tdm_solve(l, c, u, q.ravel()).reshape(q.shape)
@@ -145,10 +145,10 @@ def test_linear_system_init_builds_the_x_solver(ndims):
             assert getattr(vt, name)(ls.tdm_x, C.byref(p)) == 0
             np.ctypeslib.as_array(p, shape=(gl[0],))[:] = arr
         q = rng.standard_normal((nr.value, gl[0]))
-        sva.hip.runtime().hipMemcpy(ls.x1pncl, q.ctypes.data_as(C.c_void_p), q.nbytes, 1)
+        sva.hip.copy(ls.x1pncl, q.ctypes.data_as(C.c_void_p), q.nbytes, 1, ndims)
         assert vt.solve(ls.tdm_x, ls.x1pncl) == 0
         got = np.empty_like(q)
-        sva.hip.runtime().hipMemcpy(got.ctypes.data_as(C.c_void_p), ls.x1pncl, q.nbytes, 2)
+        sva.hip.copy(got.ctypes.data_as(C.c_void_p), ls.x1pncl, q.nbytes, 2, ndims)
         assert np.array_equal(got, OL.tdm_solve(l, c, u, q.ravel()).reshape(q.shape))
         assert lib.linear_system_init(None, implicit, sizes, C.byref(ls)) != 0
         assert b"already initialised" in lib.vof_last_error()
