@@ -200,6 +200,18 @@ typedef struct {
   int (* const destruct)(tdm_info_t * info);
 } tdm_t;
 extern const tdm_t tdm;
+/* tdm.solve without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  l, c, u: DEVICE arrays of `size`
+ * coefficients; data: as tdm.solve (device, nrhs lines, real or interleaved
+ * complex), solved in place with the same kernels, so the answers are those of
+ * tdm.solve bit for bit.  The factor table lives in a fixed device array of
+ * the library (one per GPU, held until the process ends): size must be at
+ * most 16384, larger sizes fail with an error.  stream: a hipStream_t (NULL =
+ * the null stream); the call enqueues and returns without waiting.  Calls
+ * that overlap on one GPU race on that table. */
+int tdm_solve_noalloc(int size, int nrhs, bool is_periodic, bool is_complex, const double * l,
+                      const double * c, const double * u, void * data, void * stream);
 
 /* ---- linear system (replaces include/linear_system.h:37-46) -------------- */
 

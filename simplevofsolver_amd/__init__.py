@@ -80,6 +80,7 @@ ABI_SYMBOLS_COMMON = [
     "vof_batch_create", "vof_batch_integrate", "vof_batch_synchronize", "vof_batch_destroy",
     "vof_autotune", "vof_batch_autotune", "vof_batch_set_graphs", "vof_batch_graph_counts",
     "vof_integrate_fixed_dt", "vof_device_alloc", "vof_device_free", "vof_device_copy",
+    "tdm_solve_noalloc",
 ]
 ABI_SYMBOLS_2D = ["vof_adjoint_gradient", "vof_adjoint_loss"]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]

@@ -3815,6 +3815,50 @@ extern "C" const tdm_t tdm = {
   tdm_construct, tdm_get_l, tdm_get_c, tdm_get_u, tdm_get_size, tdm_get_nrhs, tdm_solve, tdm_destruct,
 };
 
+// tdm_solve_noalloc: the solve of the vtable for callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc).  The factor table lives in `scratch`, a
+// fixed device array of this library; each GPU holds its own copy, found
+// with hipGetSymbolAddress once per device (in host code the array's name is
+// not a device address).  Every call on one GPU shares that GPU's copy, so
+// calls that overlap on one GPU race; the table is fully rewritten (by
+// k_tdm_table) before the line kernel of the same call reads it.
+#endif  // !__HIP_DEVICE_COMPILE__
+
+constexpr int kTdmNoallocMax = 16384;   // largest system size of tdm_solve_noalloc
+__device__ __align__(256) double scratch[3 * kTdmNoallocMax + 3];
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+static std::mutex g_scratch_mu;
+static double * g_scratch_ptr[64];      // per device ordinal; null until looked up
+
+static int scratch_for_current_device(double ** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail("tdm_solve_noalloc: device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void * p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail("tdm_solve_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" int tdm_solve_noalloc(int size, int nrhs, bool is_periodic, bool is_complex, const double * l,
+                                 const double * c, const double * u, void * data, void * stream) {
+  if (tdm_check_sizes("tdm_solve_noalloc", size, nrhs, is_periodic)) return 1;
+  if (size > kTdmNoallocMax)
+    return fail("ERROR(tdm_solve_noalloc): size %d exceeds the fixed table's maximum %d", size, kTdmNoallocMax);
+  if (!l || !c || !u || !data) return fail("ERROR(tdm_solve_noalloc): null pointer");
+  double * tab = nullptr;
+  if (scratch_for_current_device(&tab)) return 1;
+  return tdm_enqueue(size, nrhs, is_periodic, is_complex, l, c, u, tab, data, (hipStream_t)stream);
+}
+
 #endif  // !__HIP_DEVICE_COMPILE__
 
 #if !defined(__HIP_DEVICE_COMPILE__)
