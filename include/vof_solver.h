@@ -273,7 +273,7 @@ int domain_finalise(domain_t * domain);
  *     reads them) and skips the beta-weighted source reads of the first
  *     stage (their weight is 0). */
 int vof_set_schedule(domain_t * domain, int schedule);
-/* stages fused by schedule 1 (bit mask, default 127 = all): 1 curvature
+/* stages fused by schedule 1 (bit mask, default 63): 1 curvature
  * tensor, 2 VOF update, 4 sources with the surface force, 8 boundary images
  * of the predictor, 16 boundary images of the potential, 32 correction with
  * pressure, 64 the whole predictor in one kernel (sources, x solve and

@@ -3381,6 +3381,44 @@ __device__ __forceinline__ void k_pred_lines_body(const VBlock & vb, Grid g, con
     }
   }
   __syncthreads();
+  // 3a. the update is flattened over the LDS lines: element e = tid + 256 q
+  // (q < 32; the LDS holds at most 8192 values) is row r of field slot,
+  // position m.  The old field values of this thread's elements are loaded
+  // before the sweeps, so that their latency passes during them.
+  constexpr int kPE = 32;
+  const int n1 = ny + 2;
+  const float inv_ld = 1.f / (float)ld;
+  const int rsh = 31 - __clz(R);   // R is a power of two
+  // e / ld through a float product: e < 2^13 and (e + 1/2) / ld stays at
+  // least 1/(2 ld) away from an integer, far beyond the product's rounding
+  auto decode = [&](int e, int & slot, int & r, int & m) {
+    const int line = (int)(((float)e + 0.5f) * inv_ld);
+    m = e - line * ld;
+    slot = line >> rsh;
+    r = line & (R - 1);
+  };
+  auto old_of = [&](int kind) -> const double * {
+    return kind == 0 ? f.base.ux : kind == 1 ? f.base.uy : kind == 2 ? f.base.uz : f.base.t;
+  };
+  auto row_of = [&](int kind, int j) -> size_t {
+    const size_t n0 = kind == 0 ? (size_t)nx + 1 : (size_t)nx + 2;
+#if NDIMS == 3
+    return n0 * ((size_t)j + (size_t)n1 * (size_t)k);
+#else
+    return n0 * (size_t)j;
+#endif
+  };
+  double ov[kPE];
+#pragma unroll
+  for (int q = 0; q < kPE; q++) {
+    int slot, r, m;
+    decode((int)threadIdx.x + 256 * q, slot, r, m);
+    ov[q] = 0.;
+    if (slot < kPNF && r < nrows) {
+      const int kind = pred_kind(slot);
+      if (m < (kind == 0 ? nx - 1 : nx)) ov[q] = old_of(kind)[row_of(kind, j0 + r) + 1 + m];
+    }
+  }
   // 2. one wavefront per field, one lane per row
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int lane = (int)threadIdx.x & 63;
@@ -3392,54 +3430,42 @@ __device__ __forceinline__ void k_pred_lines_body(const VBlock & vb, Grid g, con
     lines_sweep(F, S + (size_t)w * fs + (size_t)lane * ld);
   }
   __syncthreads();
-  // 3. new field rows: wavefront w takes lines w, w + 4, ... (field-major);
-  // a line's field, row and halo decision are wave-uniform, lanes walk x
-  const int n1 = ny + 2;
-  for (int l = w; l < kPNF * nrows; l += 4) {
-    const int slot = l / nrows, r = l - slot * nrows;
-    const int kind = pred_kind(slot);
-    const int j = j0 + r;
-    const int n = kind == 0 ? nx - 1 : nx;
-    const int n0 = kind == 0 ? nx + 1 : nx + 2;
-    const double * old = kind == 0 ? f.base.ux : kind == 1 ? f.base.uy : kind == 2 ? f.base.uz : f.base.t;
-    double * const out = pick4(P.out, kind);
-#if NDIMS == 3
-    const size_t roff = (size_t)n0 * ((size_t)j + (size_t)n1 * (size_t)k);
-#else
-    const size_t roff = (size_t)n0 * (size_t)j;
-#endif
-    const double * __restrict__ orow = old + roff;
-    double * __restrict__ nrow = out + roff;
-    const double * srow = S + (size_t)slot * fs + (size_t)r * ld;
-    // rows next to the periodic halo also write their images (walls
-    // included): the reference's halo copy of the updated field
-    const bool edge = j < 2 || j >= ny
-#if NDIMS == 3
-                      || k < 2 || k >= nz
-#endif
-        ;
-    for (int m0 = 0; m0 < n; m0 += 64 * kWUnroll) {
-      double v[kWUnroll];
+  // 3b. new field values, the x walls (constants the predictor never
+  // changes) and, in rows next to the periodic halo, the images of both (the
+  // reference's halo copy of the updated field).  The element positions are
+  // decoded again rather than kept from 3a (an empty asm hides that the
+  // thread index is the same): holding 32 addresses across the sweeps would
+  // cost the registers of a second wave per SIMD.
+  int tid3 = (int)threadIdx.x;
+  __asm__ __volatile__("" : "+v"(tid3));
 #pragma unroll
-      for (int u = 0; u < kWUnroll; u++) {
-        const int m = m0 + 64 * u + lane;
-        if (m < n) v[u] = orow[1 + m] + srow[m];
+  for (int q = 0; q < kPE; q++) {
+    const int e = tid3 + 256 * q;
+    int slot, r, m;
+    decode(e, slot, r, m);
+    if (slot < kPNF && r < nrows) {
+      const int kind = pred_kind(slot);
+      const int n = kind == 0 ? nx - 1 : nx;
+      const int n0 = kind == 0 ? nx + 1 : nx + 2;
+      const int j = j0 + r;
+      const size_t roff = row_of(kind, j);
+      double * const out = pick4(P.out, kind);
+      const bool edge = j < 2 || j >= ny
+#if NDIMS == 3
+                        || k < 2 || k >= nz
+#endif
+          ;
+      if (m < n) {
+        const double v = ov[q] + S[e];
+        out[roff + 1 + m] = v;
+        if (edge) store_images(out, 1 + m, n0, n1, j, PLANE1(k), ny, nz, 1, v);
       }
-#pragma unroll
-      for (int u = 0; u < kWUnroll; u++) {
-        const int m = m0 + 64 * u + lane;
-        if (m < n) {
-          nrow[1 + m] = v[u];
-          if (edge) store_images(out, 1 + m, n0, n1, j, PLANE1(k), ny, nz, 1, v[u]);
-        }
+      if (m == 0 || m == n - 1) {
+        const int xw = m == 0 ? 0 : n + 1;
+        const double wv = old_of(kind)[roff + xw];
+        out[roff + xw] = wv;
+        if (edge) store_images(out, xw, n0, n1, j, PLANE1(k), ny, nz, 1, wv);
       }
-    }
-    // the x walls (constants the predictor never changes)
-    if (lane < 2) {
-      const int xw = lane == 0 ? 0 : n + 1;
-      const double wv = orow[xw];
-      nrow[xw] = wv;
-      if (edge) store_images(out, xw, n0, n1, j, PLANE1(k), ny, nz, 1, wv);
     }
   }
 }
