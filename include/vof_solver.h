@@ -343,11 +343,25 @@ int vof_batch_destroy(vof_batch_t * batch);
 /* one time step with a given dt instead of fluid_decide_dt (fused schedule):
  * the frozen dt sequence of the adjoint and its finite-difference checks */
 int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, interface_t * interface, double dt);
+/* discrete decisions of the 2D fused step, for finite-difference checks of
+ * the adjoint: mode 1 records, RK stage by RK stage from the next step on,
+ * the upwind side and single-phase test of every face flux, the interface
+ * test and Newton update count of every cell's intercept, and which corner
+ * gradients lie below the adjoint's floor (with their normals); mode 2
+ * imposes the recorded decisions on the following stages, again from the
+ * first recorded one (so a perturbed run follows the map the adjoint
+ * differentiates); 0 stops.  2D only (3D: an error for modes 1 and 2).  No
+ * counterpart in the reference. */
+int vof_set_freeze(domain_t * domain, int mode);
 
 #if NDIMS == 2
 /* ---- adjoint (2D): gradient of a loss after nsteps w.r.t. the initial
  * velocity.  Loss = droplet centroid x (sum xc vof dV / sum vof dV) +
- * interface length (sum |grad vof| dV, central differences).  The forward
+ * interface length (sum |grad vof| dV, central differences, with |grad vof|
+ * regularised as sqrt(|g|^2 + g0^2) - g0, g0 = 1e-6 / (2 dx)).  The map differentiated is the step with its
+ * discrete decisions held (the ones vof_set_freeze records) and the normals
+ * of flat corners (|grad vof| DXC below the floor, "adj_floor_exp") held
+ * constant.  The forward
  * runs dts[0..nsteps-1] from the current state, keeping a checkpoint every
  * `every` steps; the reverse sweep recomputes each segment.  grad_ux /
  * grad_uy are host arrays of the padded ux / uy shapes (values on the
@@ -393,6 +407,7 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);
  * "adj_floor_exp" = e sets the adjoint's corner-gradient floor to 10^-e
+ * (default 2: corners with |grad vof| DXC < 1e-2 are held constant)
  * (this one changes adjoint results) */
 int vof_set_tuning(domain_t * domain, const char name[], int value);
 /* step.npy / time.npy of an IC or save directory (the reference's main.c

@@ -80,7 +80,7 @@ ABI_SYMBOLS_COMMON = [
     "vof_batch_create", "vof_batch_integrate", "vof_batch_synchronize", "vof_batch_destroy",
     "vof_autotune", "vof_batch_autotune", "vof_batch_set_graphs", "vof_batch_graph_counts",
     "vof_integrate_fixed_dt", "vof_device_alloc", "vof_device_free", "vof_device_copy",
-    "tdm_solve_noalloc",
+    "tdm_solve_noalloc", "vof_set_freeze",
 ]
 ABI_SYMBOLS_2D = ["vof_adjoint_gradient", "vof_adjoint_loss"]
 ABI_SYMBOLS_3D = ["fluid_update_boundaries_uz"]
@@ -147,6 +147,7 @@ def load(ndims: int):
     lib.vof_batch_synchronize.argtypes = [vp]
     lib.vof_batch_destroy.argtypes = [vp]
     lib.vof_integrate_fixed_dt.argtypes = [vp, vp, vp, C.c_double]
+    lib.vof_set_freeze.argtypes = [vp, C.c_int]
     if ndims == 2:
         lib.vof_adjoint_gradient.argtypes = [vp, vp, vp, C.c_int, dp, C.c_int, dp, dp, dp]
         lib.vof_adjoint_loss.argtypes = [vp, vp, dp]
@@ -335,6 +336,12 @@ class Solver:
         """One step with the given dt (no fluid_decide_dt)."""
         self._chk(self.lib.vof_integrate_fixed_dt(C.byref(self.domain), C.byref(self.fluid),
                                                   C.byref(self.interface), float(dt)), "vof_integrate_fixed_dt")
+
+    def set_freeze(self, mode: int):
+        """Decision record (1) / replay (2) / off (0) of the 2D fused step:
+        replayed runs keep the recorded upwind sides, single-phase tests,
+        Newton update counts and flat corners (vof_set_freeze)."""
+        self._chk(self.lib.vof_set_freeze(C.byref(self.domain), int(mode)), "vof_set_freeze")
 
     def adjoint_loss(self) -> float:
         out = C.c_double()

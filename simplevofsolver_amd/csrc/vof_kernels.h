@@ -42,6 +42,24 @@ __device__ __constant__ double kGaussW[NGAUSS] = {+0.5, +0.5};
 constexpr double kVofBeta = 1.;
 constexpr double kVofMin = 1.e-8;
 
+// Discrete decisions of the 2D step, recorded by one forward run (FZ = 1)
+// and imposed on later runs (FZ = 2): the map the adjoint differentiates
+// holds them fixed, so that finite differences of that map can be compared
+// with it.  FZ = 0: the step as the reference computes it.
+//   corner: corner gradient below the floor (1): its normal is the recorded one
+//   cell:   bit 7 interface cell (normal and intercept formed), bits 0-3 the
+//           number of Newton updates of its intercept
+//   fx, fy: x / y face: bit 0 upwind cell is the upper one (vel < 0), bit 1
+//           the upwind cell is single-phase (flux vel * vof)
+struct Freeze {
+  unsigned char * corner;   // I_DVOF index
+  unsigned char * cell;     // I_C1 index
+  unsigned char * fx;       // I_UX index of the face
+  unsigned char * fy;       // I_C1 index of the face's upper cell
+  double * dvof;            // corner normals of the recording run (2 per corner)
+  double floor;             // |grad vof| DXC below which a corner counts as flat
+};
+
 struct Grid {
   int nx, ny, nz;
   double dy, dz;
@@ -272,7 +290,8 @@ __global__ void k_gradient(Grid g, const double * __restrict__ vof, double * __r
 
 // compute_intercept (:50-92): Newton on D = exp(-2 beta d), at most 8
 // iterations, residual tested after the update
-__device__ inline double intercept(const double vof, const double * n) {
+template <int FZ = 0>
+__device__ inline double intercept(const double vof, const double * n, int * iters = nullptr) {
   const int cntmax = 8;
   const double resmax = 1.e-12;
 #if NDIMS == 2
@@ -289,7 +308,10 @@ __device__ inline double intercept(const double vof, const double * n) {
             exp(-2. * kVofBeta * (n[0] * kGaussP[ii] + n[1] * kGaussP[jj] + n[2] * kGaussP[kk]));
 #endif
   double val = 1. / vof - 1.;
-  for (int cnt = 0; cnt < cntmax; cnt++) {
+  // FZ = 2: exactly the recorded number of updates (no stopping test)
+  const int nit = FZ == 2 ? *iters : cntmax;
+  int done = cntmax;
+  for (int cnt = 0; cnt < nit; cnt++) {
     double f0 = -1. * vof;
     double f1 = 0.;
 #if NDIMS == 2
@@ -318,8 +340,12 @@ __device__ inline double intercept(const double vof, const double * n) {
       }
 #endif
     val -= f0 / f1;
-    if (fabs(f0) < resmax) break;
+    if (FZ != 2 && fabs(f0) < resmax) {
+      done = cnt + 1;
+      break;
+    }
   }
+  if (FZ == 1) *iters = done;
   return -0.5 / kVofBeta * log(val);
 }
 
@@ -461,12 +487,24 @@ __device__ inline double indicator(const double * n, const double * x) {
 // flxy.c:11-44 and the z counterpart; the same quadrature order).  Both
 // cells' vof are loaded by the caller, so the upwind choice does not wait on
 // the velocity; only the normal is read through the choice.
-template <int D>
-__device__ inline double face_flux(double vel, double vm, double vp, const double * nm, const double * np) {
-  const double lvof = vel < 0. ? vp : vm;
-  if (lvof < kVofMin || 1. - kVofMin < lvof) return vel * lvof;
-  const double * n = vel < 0. ? np : nm;
-  const double c = vel < 0. ? -0.5 : +0.5;
+template <int D, int FZ = 0>
+__device__ inline double face_flux(double vel, double vm, double vp, const double * nm, const double * np,
+                                   unsigned char * dec = nullptr) {
+  bool up, sp;   // upwind cell is p; the upwind cell is single-phase
+  if (FZ == 2) {
+    const unsigned b = *dec;
+    up = (b & 1u) != 0;
+    sp = (b & 2u) != 0;
+  } else {
+    up = vel < 0.;
+    const double lv = up ? vp : vm;
+    sp = lv < kVofMin || 1. - kVofMin < lv;
+    if (FZ == 1) *dec = (unsigned char)((up ? 1u : 0u) | (sp ? 2u : 0u));
+  }
+  const double lvof = up ? vp : vm;
+  if (sp) return vel * lvof;
+  const double * n = up ? np : nm;
+  const double c = up ? -0.5 : +0.5;
   double flux = 0.;
 #if NDIMS == 2
   for (int q = 0; q < NGAUSS; q++) {
@@ -2656,9 +2694,11 @@ constexpr int kIX = 32, kIY = 4, kIZ = 4;
 #endif
 constexpr int kIThreads = kIX * kIY * kIZ;
 
-__device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
-                                                           double * __restrict__ normal,
-                                                           double * __restrict__ curv) {
+template <int FZ>
+__device__ __forceinline__ void k_iface_fused_fz_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
+                                                      double * __restrict__ normal, double * __restrict__ curv,
+                                                      const Freeze & fz) {
+  static_assert(FZ == 0 || NDIMS == 2, "decision replay is 2D");
   constexpr int VX = kIX + 2, VY = kIY + 2;
   constexpr int CX = kIX + 1, CY = kIY + 1;
 #if NDIMS == 2
@@ -2714,6 +2754,20 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
     const double ninv = 1. / fmax(norm, DBL_EPSILON);
     sc[0][e] = gx * ninv;
     sc[1][e] = gy * ninv;
+    if (FZ != 0) {
+      const int j = j0 + b;
+      if (i <= nx + 1 && j <= ny + 1) {
+        const size_t ci = I_DVOF(i, j, 1);
+        if (FZ == 1) {
+          fz.corner[ci] = norm < fz.floor * rdx ? 1 : 0;
+          fz.dvof[2 * ci] = sc[0][e];
+          fz.dvof[2 * ci + 1] = sc[1][e];
+        } else if (fz.corner[ci]) {
+          sc[0][e] = fz.dvof[2 * ci];
+          sc[1][e] = fz.dvof[2 * ci + 1];
+        }
+      }
+    }
 #else
     const double a0 = SV(a, b, c), b0 = SV(a + 1, b, c), c0 = SV(a, b + 1, c), d0 = SV(a + 1, b + 1, c);
     const double a1 = SV(a, b, c + 1), b1 = SV(a + 1, b, c + 1), c1 = SV(a, b + 1, c + 1), d1 = SV(a + 1, b + 1, c + 1);
@@ -2766,7 +2820,14 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
 #endif
   store_periodic(curv, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, cv);
   const double lvof = sv[(a + 1) + VX * ((b + 1) + VY * (NDIMS == 3 ? c + 1 : 0))];
-  if (lvof < kVofMin || 1. - kVofMin < lvof) return;
+  bool iface;
+  if (FZ == 2) {
+    iface = (fz.cell[I_C1(i, j, k)] & 0x80u) != 0;
+  } else {
+    iface = !(lvof < kVofMin || 1. - kVofMin < lvof);
+    if (FZ == 1) fz.cell[I_C1(i, j, k)] = iface ? 0x80u : 0u;
+  }
+  if (!iface) return;
   double nv[NDIMS + 1];
 #if NDIMS == 2
   {
@@ -2796,7 +2857,11 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
   }
 #endif
 #undef SC
-  nv[NDIMS] = intercept(lvof, nv);
+  {
+    int it = FZ == 2 ? (int)(fz.cell[I_C1(i, j, k)] & 15u) : 0;
+    nv[NDIMS] = intercept<FZ>(lvof, nv, &it);
+    if (FZ == 1) fz.cell[I_C1(i, j, k)] = (unsigned char)(0x80u | (unsigned)it);
+  }
   // NORMAL rows are 0..ny+1 like cell arrays but x runs 1..nx without padding
   for (int d = 0; d <= NDIMS; d++) {
     // component d of each image: the component array is interleaved, so
@@ -2807,11 +2872,25 @@ __device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, co
   }
 }
 
+__device__ __forceinline__ void k_iface_fused_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
+                                                   double * __restrict__ normal, double * __restrict__ curv) {
+  k_iface_fused_fz_body<0>(vb, g, vof, normal, curv, Freeze{});
+}
+
 __global__ __launch_bounds__(kIThreads) void k_iface_fused(Grid g, const double * __restrict__ vof,
                                                            double * __restrict__ normal,
                                                            double * __restrict__ curv) {
   k_iface_fused_body(hw_block(), g, vof, normal, curv);
 }
+
+#if NDIMS == 2
+template <int FZ>
+__global__ __launch_bounds__(kIThreads) void k_iface_fused_fz(Grid g, const double * __restrict__ vof,
+                                                              double * __restrict__ normal,
+                                                              double * __restrict__ curv, Freeze fz) {
+  k_iface_fused_fz_body<FZ>(hw_block(), g, vof, normal, curv, fz);
+}
+#endif
 
 // Work-group barrier that waits for LDS operations only: global loads issued
 // before it stay in flight (the __syncthreads fence would wait for them).
@@ -2964,11 +3043,13 @@ __global__ __launch_bounds__(kMThreads) void k_iface_march(Grid g, const double 
 // interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
 // (compute_flux_*), the flux divergence and the RK update, written to the
 // other VOF buffer with its halo images (walls stay zero).
-__device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+template <int FZ>
+__device__ __forceinline__ void k_vof_fused_fz_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                             const double * __restrict__ uz, const double * __restrict__ vof,
                             const double * __restrict__ normal, double * __restrict__ srca,
                             const double * __restrict__ srcb, double * __restrict__ vof_new,
-                            double dta, double dtb, int use_b, int keep_a) {
+                            double dta, double dtb, int use_b, int keep_a, const Freeze & fz) {
+  static_assert(FZ == 0 || NDIMS == 2, "decision replay is 2D");
   const int nx = g.nx, ny = g.ny;
   THREAD_IJK(1, 1, 1)
   if (i > nx || j > ny) return;
@@ -2995,10 +3076,14 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
   const double vzm = vof[I_VOF(i, j, k - 1)], vzp = vof[I_VOF(i, j, k + 1)];
 #endif
   const double * nc = normal + NC * I_NRM(i, j, k);
-  const double fxm = i == 1 ? 0. : face_flux<0>(uxm, vxm, vc, normal + NC * I_NRM(i - 1, j, k), nc);
-  const double fxp = i == nx ? 0. : face_flux<0>(uxp, vc, vxp, nc, normal + NC * I_NRM(i + 1, j, k));
-  const double fym = face_flux<1>(uym, vym, vc, normal + NC * I_NRM(i, j - 1, k), nc);
-  const double fyp = face_flux<1>(uyp, vc, vyp, nc, normal + NC * I_NRM(i, j + 1, k));
+  unsigned char * dxm = FZ ? fz.fx + I_UX(i, j, k) : nullptr;
+  unsigned char * dxp = FZ ? fz.fx + I_UX(i + 1, j, k) : nullptr;
+  unsigned char * dym = FZ ? fz.fy + I_C1(i, j, k) : nullptr;
+  unsigned char * dyp = FZ ? fz.fy + I_C1(i, j + 1, k) : nullptr;
+  const double fxm = i == 1 ? 0. : face_flux<0, FZ>(uxm, vxm, vc, normal + NC * I_NRM(i - 1, j, k), nc, dxm);
+  const double fxp = i == nx ? 0. : face_flux<0, FZ>(uxp, vc, vxp, nc, normal + NC * I_NRM(i + 1, j, k), dxp);
+  const double fym = face_flux<1, FZ>(uym, vym, vc, normal + NC * I_NRM(i, j - 1, k), nc, dym);
+  const double fyp = face_flux<1, FZ>(uyp, vc, vyp, nc, normal + NC * I_NRM(i, j + 1, k), dyp);
   double s = 0.;
   s += g.xt[kXT * i + XT_RXF] * (fxm - fxp);   // 1 / DXF(i)
   s += g.rdy * (fym - fyp);
@@ -3014,6 +3099,14 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
   store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
 }
 
+__device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                            const double * __restrict__ uz, const double * __restrict__ vof,
+                            const double * __restrict__ normal, double * __restrict__ srca,
+                            const double * __restrict__ srcb, double * __restrict__ vof_new,
+                            double dta, double dtb, int use_b, int keep_a) {
+  k_vof_fused_fz_body<0>(vb, g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a, Freeze{});
+}
+
 __global__ __launch_bounds__(256) void k_vof_fused(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                             const double * __restrict__ uz, const double * __restrict__ vof,
                             const double * __restrict__ normal, double * __restrict__ srca,
@@ -3021,6 +3114,17 @@ __global__ __launch_bounds__(256) void k_vof_fused(Grid g, const double * __rest
                             double dta, double dtb, int use_b, int keep_a) {
   k_vof_fused_body(hw_block(), g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a);
 }
+
+#if NDIMS == 2
+template <int FZ>
+__global__ __launch_bounds__(256) void k_vof_fused_fz(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
+                            const double * __restrict__ vof, const double * __restrict__ normal,
+                            double * __restrict__ srca, const double * __restrict__ srcb,
+                            double * __restrict__ vof_new, double dta, double dtb, int use_b, int keep_a, Freeze fz) {
+  k_vof_fused_fz_body<FZ>(hw_block(), g, ux, uy, nullptr, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a,
+                          fz);
+}
+#endif
 
 // fluid_predict_field sources of all fields at once, with the CSF force
 // (force.c:11-66) formed in place from vof and curvature; writes srca and the

@@ -239,6 +239,12 @@ struct vof_ctx {
   double * d_vof_spare;                  // second VOF buffer of the fused update
   size_t vof_bytes;
   double * d_fspare[4];                  // second buffer of ux, uy, uz, t (one-kernel predictor)
+  // decision record / replay of the 2D step (vof_set_freeze): one slot per
+  // RK stage, each one device block {corner, cell, x face, y face flags,
+  // corner normals}
+  int freeze_mode;                       // 0 off, 1 record, 2 replay
+  size_t freeze_pos, freeze_n;           // next stage slot, slots recorded
+  std::vector<void *> freeze_slots;
   int pred_rows;                         // rows marched per thread by the one-kernel predictor (1, 2, 4, 8)
   Grid grid;
   double * d_dxf;
@@ -950,7 +956,7 @@ static int ctx_create(domain_t * domain) {
   // are short and the cross-stream events cost more than the overlap gains
   ctx->iface_overlap = (size_t)ctx->nx * (size_t)ctx->ny * (size_t)ctx->nz >= ((size_t)1 << 20) ? 1 : 0;
   ctx->poisson_refine = -1;
-  ctx->adj_floor = 1e-9;
+  ctx->adj_floor = 1e-2;
   ctx->lines_lpb = 0;
   ctx->rhs_rows = 8;
   ctx->iface_planes = 8;
@@ -958,6 +964,8 @@ static int ctx_create(domain_t * domain) {
   ctx->vof_bytes = 0;
   for (int m = 0; m < 4; m++) ctx->d_fspare[m] = nullptr;
   ctx->pred_rows = 8;
+  ctx->freeze_mode = 0;
+  ctx->freeze_pos = ctx->freeze_n = 0;
   ctx->prof_on = false;
   ctx->ev_next = 0;
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
@@ -1183,6 +1191,7 @@ extern "C" int domain_finalise(domain_t * domain) {
     dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
     if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
     for (int m = 0; m < 4; m++) dev_free(ctx->d_fspare[m]);
+    for (void * b : ctx->freeze_slots) dev_free(b);
     for (int m = 0; m < 9; m++) dev_free(ctx->d_stat[m]);
     host_free(ctx->h_scalar);
     prof_flush(ctx);
@@ -2095,7 +2104,7 @@ static int launch_iface(vof_ctx * ctx, interface_t * interface);
 static bool overlap_ok(const vof_ctx * ctx) {
   const int fz = ctx->fuse;
   return ctx->iface_overlap && (fz & F_IFACE) && (fz & F_VOF) && (fz & F_RHS) && !ctx->rec && !ctx->tape &&
-         !ctx->prof_on;
+         !ctx->prof_on && !ctx->freeze_mode;
 }
 
 // the side stream and its events, created on first use (a context that
@@ -2136,10 +2145,49 @@ static int integrate_fused(const domain_t * domain, fluid_t * fluid, interface_t
   return integrate_fused_stages(domain, fluid, interface, *dt);
 }
 
+#if NDIMS == 2
+// the decision slot of the current RK stage (allocated while recording)
+static int freeze_slot(vof_ctx * ctx, vofk::Freeze * fz) {
+  const size_t nx = (size_t)ctx->nx, ny = (size_t)ctx->ny;
+  const size_t nc = (nx + 1) * (ny + 3), ncell = (nx + 2) * (ny + 2), nfx = (nx + 1) * (ny + 2);
+  const size_t flags = (nc + ncell + nfx + ncell + 7) / 8 * 8;
+  if (ctx->freeze_mode == 1 && ctx->freeze_pos == ctx->freeze_slots.size()) {
+    void * b = nullptr;
+    HIPCHK(hipMalloc(&b, flags + sizeof(double) * 2 * nc));
+    ctx->freeze_slots.push_back(b);
+  }
+  if (ctx->freeze_mode == 2 && ctx->freeze_pos >= ctx->freeze_n)
+    return fail("vof_set_freeze: replay past the %zu recorded RK stages", ctx->freeze_n);
+  unsigned char * b = (unsigned char *)ctx->freeze_slots[ctx->freeze_pos];
+  fz->corner = b;
+  fz->cell = b + nc;
+  fz->fx = b + nc + ncell;
+  fz->fy = b + nc + ncell + nfx;
+  fz->dvof = (double *)(b + flags);
+  fz->floor = ctx->adj_floor;
+  return 0;
+}
+#endif
+
 // RK stages rk0 .. rk1-1 (all three by default) of one step with dt known
 // fused curvature tensor (curvature_tensor.c:178-186) of the current vof
 // on ctx->stream
 static int launch_iface(vof_ctx * ctx, interface_t * interface) {
+#if NDIMS == 2
+  if (ctx->freeze_mode) {
+    vofk::Freeze fz;
+    if (freeze_slot(ctx, &fz)) return 1;
+    const dim3 gi((unsigned)((ctx->nx + vofk::kIX - 1) / vofk::kIX), (unsigned)((ctx->ny + vofk::kIY - 1) / vofk::kIY), 1);
+    const double * v = (const double *)interface->vof.data;
+    if (ctx->freeze_mode == 1)
+      KLN(ctx, "k_iface_fused", vofk::k_iface_fused_fz<1>, gi, dim3(vofk::kIThreads), ctx->grid, v,
+          (double *)interface->normal.data, (double *)interface->curv.data, fz);
+    else
+      KLN(ctx, "k_iface_fused", vofk::k_iface_fused_fz<2>, gi, dim3(vofk::kIThreads), ctx->grid, v,
+          (double *)interface->normal.data, (double *)interface->curv.data, fz);
+    return 0;
+  }
+#endif
   const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
   const double * v = (const double *)interface->vof.data;
   double * nrm = (double *)interface->normal.data;
@@ -2220,6 +2268,26 @@ static int integrate_fused_stages(const domain_t * domain, fluid_t * fluid, inte
       if (rk != 0) swap_data(&interface->src[0], &interface->src[1]);
       const double dta = dtv * kRk[rk][0];
       const double dtb = dtv * kRk[rk][1];
+#if NDIMS == 2
+      if (ctx->freeze_mode) {
+        vofk::Freeze fzs;
+        if (freeze_slot(ctx, &fzs)) return 1;
+        if (ctx->freeze_mode == 1)
+          KLN(ctx, "k_vof_fused", vofk::k_vof_fused_fz<1>, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid,
+              (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)interface->vof.data,
+              (const double *)interface->normal.data, (double *)interface->src[0].data,
+              (const double *)interface->src[1].data, ctx->d_vof_spare, dta, dtb, (int)(rk != 0), (int)(rk != 2),
+              fzs);
+        else
+          KLN(ctx, "k_vof_fused", vofk::k_vof_fused_fz<2>, grid3(nx, ny, nz, kBlk), kBlk, ctx->grid,
+              (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)interface->vof.data,
+              (const double *)interface->normal.data, (double *)interface->src[0].data,
+              (const double *)interface->src[1].data, ctx->d_vof_spare, dta, dtb, (int)(rk != 0), (int)(rk != 2),
+              fzs);
+        ctx->freeze_pos++;
+        if (ctx->freeze_mode == 1) ctx->freeze_n = ctx->freeze_pos;
+      } else
+#endif
       KLB(ctx, "k_vof_fused", 256, vofk::k_vof_fused_body, vofk::k_vof_fused, grid3(nx, ny, nz, kBlk), kBlk, 0,
           ctx->grid, (const double *)fluid->ux.data, (const double *)fluid->uy.data, (const double *)uzp(),
           (const double *)interface->vof.data, (const double *)interface->normal.data,
@@ -2947,6 +3015,27 @@ extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, siz
   return rc;
 }
 
+// decision record / replay of the 2D fused step (the map the adjoint
+// differentiates, for its finite-difference checks): mode 1 records the
+// decisions of every following RK stage from the first slot on, mode 2
+// imposes the recorded ones stage by stage from the first slot on, 0 stops
+extern "C" int vof_set_freeze(domain_t * domain, int mode) {
+  CTX_OR_FAIL(domain);
+#if NDIMS == 2
+  if (mode < 0 || mode > 2) return fail("vof_set_freeze: mode %d is not 0, 1 or 2", mode);
+  if (mode == 2 && ctx->freeze_n == 0) return fail("vof_set_freeze: no recorded decisions to replay");
+  if (mode != 0 && (ctx->schedule != 1 || (ctx->fuse & (F_IFACE | F_VOF)) != (F_IFACE | F_VOF)))
+    return fail("vof_set_freeze: needs the fused curvature and VOF kernels");
+  if (mode == 1) ctx->freeze_n = 0;
+  ctx->freeze_mode = mode;
+  ctx->freeze_pos = 0;
+  return 0;
+#else
+  (void)ctx;
+  return mode == 0 ? 0 : fail("vof_set_freeze: decision replay is implemented for 2D");
+#endif
+}
+
 // one step with a given dt (the frozen dt sequence of the adjoint and its
 // finite-difference checks); fused schedule
 extern "C" int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, interface_t * interface,
@@ -2964,11 +3053,18 @@ extern "C" int vof_integrate_fixed_dt(const domain_t * domain, fluid_t * fluid, 
 //
 // Loss after nsteps: the droplet centroid x = sum xc vof dV / sum vof dV
 // plus the interface length sum |grad vof| dV (central differences, x walls
-// at vof 0, periodic y).  The forward runs with the given dt sequence and
+// at vof 0, periodic y; |grad vof| regularised below vof differences of 1e-6
+// across a cell).  The map differentiated is the step with its discrete
+// decisions held (upwind sides, single-phase tests, Newton update counts:
+// vof_set_freeze replays them) and with the corner normals of flat corners
+// (|grad vof| DXC below the floor, 1e-2 by default) held constant.  The
+// forward runs with the given dt sequence and
 // keeps the state (ux, uy, p, vof) every `every` steps; each segment is then
 // recomputed with its stage states recorded and reversed stage by stage.
 // The Poisson and x-line solves are transposed as D S D^-1 around the forward
 // solvers S (the operators are D^-1 K with K symmetric, D the cell widths).
+
+static constexpr double kLenFloor = 1e-6;
 
 static double loss_of(const vof_ctx * ctx, const domain_t * domain, const std::vector<double> & vof,
                       std::vector<double> * grad) {
@@ -2982,14 +3078,24 @@ static double loss_of(const vof_ctx * ctx, const domain_t * domain, const std::v
     j = j < 1 ? j + ny : (j > ny ? j - ny : j);
     return vof[I_VOF(i, j, 1)];
   };
-  double s0 = 0., s1 = 0., len = 0.;
+  // compensated sums: the loss of a perturbed run is compared with the
+  // unperturbed one to ~1e-16 relative by finite differences
+  struct Sum {
+    double s = 0., c = 0.;
+    void add(double v) {
+      const double y = v - c, t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
+  };
+  Sum s0, s1, len;
   for (int j = 1; j <= ny; j++)
     for (int i = 1; i <= nx; i++) {
       const double dv = dxf[i - 1] * dy;
-      s0 += xc[i] * V(i, j) * dv;
-      s1 += V(i, j) * dv;
+      s0.add(xc[i] * V(i, j) * dv);
+      s1.add(V(i, j) * dv);
     }
-  const double X = s0 / s1;
+  const double X = s0.s / s1.s;
   if (grad) grad->assign(vof.size(), 0.);
   auto G = [&](int i, int j, double v) {
     if (i < 1 || i > nx) return;
@@ -3002,19 +3108,23 @@ static double loss_of(const vof_ctx * ctx, const domain_t * domain, const std::v
       const double hx = dxc[i - 1] + dxc[i];
       const double gx = (V(i + 1, j) - V(i - 1, j)) / hx;
       const double gy = (V(i, j + 1) - V(i, j - 1)) / (2. * dy);
-      const double nrm = sqrt(gx * gx + gy * gy);
-      len += nrm * dv;
+      // |grad vof| regularised below g0 (vof differences of 1e-6 across the
+      // cell): sqrt(|g|^2 + g0^2) - g0 is |g| - g0 + O(g0^2 / |g|) at the
+      // interface and |g|^2 / (2 g0) in single-phase regions, whose
+      // rounding-level vof variations then add nothing measurable (and no
+      // kink) to the length
+      const double g0 = kLenFloor / hx;
+      const double nrm = sqrt(gx * gx + gy * gy + g0 * g0);
+      len.add((nrm - g0) * dv);
       if (grad) {
-        G(i, j, dv * (xc[i] - X) / s1);
-        if (nrm > 0.) {
-          G(i + 1, j, dv * gx / nrm / hx);
-          G(i - 1, j, -dv * gx / nrm / hx);
-          G(i, j + 1, dv * gy / nrm / (2. * dy));
-          G(i, j - 1, -dv * gy / nrm / (2. * dy));
-        }
+        G(i, j, dv * (xc[i] - X) / s1.s);
+        G(i + 1, j, dv * gx / nrm / hx);
+        G(i - 1, j, -dv * gx / nrm / hx);
+        G(i, j + 1, dv * gy / nrm / (2. * dy));
+        G(i, j - 1, -dv * gy / nrm / (2. * dy));
       }
     }
-  return X + len;
+  return X + len.s;
 }
 
 extern "C" int vof_adjoint_loss(const domain_t * domain, const interface_t * interface, double * loss) {
@@ -3051,6 +3161,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
     if (buoy != 0.) return fail("vof_adjoint_gradient: buoyancy (temperature coupling) is not differentiated");
   }
   if (ctx->poisson_refine >= 0) return fail("vof_adjoint_gradient: plain FP64 Poisson only");
+  if (ctx->freeze_mode) return fail("vof_adjoint_gradient: decision record / replay is on (vof_set_freeze 0 first)");
   if (ensure_vof_spare(ctx, interface)) return 1;
   if ((ctx->fuse & F_PRED) && ensure_field_spares(ctx, fluid)) return 1;
   const int nx = ctx->nx, ny = ctx->ny;
