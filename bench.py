@@ -12,8 +12,9 @@ Per-kernel times come from HIP events recorded on the library's own stream
 around every launch (vof_profile) during a second timed pass of the same K
 steps, so the events stay out of the throughput timing; the dominant
 kernel's algorithmic bytes per launch divided by its mean launch time gives
-``roofline.achieved``.  ``cpu_baseline`` times the CPU oracle (a single-core
-C restatement of the reference step) on one step of the same workload.
+``roofline.achieved``.  ``cpu_baseline`` times the CPU oracle (a single-threaded
+C restatement of the reference step) on the same workload: one process
+(median of 3 steps) and one process per host core at once (at most 16).
 """
 from __future__ import annotations
 
@@ -102,21 +103,135 @@ def read_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(ic, steps: int):
-    """Time the single-core CPU oracle on `steps` steps of the same workload."""
+def cpu_worker(n: int, reps: int, sync_dir: str | None, nworkers: int) -> int:
+    """One CPU-baseline process: the oracle (single-threaded C) on the n^3
+    benchmark workload, `reps` steps each timed on its own; prints the step
+    times as one JSON line.  With sync_dir, every worker of a concurrent
+    group has built its oracle before any starts timing."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
+    from simplevofsolver_amd.ic import droplets_3d
+    ic, _ = droplets_3d(n, n, n, seed=2)
     o = Oracle(ic)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    del ic
+    if sync_dir:
+        open(os.path.join(sync_dir, f"ready{os.getpid()}"), "w").close()
+        t_wait = time.perf_counter()
+        while len([f for f in os.listdir(sync_dir) if f.startswith("ready")]) < nworkers:
+            if time.perf_counter() - t_wait > 600:
+                print(json.dumps({"error": "workers never all ready"}), flush=True)
+                return 1
+            time.sleep(0.05)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
         o.integrate()
-    el = time.perf_counter() - t0
-    cells = 1
-    for n in ic.glsizes:
-        cells *= n
-    return {"value": cells * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) of the same {'x'.join(map(str, ic.glsizes))} workload "
-                      f"on one host core ({el:.1f} s)"}
+        times.append(time.perf_counter() - t0)
+    print(json.dumps({"times": times}), flush=True)
+    return 0
+
+
+def cpu_baseline(n: int, reps: int = 3, workers: int | None = None):
+    """The CPU oracle on the same n^3 workload, timed on the host before the
+    GPU is touched: (1) one single-threaded process, median of `reps` steps;
+    (2) `workers` such processes at once (one per core, at most 16: the host
+    share of one GPU), the aggregate rate of their concurrent steps (median
+    step time).  (2) is an upper bound for a parallel CPU run of the step
+    on those cores (a decomposed run also exchanges halos and transposes).
+    The figure reported is (2); (1) is kept beside it."""
+    import statistics
+    import subprocess
+    cells = n ** 3
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--n", str(n), "--cpu-reps", str(reps)]
+
+    def run(nw):
+        with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as sync:
+            extra = ["--cpu-sync", sync, "--cpu-workers", str(nw)] if nw > 1 else []
+            procs = [subprocess.Popen(cmd + extra, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+                     for _ in range(nw)]
+            times = []
+            for p in procs:
+                out, err = p.communicate(timeout=900)
+                if p.returncode != 0:
+                    raise RuntimeError(f"CPU baseline worker failed: {err[-500:]}")
+                times.append(json.loads(out.strip().splitlines()[-1])["times"])
+        return times
+
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nw = workers or max(1, min(16, ncpu))
+    t0 = time.perf_counter()
+    single = run(1)[0]
+    t1 = time.perf_counter()
+    print(f"[bench] CPU oracle, one core: step times {', '.join(f'{t:.2f}' for t in single)} s", file=sys.stderr,
+          flush=True)
+    many = run(nw) if nw > 1 else [single]
+    t2 = time.perf_counter()
+    med1 = statistics.median(single)
+    medn = statistics.median([t for w in many for t in w])
+    print(f"[bench] CPU oracle, {nw} concurrent processes: median step {medn:.2f} s", file=sys.stderr, flush=True)
+    return {"value": round(nw * cells / medn / 1e6, 3), "unit": "Mcell-steps/s", "cores": nw, "kind": "port",
+            "sample": f"{nw} concurrent single-threaded oracle processes, each {reps} timed steps of the same "
+                      f"{n}^3 workload (median step {medn:.2f} s; {t2 - t1:.0f} s wall); host reports {ncpu} "
+                      f"CPUs; an upper bound for a parallel CPU step on {nw} cores",
+            "single_core": {"value": round(cells / med1 / 1e6, 3), "cores": 1, "reps": reps,
+                            "step_s": [round(t, 3) for t in single], "median_step_s": round(med1, 3),
+                            "wall_s": round(t1 - t0, 1)}}
+
+
+def drift_record(d0, d1, steps, ndims):
+    """Changes of the conserved / monitored quantities over `steps` steps
+    (logging/energy.c, logging/interface.c, logging/divergence.c)."""
+    ke = ("ke_x", "ke_y", "ke_z") if ndims == 3 else ("ke_x", "ke_y")
+    return {
+        "vof_volume_rel": abs(d1["vof_volume"] - d0["vof_volume"]) / d0["vof_volume"],
+        "max_div": d1["max_div"],
+        "kinetic_energy_rel": sum(d1[k] for k in ke) / sum(d0[k] for k in ke) - 1.0,
+        "interface_area_rel": d1["interface_area"] / d0["interface_area"] - 1.0,
+        "vof_min": d1["vof_min"],
+        "vof_max": d1["vof_max"],
+        "steps": steps,
+    }
+
+
+def drift_2d(sva, steps, oracle_steps=100):
+    """The 2D workload (128 x 256 on 1 x 2, droplet r = 0.25 at (0.5, 0.5),
+    velocity perturbation 1e-3, Re = 100, We = 10, CFL 0.4): drift of volume,
+    kinetic energy and interface length over `steps` steps, and after
+    `oracle_steps` steps the largest relative difference of ux, uy, vof from
+    the CPU oracle stepped from the same state."""
+    import numpy as np
+    from simplevofsolver_amd.ic import droplet_2d
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    ic = droplet_2d(seed=1)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        g = sva.solver_from_ic(ic, d)
+    try:
+        o = Oracle(ic)
+        d0 = g.diagnostics()
+        t0 = time.perf_counter()
+        for _ in range(oracle_steps):
+            g.integrate()
+            o.integrate()
+        diff = {}
+        for name in ("ux", "uy", "vof"):
+            a, b = g.get(name), o.get(name)
+            diff[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        for _ in range(max(0, steps - oracle_steps)):
+            g.integrate()
+        g.synchronize()
+        el = time.perf_counter() - t0
+        rec = drift_record(d0, g.diagnostics(), max(steps, oracle_steps), ndims=2)
+        rec["vs_oracle_after"] = {"steps": oracle_steps, **diff}
+        rec["wall_s"] = round(el, 2)
+        rec["workload"] = "2D 128x256 on 1x2, droplet r=0.25, perturbation 1e-3, Re=100, We=10, FP64"
+        print(f"[bench] 2D drift over {rec['steps']} steps: KE {rec['kinetic_energy_rel']:+.3e}, interface "
+              f"{rec['interface_area_rel']:+.3e}, volume {rec['vof_volume_rel']:.1e}; vs oracle after "
+              f"{oracle_steps} steps {diff}", file=sys.stderr, flush=True)
+        return rec
+    finally:
+        g.close()
 
 
 def mixed_form(g, cells, steps, plain_value, refine):
@@ -440,7 +555,11 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="cells per direction (3D cube)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="timed oracle steps per CPU-baseline process")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="concurrent CPU-baseline processes (0 = one per host CPU, at most 16)")
+    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sync", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", type=int, default=None)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
@@ -449,6 +568,8 @@ def main(argv=None) -> int:
                          "instead of the plain FP64 step")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
                     help="tile parameter passed to vof_set_tuning (fft_elems, lines_lpb, xcd_bands, rhs_rows)")
+    ap.add_argument("--forms", default="mixed,autotuned,batched,precision,adjoint",
+                    help="comma-separated forms to run after the plain step")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the mixed-precision and batched forms")
     ap.add_argument("--autotune", type=int, default=3, metavar="K",
@@ -457,13 +578,26 @@ def main(argv=None) -> int:
                     help="steps of the 512x256x256 precision workload (0 = skip)")
     ap.add_argument("--precision-refine", type=int, default=2,
                     help="FP64 refinements of its mixed-precision run")
+    ap.add_argument("--drift-steps", type=int, default=200,
+                    help="steps of the 3D workload after which its drift is reported")
+    ap.add_argument("--drift-2d-steps", type=int, default=1000,
+                    help="steps of the 2D 128x256 droplet workload for its drift report (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=96)
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args(argv)
+    if args.cpu_worker:
+        return cpu_worker(args.n, args.cpu_reps, args.cpu_sync, max(1, args.cpu_workers))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design note, not code)", file=sys.stderr)
         return 2
+
+    # the CPU baseline runs first, in child processes started before this
+    # process touches the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.n, args.cpu_reps, args.cpu_workers or None)
 
     import simplevofsolver_amd as sva
     from simplevofsolver_amd.ic import droplets_3d
@@ -506,6 +640,12 @@ def main(argv=None) -> int:
     rep = g.profile_report()
     g.profile(False)
     d1 = g.diagnostics()
+    # drift over the workload's length: continue to `drift_steps` steps
+    steps_done = args.warmup + 2 * args.steps
+    for _ in range(max(0, args.drift_steps - steps_done)):
+        g.integrate()
+    d2 = g.diagnostics()
+    steps_drift = max(steps_done, args.drift_steps)
     # interface cells of the last state (their count moves by well under 1 %
     # over these steps): the cells whose normals the interface kernels write
     vint = _interior("vof", g.get("vof"), n, n, n)
@@ -522,13 +662,14 @@ def main(argv=None) -> int:
     traffic = read_traffic(kname, f"3d{n}")
     copy_gbs = g.copy_probe(1 << 27, 10)
     forms = {}
-    if not args.no_forms and args.poisson_mixed < 0:
+    want = set() if args.no_forms else set(args.forms.split(","))
+    if "mixed" in want and args.poisson_mixed < 0:
         try:
             forms["mixed_precision"] = mixed_form(g, cells, args.steps, value, 1)
             forms["mixed_precision_fp32_solve"] = mixed_form(g, cells, args.steps, value, 0)
         except Exception as e:  # reported, never fatal for the plain line
             forms["mixed_precision"] = {"error": repr(e)}
-    if not args.no_forms and args.poisson_mixed < 0 and args.autotune > 0:
+    if "autotuned" in want and args.poisson_mixed < 0 and args.autotune > 0:
         try:
             forms["autotuned"] = autotuned_form(g, cells, args.steps, value, args.autotune)
         except Exception as e:
@@ -543,16 +684,17 @@ def main(argv=None) -> int:
                   file=sys.stderr)
         print(f"[bench] copy probe {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     g.close()
-    if not args.no_forms:
+    if "batched" in want:
         try:
             forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps, args.autotune)
         except Exception as e:
             forms["batched"] = {"error": repr(e)}
-        if args.precision_steps > 0:
-            try:
-                forms["precision"] = precision_form(sva, args.precision_steps, args.precision_refine)
-            except Exception as e:
-                forms["precision"] = {"error": repr(e)}
+    if "precision" in want and args.precision_steps > 0:
+        try:
+            forms["precision"] = precision_form(sva, args.precision_steps, args.precision_refine)
+        except Exception as e:
+            forms["precision"] = {"error": repr(e)}
+    if "adjoint" in want:
         try:
             forms["adjoint"] = adjoint_form(sva)
         except Exception as e:
@@ -592,23 +734,15 @@ def main(argv=None) -> int:
         },
         "step_bytes_per_cell": 960,
         "step_hbm_frac": round(960 * value * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
-        "drift": {
-            "vof_volume_rel": abs(d1["vof_volume"] - d0["vof_volume"]) / d0["vof_volume"],
-            "max_div": d1["max_div"],
-            "kinetic_energy_rel": ((d1["ke_x"] + d1["ke_y"] + d1["ke_z"]) /
-                                   (d0["ke_x"] + d0["ke_y"] + d0["ke_z"]) - 1.0),
-            "interface_area_rel": d1["interface_area"] / d0["interface_area"] - 1.0,
-            "vof_min": d1["vof_min"],
-            "vof_max": d1["vof_max"],
-            "steps": args.warmup + 2 * args.steps,
-        },
+        "drift": drift_record(d0, d2, steps_drift, ndims=3),
     }
     out["forms"] = forms or None
-    if not args.no_cpu_baseline:
-        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)
-    else:
-        out["cpu_baseline"] = None
+    if args.drift_2d_steps > 0:
+        try:
+            out["drift_2d"] = drift_2d(sva, args.drift_2d_steps)
+        except Exception as e:
+            out["drift_2d"] = {"error": repr(e)}
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     return 0
 

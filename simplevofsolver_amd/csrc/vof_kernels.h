@@ -2893,11 +2893,14 @@ __global__ __launch_bounds__(kIThreads) void k_iface_fused_fz(Grid g, const doub
 #endif
 
 // Work-group barrier that waits for LDS operations only: global loads issued
-// before it stay in flight (the __syncthreads fence would wait for them).
-// Only for LDS hand-offs; the "memory" clobber keeps the compiler from
-// moving memory accesses across it.
+// before it stay in flight (a __syncthreads fence would wait for them).  The
+// release / acquire fences are scoped to the work-group and to LDS ("local"),
+// so the compiler orders the LDS accesses around the barrier itself and
+// waits on the LDS counter only (s_waitcnt lgkmcnt(0); s_barrier).
 __device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 #if NDIMS == 3
