@@ -1829,12 +1829,23 @@ template <> __device__ inline float2 from_d2<float2>(double2 a) { return make_fl
 
 // T2 is the storage type of the spectrum (double2, or float2 in the
 // mixed-precision solver); the elimination itself is FP64 either way
+// Two lanes per spectral line: the real and imaginary parts are two
+// systems with the same (real) coefficients (the reference's real-by-complex
+// arithmetic), so lane 2 line + c sweeps component c of the line.  Both
+// lanes form the same factors and both store them (store_v: the same value
+// to the same address, so each lane's back substitution reads its own
+// store).  At 256^3
+// that is 66048 lanes, one wavefront for every SIMD of the chip, where one
+// lane per line left half of them idle.  Per component the operations are
+// those of the one-lane-per-line elimination, in the same order.
 template <class T2, int U = kTsU>
 __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, int nlines, int nh, const double * __restrict__ pl,
                               const double * __restrict__ pu, const double * __restrict__ ey,
                               const double * __restrict__ ez, T2 * __restrict__ spec,
                               double * __restrict__ vbuf, int store_v) {
-  const int line = (int)(vb.x * blockDim.x + threadIdx.x);
+  using S = typename Cx<T2>::S;   // component type of the spectrum
+  const int lane = (int)(vb.x * blockDim.x + threadIdx.x);
+  const int line = lane >> 1, comp = lane & 1;
   if (line >= nlines) return;
   const int ky = line % nh;
 #if NDIMS == 2
@@ -1844,29 +1855,30 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
   const int kz = line / nh;
   const double eval = ey[ky] + ez[kz];
 #endif
-  const size_t st = (size_t)nlines;
-  T2 * q = spec + line;
+  const size_t st = 2 * (size_t)nlines;   // components between rows
+  S * q = reinterpret_cast<S *>(spec) + 2 * (size_t)line + comp;
   double * v = vbuf + line;
+  const size_t sv = (size_t)nlines;
+  const bool sv0 = store_v != 0;
   const int n = nx;
   // row 0 (with the Neumann fold c[0] += l[0])
   double cc = -pl[0] - pu[0] + eval;
   cc += pl[0];
   double vprev = pu[0] / cc;
-  double2 qprev = to_d2(q[0]);
-  qprev.x = qprev.x / cc;
-  qprev.y = qprev.y / cc;
-  q[0] = from_d2<T2>(qprev);
-  if (store_v) v[0] = vprev;
+  double qprev = (double)q[0];
+  qprev = qprev / cc;
+  q[0] = (S)qprev;
+  if (sv0) v[0] = vprev;
   {
     // rows are kept in the storage type until used: a conversion right
     // after a conditional load would wait for that load and serialise them
-    T2 cur[U];
+    S cur[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (1 + u < n - 1) cur[u] = q[(size_t)(1 + u) * st];
     }
     for (int i0 = 1; i0 < n - 1; i0 += U) {
-      T2 nxt[U];
+      S nxt[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int i = i0 + U + u;
@@ -1879,11 +1891,10 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
           const double c = -pl[i] - pu[i] + eval;
           const double val = 1. / (c - pl[i] * vprev);
           const double vi = val * (pu[i]);
-          double2 qi = to_d2(cur[u]);
-          qi.x = val * (qi.x - pl[i] * qprev.x);
-          qi.y = val * (qi.y - pl[i] * qprev.y);
-          q[(size_t)i * st] = from_d2<T2>(qi);
-          if (store_v) v[(size_t)i * st] = vi;
+          double qi = (double)cur[u];
+          qi = val * (qi - pl[i] * qprev);
+          q[(size_t)i * st] = (S)qi;
+          if (sv0) v[(size_t)i * sv] = vi;
           qprev = qi;
           vprev = vi;
         }
@@ -1897,7 +1908,7 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
     double c = -pl[i] - pu[i] + eval;
     c += pu[i];
     const double val = c - pl[i] * vprev;
-    double2 qi = to_d2(q[(size_t)i * st]);
+    double qi = (double)q[(size_t)i * st];
     // the (ky, kz) = (0, 0) line is singular (Neumann in x).  Its right-hand
     // side is compatible to FP64 rounding, and then the guard below picks the
     // gauge; an FP32 spectrum is compatible only to FP32 rounding, which
@@ -1905,45 +1916,42 @@ __device__ __forceinline__ void k_thomas_spec_body(const VBlock & vb, int nx, in
     const bool gauge = sizeof(T2) == sizeof(float2) && eval == 0.;
     if (!gauge && fabs(val) > DBL_EPSILON) {
       const double r = 1. / val;
-      qi.x = r * (qi.x - pl[i] * qprev.x);
-      qi.y = r * (qi.y - pl[i] * qprev.y);
+      qi = r * (qi - pl[i] * qprev);
     } else {
-      qi.x = 0.;
-      qi.y = 0.;
+      qi = 0.;
     }
-    q[(size_t)i * st] = from_d2<T2>(qi);
+    q[(size_t)i * st] = (S)qi;
     qprev = qi;
   }
   {
-    T2 cq[U];
+    S cq[U];
     double cv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = n - 2 - u;
       if (i >= 0) {
         cq[u] = q[(size_t)i * st];
-        cv[u] = v[(size_t)i * st];
+        cv[u] = v[(size_t)i * sv];
       }
     }
     for (int i0 = n - 2; i0 >= 0; i0 -= U) {
-      T2 nq[U];
+      S nq[U];
       double nv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int i = i0 - U - u;
         if (i >= 0) {
           nq[u] = q[(size_t)i * st];
-          nv[u] = v[(size_t)i * st];
+          nv[u] = v[(size_t)i * sv];
         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int i = i0 - u;
         if (i >= 0) {
-          double2 qi = to_d2(cq[u]);
-          qi.x -= cv[u] * qprev.x;
-          qi.y -= cv[u] * qprev.y;
-          q[(size_t)i * st] = from_d2<T2>(qi);
+          double qi = (double)cq[u];
+          qi -= cv[u] * qprev;
+          q[(size_t)i * st] = (S)qi;
           qprev = qi;
         }
       }

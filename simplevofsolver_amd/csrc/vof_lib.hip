@@ -1763,7 +1763,8 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 // per prefetched chunk; the arithmetic does not depend on it)
 template <class T2>
 static int launch_thomas(vof_ctx * ctx, const char * name, T2 * spec, int nlines) {
-  const dim3 g((unsigned)((nlines + 63) / 64)), b(64);
+  // two lanes per line (real and imaginary parts)
+  const dim3 g((unsigned)((2 * (size_t)nlines + 63) / 64)), b(64);
 #define TS_ARGS ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, spec, ctx->d_vbuf, vbuf_store(ctx)
   switch (ctx->thomas_rows) {
     case 16:
@@ -3275,7 +3276,7 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
                              dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y, (const double2 *)ctx->d_twy,
                              (const double *)b, 1. / (double)ny, ctx->d_spec);
         }
-        hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
+        hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((2 * ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
                            ctx->nh, (const double *)ctx->d_pl, (const double *)ctx->d_pu,
                            (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
         if (use1024) {
