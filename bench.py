@@ -562,6 +562,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-sync", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--schedule", type=int, default=None)
+    ap.add_argument("--fusion", type=int, default=None,
+                    help="vof_set_fusion mask of the fused schedule (64: one-kernel predictor)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--poisson-mixed", type=int, default=-1, metavar="K",
                     help="time the mixed-precision form (FP32 transforms + K FP64 refinements) "
@@ -609,6 +611,8 @@ def main(argv=None) -> int:
     print(f"[bench] IC {n}^3 generated in {t_ic:.1f} s", file=sys.stderr, flush=True)
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
         g = sva.solver_from_ic(ic, d, schedule=args.schedule)
+    if args.fusion is not None:
+        g.set_fusion(args.fusion)
     for t in args.tuning:
         name, val = t.split("=")
         g.set_tuning(name, int(val))
