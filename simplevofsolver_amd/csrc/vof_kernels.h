@@ -1330,6 +1330,93 @@ __device__ __forceinline__ void lines_sweep(const LineField & F, double * q) {
   }
 }
 
+// The same by a whole wavefront (the one-kernel predictor, whose waves have
+// nothing else to do meanwhile): lane l < nl eliminates line l (rows at
+// q0 + l ld); the other lanes follow line 0 without storing.  The factors
+// L, VAL, V of 64 consecutive rows are loaded one row per lane (a coalesced
+// vector load, issued one chunk ahead) and each row's pair is broadcast with
+// v_readlane, so that neither scalar loads (whose out-of-order return makes
+// each use wait for every outstanding LDS read) nor their latency sit on the
+// recurrence; the right-hand sides of the next 8 rows are read from LDS
+// before the current 8 rows' steps.  Per line the operations are gtsv's:
+// q[0] / c0, then q[i] = VAL[i] (q[i] - L[i] q[i-1]), the last row
+// rlast (q - L q) (0 when the last pivot is singular), back substitution
+// q[i] -= V[i] q[i+1].
+__device__ __forceinline__ double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ void lines_sweep_wave(const double * tab, int n, double * q0, int ld, int nl,
+                                                 int lane) {
+  constexpr int C = 8;
+  const bool act = lane < nl;
+  double * q = q0 + (size_t)(act ? lane : 0) * ld;
+  const double c0 = tab[0], rlast = tab[1];
+  const bool flag = tab[2] != 0.;
+  const double * L = tab + 4;
+  const double * VAL = tab + 4 + n;
+  const double * V = tab + 4 + 2 * n;
+  const int last = n - 1;
+  double prev = q[0] / c0;
+  if (act) q[0] = prev;
+  // forward elimination, rows 1 .. last-1 in chunks of 64 (one row per lane)
+  {
+    double tl = L[min(1 + lane, last)], tv = VAL[min(1 + lane, last)];
+    for (int c = 1; c < last; c += 64) {
+      const int nxt = min(c + 64 + lane, last);
+      const double nl_ = L[nxt], nv_ = VAL[nxt];
+      const int m = min(64, last - c);
+      int u = 0;
+      for (; u + C <= m; u += C) {
+        double x[C];
+#pragma unroll
+        for (int e = 0; e < C; e++) x[e] = q[c + u + e];
+#pragma unroll
+        for (int e = 0; e < C; e++) {
+          prev = bcast(tv, u + e) * (x[e] - bcast(tl, u + e) * prev);
+          if (act) q[c + u + e] = prev;
+        }
+      }
+      for (; u < m; u++) {
+        prev = bcast(tv, u) * (q[c + u] - bcast(tl, u) * prev);
+        if (act) q[c + u] = prev;
+      }
+      tl = nl_;
+      tv = nv_;
+    }
+  }
+  prev = flag ? rlast * (q[last] - L[last] * prev) : 0.;
+  if (act) q[last] = prev;
+  // back substitution, rows last-1 .. 0 in descending chunks of 64
+  {
+    double tw = V[max(last - 1 - lane, 0)];
+    for (int c = last - 1; c >= 0; c -= 64) {
+      const int nxt = max(c - 64 - lane, 0);
+      const double nw_ = V[nxt];
+      const int m = min(64, c + 1);
+      int u = 0;
+      for (; u + C <= m; u += C) {
+        double x[C];
+#pragma unroll
+        for (int e = 0; e < C; e++) x[e] = q[c - u - e];
+#pragma unroll
+        for (int e = 0; e < C; e++) {
+          prev = x[e] - bcast(tw, u + e) * prev;
+          if (act) q[c - u - e] = prev;
+        }
+      }
+      for (; u < m; u++) {
+        prev = q[c - u] - bcast(tw, u) * prev;
+        if (act) q[c - u] = prev;
+      }
+      tw = nw_;
+    }
+  }
+}
+
 // phase 3: field rows += solution by wavefronts w = 0..nw-1 (lane = lane
 // index).  Wavefront w owns lines w, w + nw, ... (two per pass), so a
 // line's row / plane, its address and whether it has periodic halo images
@@ -3537,12 +3624,9 @@ __device__ __forceinline__ void k_pred_lines_body(const VBlock & vb, Grid g, con
   // 2. one wavefront per field, one lane per row
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int lane = (int)threadIdx.x & 63;
-  if (w < kPNF && lane < nrows) {
+  if (w < kPNF) {
     const int kind = pred_kind(w);
-    LineField F{};
-    F.tab = pick4(P.tab, kind);
-    F.n = kind == 0 ? nx - 1 : nx;
-    lines_sweep(F, S + (size_t)w * fs + (size_t)lane * ld);
+    lines_sweep_wave(pick4(P.tab, kind), kind == 0 ? nx - 1 : nx, S + (size_t)w * fs, ld, nrows, lane);
   }
   __syncthreads();
   // 3b. new field values, the x walls (constants the predictor never
