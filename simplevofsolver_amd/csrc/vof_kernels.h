@@ -1049,6 +1049,13 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
 // The tri-diagonal matrix (I - prefactor Lx) is the same for every line, so
 // the elimination factors the reference recomputes per line are prepared
 // once on the host with the same arithmetic.
+// Streaming store of a stage output: no other work-group of the writing
+// kernel reads it, and a 256^3 array (134 MB) outlives the 4 MB L2 of an XCD
+// before the next kernel reads it, so the line is not kept in L2, where it
+// would evict the neighbour planes the stencil kernels re-read (k_rhs_march
+// 704 -> 672 us at 256^3).  The value stored is the same.
+__device__ __forceinline__ void st_stream(double * p, double v) { __builtin_nontemporal_store(v, p); }
+
 // Store v at padded position (x, row r, plane p) and at every periodic image
 // of it inside the halo (h rows / planes on each side; r, p are storage
 // indices of an interior entry).  In 2D the plane index is ignored.
@@ -1066,10 +1073,10 @@ __device__ inline void store_periodic(double * __restrict__ a, int x, int n0, in
   if (p < 2 * h) ps[np++] = p + nz;
   if (p >= nz) ps[np++] = p - nz;
   for (int b = 0; b < np; b++)
-    for (int c = 0; c < nr; c++) a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])] = v;
+    for (int c = 0; c < nr; c++) st_stream(&a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])], v);
 #else
   (void)p; (void)nz; (void)n1;
-  for (int c = 0; c < nr; c++) a[(size_t)x + (size_t)n0 * (size_t)rs[c]] = v;
+  for (int c = 0; c < nr; c++) st_stream(&a[(size_t)x + (size_t)n0 * (size_t)rs[c]], v);
 #endif
 }
 
@@ -1089,10 +1096,10 @@ __device__ inline void store_images(double * __restrict__ a, int x, int n0, int 
   if (p >= nz) ps[np++] = p - nz;
   for (int b = 0; b < np; b++)
     for (int c = 0; c < nr; c++)
-      if (b != 0 || c != 0) a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])] = v;
+      if (b != 0 || c != 0) st_stream(&a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])], v);
 #else
   (void)p; (void)nz; (void)n1;
-  for (int c = 1; c < nr; c++) a[(size_t)x + (size_t)n0 * (size_t)rs[c]] = v;
+  for (int c = 1; c < nr; c++) st_stream(&a[(size_t)x + (size_t)n0 * (size_t)rs[c]], v);
 #endif
 }
 
@@ -1449,8 +1456,8 @@ __device__ __forceinline__ void lines_update(const LineField & F, const LineJob 
         for (int u = 0; u < kWUnroll; u++) {
           const int m = m0 + 64 * u + lane;
           if (m < n) {
-            ra[m] = va[u];
-            if (hb) rb[m] = vb[u];
+            st_stream(&ra[m], va[u]);
+            if (hb) st_stream(&rb[m], vb[u]);
           }
         }
       }
@@ -3190,7 +3197,7 @@ __device__ __forceinline__ void k_vof_fused_fz_body(const VBlock & vb, Grid g, c
   const double fzp = face_flux<2>(uzp, vc, vzp, nc, normal + NC * I_NRM(i, j, k + 1));
   s += g.rdz * (fzm - fzp);
 #endif
-  if (keep_a) srca[is] = s;
+  if (keep_a) st_stream(&srca[is], s);
   double v = vc;
   v += dta * s;
   if (use_b) v += dtb * sb;
@@ -3443,12 +3450,12 @@ __device__ __forceinline__ void rhs_rows(const Grid & g, const FusedRhs & f, int
     const double ca = b.ca, cb = b.cb, cg = b.cg;
     const bool ka = f.keep_a;
     if (ka) {
-      if (has_ux) f.srca[0][ix] = sa0;
-      f.srca[1][is] = sa1;
+      if (has_ux) st_stream(&f.srca[0][ix], sa0);
+      st_stream(&f.srca[1][is], sa1);
 #if NDIMS == 3
-      f.srca[2][is] = sa2;
+      st_stream(&f.srca[2][is], sa2);
 #endif
-      f.srca[3][is] = sa3;
+      st_stream(&f.srca[3][is], sa3);
     }
     sink(r, is, ix, +ca * sa0 + cb * sb0 + cg * sg0, +ca * sa1 + cb * sb1 + cg * sg1,
          +ca * sa2 + cb * sb2 + cg * sg2, +ca * sa3 + cb * sb3 + cg * sg3);
@@ -3484,14 +3491,14 @@ __device__ __forceinline__ void k_rhs_march_body(const VBlock & vb, Grid g, Fuse
   const int nr = min(R, ny + 1 - jb);
   const bool has_ux = i >= 2;
   rhs_rows<R>(g, f, i, jb, k, nr, [&](int, size_t is, size_t ix, double d0, double d1, double d2, double d3) {
-    if (has_ux) f.delta[0][ix] = d0;
-    f.delta[1][is] = d1;
+    if (has_ux) st_stream(&f.delta[0][ix], d0);
+    st_stream(&f.delta[1][is], d1);
 #if NDIMS == 3
-    f.delta[2][is] = d2;
+    st_stream(&f.delta[2][is], d2);
 #else
     (void)d2;
 #endif
-    f.delta[3][is] = d3;
+    st_stream(&f.delta[3][is], d3);
   });
 }
 
