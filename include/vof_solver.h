@@ -402,7 +402,11 @@ int vof_batch_autotune(vof_batch_t * batch, int steps, char * report, size_t len
  * its x solve, Poisson solve and correction), 2 from the start of its
  * Thomas pass, 0 on one stream (default: 1 from 2^20 cells on, else 0);
  * "thomas_rows" = 8 (default), 16 or 32 rows of the spectrum prefetched per
- * lane by the Poisson solver's x eliminations;
+ * lane by the Poisson solver's sequential x eliminations;
+ * "thomas_part" = 1 (default) solves the Poisson solver's x systems by
+ * partitions (segments of 2 .. 8 rows, one lane each, and a reduced system
+ * of the segments' end rows) where 4 <= nx <= 256, 0 in sequence (this one
+ * changes results by rounding only, a few 1e-14 relative to each line);
  * "fft_radix" = 16 (default: 256-point transforms on the register
  * radix-16 path) or 4 (every length on the mixed-radix 4/2/3/5 path; this
  * one changes results by rounding only, about 1e-16 relative);

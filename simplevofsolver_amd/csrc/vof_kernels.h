@@ -1053,12 +1053,25 @@ __global__ void k_rhs_t(Grid g, RhsArgs a) {
 // kernel reads it, and a 256^3 array (134 MB) outlives the 4 MB L2 of an XCD
 // before the next kernel reads it, so the line is not kept in L2, where it
 // would evict the neighbour planes the stencil kernels re-read (k_rhs_march
-// 704 -> 672 us at 256^3).  The value stored is the same.
+// 704 -> 672 us at 256^3).  The value stored is the same.  Only for stores
+// that fill whole 128-byte lines across a wavefront: lines written in parts
+// by different wavefronts are merged in L2 by plain stores (the inverse y
+// transform's psi columns: 97 us with plain stores, 121 us streaming).
 __device__ __forceinline__ void st_stream(double * p, double v) { __builtin_nontemporal_store(v, p); }
+
+template <bool NT>
+__device__ __forceinline__ void st_maybe(double * p, double v) {
+  if (NT)
+    st_stream(p, v);
+  else
+    *p = v;
+}
 
 // Store v at padded position (x, row r, plane p) and at every periodic image
 // of it inside the halo (h rows / planes on each side; r, p are storage
-// indices of an interior entry).  In 2D the plane index is ignored.
+// indices of an interior entry).  In 2D the plane index is ignored.  NT:
+// streaming stores (st_stream).
+template <bool NT = true>
 __device__ inline void store_periodic(double * __restrict__ a, int x, int n0, int n1, int r, int p,
                                       int ny, int nz, int h, double v) {
   int rs[3];
@@ -1073,10 +1086,10 @@ __device__ inline void store_periodic(double * __restrict__ a, int x, int n0, in
   if (p < 2 * h) ps[np++] = p + nz;
   if (p >= nz) ps[np++] = p - nz;
   for (int b = 0; b < np; b++)
-    for (int c = 0; c < nr; c++) st_stream(&a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])], v);
+    for (int c = 0; c < nr; c++) st_maybe<NT>(&a[(size_t)x + (size_t)n0 * ((size_t)rs[c] + (size_t)n1 * (size_t)ps[b])], v);
 #else
   (void)p; (void)nz; (void)n1;
-  for (int c = 0; c < nr; c++) st_stream(&a[(size_t)x + (size_t)n0 * (size_t)rs[c]], v);
+  for (int c = 0; c < nr; c++) st_maybe<NT>(&a[(size_t)x + (size_t)n0 * (size_t)rs[c]], v);
 #endif
 }
 
@@ -2066,6 +2079,358 @@ __global__ __launch_bounds__(64) void k_thomas_spec(int nx, int nlines, int nh, 
   k_thomas_spec_body<T2, U>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf, store_v);
 }
 
+// The elimination of k_thomas_spec for one whole line, both components by
+// one lane (per component the operations and their order are those of
+// k_thomas_spec with store_v; the gauge choice of its last row included).
+template <class T2>
+__device__ void thomas_line_seq(int n, int nlines, int line, double eval, const double * __restrict__ pl,
+                                const double * __restrict__ pu, T2 * __restrict__ spec, double * __restrict__ vbuf) {
+  T2 * q = spec + line;
+  double * v = vbuf + line;
+  const size_t st = (size_t)nlines;
+  double cc = -pl[0] - pu[0] + eval;
+  cc += pl[0];
+  double vprev = pu[0] / cc;
+  double2 qprev = to_d2(q[0]);
+  qprev.x = qprev.x / cc;
+  qprev.y = qprev.y / cc;
+  q[0] = from_d2<T2>(qprev);
+  v[0] = vprev;
+  for (int i = 1; i < n - 1; i++) {
+    const double c = -pl[i] - pu[i] + eval;
+    const double val = 1. / (c - pl[i] * vprev);
+    const double vi = val * (pu[i]);
+    double2 qi = to_d2(q[(size_t)i * st]);
+    qi.x = val * (qi.x - pl[i] * qprev.x);
+    qi.y = val * (qi.y - pl[i] * qprev.y);
+    q[(size_t)i * st] = from_d2<T2>(qi);
+    v[(size_t)i * st] = vi;
+    qprev = qi;
+    vprev = vi;
+  }
+  {
+    const int i = n - 1;
+    double c = -pl[i] - pu[i] + eval;
+    c += pu[i];
+    const double val = c - pl[i] * vprev;
+    double2 qi = to_d2(q[(size_t)i * st]);
+    const bool gauge = sizeof(T2) == sizeof(float2) && eval == 0.;
+    if (!gauge && fabs(val) > DBL_EPSILON) {
+      const double r = 1. / val;
+      qi.x = r * (qi.x - pl[i] * qprev.x);
+      qi.y = r * (qi.y - pl[i] * qprev.y);
+    } else {
+      qi = make_double2(0., 0.);
+    }
+    q[(size_t)i * st] = from_d2<T2>(qi);
+    qprev = qi;
+  }
+  for (int i = n - 2; i >= 0; i--) {
+    double2 qi = to_d2(q[(size_t)i * st]);
+    const double vi = v[(size_t)i * st];
+    qi.x -= vi * qprev.x;
+    qi.y -= vi * qprev.y;
+    q[(size_t)i * st] = from_d2<T2>(qi);
+    qprev = qi;
+  }
+}
+
+// thomas_line_seq on a line staged in LDS (ql: the rows, vl: n factors)
+template <class T2>
+__device__ void thomas_line_lds(int n, double eval, const double * __restrict__ pl, const double * __restrict__ pu,
+                                double2 * ql, double * vl) {
+  double cc = -pl[0] - pu[0] + eval;
+  cc += pl[0];
+  double vprev = pu[0] / cc;
+  double2 qprev = ql[0];
+  qprev.x = qprev.x / cc;
+  qprev.y = qprev.y / cc;
+  ql[0] = to_d2(from_d2<T2>(qprev));
+  vl[0] = vprev;
+  for (int i = 1; i < n - 1; i++) {
+    const double c = -pl[i] - pu[i] + eval;
+    const double val = 1. / (c - pl[i] * vprev);
+    const double vi = val * (pu[i]);
+    double2 qi = ql[i];
+    qi.x = val * (qi.x - pl[i] * qprev.x);
+    qi.y = val * (qi.y - pl[i] * qprev.y);
+    ql[i] = to_d2(from_d2<T2>(qi));
+    vl[i] = vi;
+    qprev = qi;
+    vprev = vi;
+  }
+  {
+    const int i = n - 1;
+    double c = -pl[i] - pu[i] + eval;
+    c += pu[i];
+    const double val = c - pl[i] * vprev;
+    double2 qi = ql[i];
+    const bool gauge = sizeof(T2) == sizeof(float2) && eval == 0.;
+    if (!gauge && fabs(val) > DBL_EPSILON) {
+      const double r = 1. / val;
+      qi.x = r * (qi.x - pl[i] * qprev.x);
+      qi.y = r * (qi.y - pl[i] * qprev.y);
+    } else {
+      qi = make_double2(0., 0.);
+    }
+    ql[i] = to_d2(from_d2<T2>(qi));
+    qprev = qi;
+  }
+  for (int i = n - 2; i >= 0; i--) {
+    double2 qi = ql[i];
+    const double vi = vl[i];
+    qi.x -= vi * qprev.x;
+    qi.y -= vi * qprev.y;
+    ql[i] = to_d2(from_d2<T2>(qi));
+    qprev = qi;
+  }
+}
+
+// Partitioned x elimination of the spectrum: the tri-diagonal systems of
+// k_thomas_spec (solve_linear_systems, dft.c:364-395: c = -l - u + lambda,
+// Neumann folds in the first and last rows), solved in another order.  Each
+// line is cut into P = min(32, nx / 2) segments of 2 .. 8 rows (nx <= 256),
+// one lane per (line, segment), 8 lines per work-group of 256 lanes:
+//  1. a lane loads its segment's rows (both components) into registers and
+//     eliminates inside the segment, forward and then backward, so that each
+//     interior row reads x_i = d_i - a_i x_s - c_i x_e in the segment's first
+//     and last unknowns x_s, x_e;
+//  2. the first and last rows of the P segments of a line form a
+//     tri-diagonal system in those 2P unknowns; the lanes write it to LDS and
+//     solve it there by cyclic reduction (log2 2P steps, 2 rows per lane);
+//  3. every lane forms its interior rows from x_s, x_e and stores its rows.
+// The spectrum crosses HBM once each way, where the sequential elimination
+// also stores and re-reads the modified right-hand side and reads its
+// factors.  All systems but one are strictly diagonally dominant (lambda < 0),
+// so no pivoting is needed in either elimination; the results differ from
+// the sequential order's by rounding (a few 1e-14 of the line's largest
+// entry).  The (ky, kz) = (0, 0) line (lambda = 0) is singular, Neumann in
+// x; it keeps the sequential elimination and its choice of the constant: its
+// segment lanes stage the line in LDS, where one lane eliminates it
+// (thomas_line_lds) while the other lines solve their reduced systems.  Only
+// line 0 has lambda = 0 (every other eigenvalue is negative), and it is line
+// t = 0 of the first work-group; a singular line elsewhere would take
+// thomas_line_seq on global memory.
+constexpr int kTpLines = 8;    // lines per work-group
+constexpr int kTpSeg = 32;     // segment lanes per line
+constexpr int kTpRows = 8;     // rows of the longest segment (nx <= 256)
+
+template <class T2>
+__device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, int nlines, int nh,
+                                                   const double * __restrict__ pl, const double * __restrict__ pu,
+                                                   const double * __restrict__ ey, const double * __restrict__ ez,
+                                                   T2 * __restrict__ spec, double * __restrict__ vbuf) {
+  // reduced rows 2 seg (first row of the segment) and 2 seg + 1 (last row):
+  // lower, diagonal, upper, right-hand side (re, im); the solution replaces
+  // the right-hand side
+  __shared__ double red[5][2 * kTpSeg][kTpLines];
+  __shared__ double2 sline[kTpSeg * kTpRows];   // the staged singular line
+  __shared__ double sfac[kTpSeg * kTpRows];
+  const int tid = (int)threadIdx.x;
+  const int t = tid % kTpLines, seg = tid / kTpLines;   // 8 consecutive lines per segment: 128-byte rows
+  const int line = (int)vb.x * kTpLines + t;
+  const int n = nx;
+  const int P = min(kTpSeg, n / 2);
+  const bool has_line = line < nlines;
+  double eval = 0.;
+  if (has_line) {
+#if NDIMS == 2
+    eval = ey[line % nh];
+    (void)ez;
+#else
+    eval = ey[line % nh] + ez[line / nh];
+#endif
+  }
+  const bool sing = eval == 0.;
+  const bool stage = has_line && sing && t == 0 && seg < P;   // rows of the staged singular line
+  const bool act = has_line && !sing && seg < P;
+  const int s = seg * n / P, e = (seg + 1) * n / P - 1;
+  const int m = e - s + 1;
+  const size_t st = (size_t)nlines;
+  T2 * __restrict__ q = spec + line;
+  // row i of the line: lower, diagonal and upper coefficients (the folds
+  // as in k_thomas_spec; no lower entry in row 0, no upper in row n - 1)
+  auto coef = [&](int i, double & lo, double & dg, double & up) {
+    lo = pl[i];
+    up = pu[i];
+    dg = -lo - up + eval;
+    if (i == 0) {
+      dg += lo;
+      lo = 0.;
+    }
+    if (i == n - 1) {
+      dg += up;
+      up = 0.;
+    }
+  };
+  double A[kTpRows], C[kTpRows];
+  double2 D[kTpRows];
+  double2 xs = make_double2(0., 0.), xe = make_double2(0., 0.);
+  if (stage) {
+#pragma unroll
+    for (int r = 0; r < kTpRows; r++)
+      if (r < m) sline[s + r] = to_d2(q[(size_t)(s + r) * st]);
+  }
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < kTpRows; r++)
+      if (r < m) D[r] = to_d2(q[(size_t)(s + r) * st]);
+    // forward: row r as a_r x_s + x_r + c_r x_{r+1} = d_r
+    {
+      double lo, dg, up;
+      coef(s + 1, lo, dg, up);
+      const double rc = 1. / dg;
+      A[1] = lo * rc;
+      C[1] = up * rc;
+      D[1].x = D[1].x * rc;
+      D[1].y = D[1].y * rc;
+    }
+#pragma unroll
+    for (int r = 2; r < kTpRows; r++) {
+      if (r < m) {
+        double lo, dg, up;
+        coef(s + r, lo, dg, up);
+        const double rc = 1. / (dg - lo * C[r - 1]);
+        A[r] = -lo * A[r - 1] * rc;
+        C[r] = up * rc;
+        D[r].x = (D[r].x - lo * D[r - 1].x) * rc;
+        D[r].y = (D[r].y - lo * D[r - 1].y) * rc;
+      }
+    }
+    // the last row, as it stands, is a reduced row
+    double ae = A[1], ce = C[1];
+    double2 de = D[1];
+#pragma unroll
+    for (int r = 2; r < kTpRows; r++) {
+      if (r == m - 1) {
+        ae = A[r];
+        ce = C[r];
+        de = D[r];
+      }
+    }
+    // backward: interior rows as x_r = d_r - a_r x_s - c_r x_e
+#pragma unroll
+    for (int r = kTpRows - 3; r >= 1; r--) {
+      if (r <= m - 3) {
+        A[r] = A[r] - C[r] * A[r + 1];
+        D[r].x = D[r].x - C[r] * D[r + 1].x;
+        D[r].y = D[r].y - C[r] * D[r + 1].y;
+        C[r] = -C[r] * C[r + 1];
+      }
+    }
+    // the first row with x_{s+1} eliminated
+    double lo, dg, up;
+    coef(s, lo, dg, up);
+    double d1 = dg, u1 = up;
+    double2 r1 = D[0];
+    if (m >= 3) {
+      d1 = dg - up * A[1];
+      u1 = -up * C[1];
+      r1.x = D[0].x - up * D[1].x;
+      r1.y = D[0].y - up * D[1].y;
+    }
+    red[0][2 * seg][t] = lo;
+    red[1][2 * seg][t] = d1;
+    red[2][2 * seg][t] = u1;
+    red[3][2 * seg][t] = r1.x;
+    red[4][2 * seg][t] = r1.y;
+    red[0][2 * seg + 1][t] = ae;
+    red[1][2 * seg + 1][t] = 1.;
+    red[2][2 * seg + 1][t] = ce;
+    red[3][2 * seg + 1][t] = de.x;
+    red[4][2 * seg + 1][t] = de.y;
+  }
+  __syncthreads();
+  if (tid == 0 && has_line && sing) thomas_line_lds<T2>(n, eval, pl, pu, sline, sfac);
+  // cyclic reduction of the reduced systems (rows seg and seg + 32 of line t
+  // per lane): step h combines row j with rows j - h and j + h so that it
+  // couples j - 2h, j, j + 2h; after ceil(log2 2P) steps every row stands
+  // alone.  The rows are diagonally dominant as the segments' rows are.
+  const int nr = 2 * P;
+  const bool red_on = has_line && !sing;
+  for (int h = 1; h < nr; h *= 2) {
+    double nw[2][5];
+#pragma unroll
+    for (int k2 = 0; k2 < 2; k2++) {
+      const int j = seg + kTpSeg * k2;
+      if (red_on && j < nr) {
+        const double lo = red[0][j][t], up = red[2][j][t];
+        double nlo = 0., nup = 0., ndg = red[1][j][t], nre = red[3][j][t], nim = red[4][j][t];
+        if (j - h >= 0) {
+          const double al = -lo / red[1][j - h][t];
+          nlo = al * red[0][j - h][t];
+          ndg += al * red[2][j - h][t];
+          nre += al * red[3][j - h][t];
+          nim += al * red[4][j - h][t];
+        }
+        if (j + h < nr) {
+          const double ga = -up / red[1][j + h][t];
+          nup = ga * red[2][j + h][t];
+          ndg += ga * red[0][j + h][t];
+          nre += ga * red[3][j + h][t];
+          nim += ga * red[4][j + h][t];
+        }
+        nw[k2][0] = nlo;
+        nw[k2][1] = ndg;
+        nw[k2][2] = nup;
+        nw[k2][3] = nre;
+        nw[k2][4] = nim;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 2; k2++) {
+      const int j = seg + kTpSeg * k2;
+      if (red_on && j < nr) {
+#pragma unroll
+        for (int c = 0; c < 5; c++) red[c][j][t] = nw[k2][c];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 2; k2++) {
+    const int j = seg + kTpSeg * k2;
+    if (red_on && j < nr) {
+      const double dg = red[1][j][t];
+      red[3][j][t] = red[3][j][t] / dg;
+      red[4][j][t] = red[4][j][t] / dg;
+    }
+  }
+  __syncthreads();
+  if (act) {
+    xs = make_double2(red[3][2 * seg][t], red[4][2 * seg][t]);
+    xe = make_double2(red[3][2 * seg + 1][t], red[4][2 * seg + 1][t]);
+    q[(size_t)s * st] = from_d2<T2>(xs);
+#pragma unroll
+    for (int r = 1; r < kTpRows - 1; r++) {
+      if (r < m - 1) {
+        double2 x;
+        x.x = D[r].x - A[r] * xs.x - C[r] * xe.x;
+        x.y = D[r].y - A[r] * xs.y - C[r] * xe.y;
+        q[(size_t)(s + r) * st] = from_d2<T2>(x);
+      }
+    }
+    q[(size_t)e * st] = from_d2<T2>(xe);
+  }
+  if (stage) {
+#pragma unroll
+    for (int r = 0; r < kTpRows; r++)
+      if (r < m) q[(size_t)(s + r) * st] = from_d2<T2>(sline[s + r]);
+  }
+  if (has_line && sing && t != 0 && seg == 0) thomas_line_seq<T2>(n, nlines, line, eval, pl, pu, spec, vbuf);
+}
+
+template <class T2>
+__global__ __launch_bounds__(kTpLines * kTpSeg) void k_thomas_part(int nx, int nlines, int nh,
+                                                                    const double * __restrict__ pl,
+                                                                    const double * __restrict__ pu,
+                                                                    const double * __restrict__ ey,
+                                                                    const double * __restrict__ ez,
+                                                                    T2 * __restrict__ spec, double * __restrict__ vbuf) {
+  k_thomas_part_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf);
+}
+
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and
 // k = n/2 bins do not contribute) and extract_output (dft.c:347-362)
 template <int ELEMS, bool F256 = false>
@@ -2128,7 +2493,7 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
       if (images) {
         // fluid_update_boundaries_psi folded in: halo images, Neumann walls
         const int pl = NDIMS == 3 ? k : 0;
-        store_periodic(psi, i, nx + 2, ny + 2, j + 1, pl, ny, nz, 1, v);
+        store_periodic<false>(psi, i, nx + 2, ny + 2, j + 1, pl, ny, nz, 1, v);
         neumann_walls(psi, i, nx, ny + 2, j + 1, pl, ny, nz, v);
       } else {
         psi[I_C1(i, j + 1, k)] = v;
@@ -2343,7 +2708,7 @@ __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb,
       double v = (double)((xl & 1) ? z.y : z.x);
       if (accumulate) v = psi[I_C1(i, j + 1, k)] + v;
       const int pln = NDIMS == 3 ? k : 0;
-      store_periodic(psi, i, nx + 2, ny + 2, j + 1, pln, ny, nz, 1, v);
+      store_periodic<false>(psi, i, nx + 2, ny + 2, j + 1, pln, ny, nz, 1, v);
       neumann_walls(psi, i, nx, ny + 2, j + 1, pln, ny, nz, v);
     }
   }

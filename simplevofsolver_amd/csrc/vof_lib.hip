@@ -231,6 +231,7 @@ struct vof_ctx {
   int iface_planes;                      // z planes marched per work-group by the 3D interface kernel (0 = blocks)
   int fft_radix;                         // 16: 256-point transforms on the register radix-16 path; 4: mixed radix
   int thomas_rows;                       // rows per prefetched chunk of the spectral Thomas kernel (8, 16, 32)
+  int thomas_part;                       // 1: partitioned spectral x solve (k_thomas_part) where 4 <= nx <= 256
   int iface_overlap;                     // 1: the next stage's interface kernel runs on `side` (see integrate)
   hipStream_t side;                      // second stream of the fused schedule (default priority)
   hipEvent_t ev_rhs, ev_iface, ev_main;  // side launch point (main); interface kernel done (side); main so far
@@ -952,6 +953,7 @@ static int ctx_create(domain_t * domain) {
   ctx->fft_elems = 1024;
   ctx->fft_radix = 16;
   ctx->thomas_rows = 8;
+  ctx->thomas_part = 1;
   // two streams pay from about 2^20 cells; below, the launches of a step
   // are short and the cross-stream events cost more than the overlap gains
   ctx->iface_overlap = (size_t)ctx->nx * (size_t)ctx->ny * (size_t)ctx->nz >= ((size_t)1 << 20) ? 1 : 0;
@@ -1759,10 +1761,18 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 // mixed-precision solve: FP32 transforms, FP64 eliminations, and
 // `poisson_refine` FP64 residual corrections; psi always carries its halo
 // images and Neumann walls (the residual reads them)
-// x eliminations of the spectrum (k_thomas_spec with ctx->thomas_rows rows
-// per prefetched chunk; the arithmetic does not depend on it)
+// x eliminations of the spectrum: the partitioned solve (k_thomas_part, one
+// lane per segment of 2 .. 8 rows) for 4 <= nx <= 256 unless thomas_part is
+// 0, else k_thomas_spec with ctx->thomas_rows rows per prefetched chunk (the
+// arithmetic does not depend on it)
 template <class T2>
 static int launch_thomas(vof_ctx * ctx, const char * name, T2 * spec, int nlines) {
+  if (ctx->thomas_part && ctx->nx >= 4 && ctx->nx <= 256) {
+    const dim3 g((unsigned)((nlines + vofk::kTpLines - 1) / vofk::kTpLines)), b(vofk::kTpLines * vofk::kTpSeg);
+    KLB(ctx, name, vofk::kTpLines * vofk::kTpSeg, (vofk::k_thomas_part_body<T2>), (vofk::k_thomas_part<T2>), g, b, 0,
+        ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, spec, ctx->d_vbuf);
+    return 0;
+  }
   // two lanes per line (real and imaginary parts)
   const dim3 g((unsigned)((2 * (size_t)nlines + 63) / 64)), b(64);
 #define TS_ARGS ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, spec, ctx->d_vbuf, vbuf_store(ctx)
@@ -2377,6 +2387,11 @@ extern "C" int vof_set_tuning(domain_t * domain, const char name[], int value) {
   if (!strcmp(name, "thomas_rows")) {
     if (value != 8 && value != 16 && value != 32) return fail("thomas_rows must be 8, 16 or 32");
     ctx->thomas_rows = value;
+    return 0;
+  }
+  if (!strcmp(name, "thomas_part")) {
+    if (value != 0 && value != 1) return fail("thomas_part must be 0 or 1");
+    ctx->thomas_part = value;
     return 0;
   }
   if (!strcmp(name, "fft_radix")) {

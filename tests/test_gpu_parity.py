@@ -201,6 +201,10 @@ def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
         # parameters of the two-kernel predictor
         g0.set_fusion(63)
         g1.set_fusion(63)
+    if tuning[0] == "thomas_rows":
+        # a parameter of the sequential x elimination of the Poisson solver
+        g0.set_tuning("thomas_part", 0)
+        g1.set_tuning("thomas_part", 0)
     g1.set_tuning(*tuning)
     nd = len(glsizes)
     try:
@@ -211,6 +215,50 @@ def test_tile_parameters_do_not_change_results(case, tuning, tmp_path):
     finally:
         g0.close()
         g1.close()
+
+
+PART_CASES = [
+    ((256, 64), False, 31),
+    ((37, 24), True, 32),
+    ((37, 24), False, 33),
+    ((5, 16), False, 34),
+    ((100, 16, 12), False, 35),
+    ((256, 8, 16), True, 36),
+]
+
+
+@pytest.mark.parametrize("case", PART_CASES, ids=[_ids(c) for c in PART_CASES])
+def test_partitioned_x_solve(case, tmp_path):
+    """The Poisson solver's partitioned x elimination (k_thomas_part: segments
+    of 2 .. 8 rows, one lane each, and a reduced system of their end rows; the
+    singular (0, 0) line in sequence) against the sequential elimination and
+    the oracle: psi within 1e-12 modulo its mean, for nx from 5 (two segments
+    of 2 and 3 rows) to 256 (32 segments of 8), uneven segment lengths
+    (nx = 37, 100) and stretched x; then a full step of each within 1e-10."""
+    import simplevofsolver_amd as sva
+    glsizes, uniformx, seed = case
+    ic = random_fields(glsizes, seed=seed, uniformx=uniformx)
+    dxf = np.diff(ic.xf)
+    o, g = make_pair(ic, tmp_path)
+    s = sva.solver_from_ic(ic, str(tmp_path / "seq"))
+    s.set_tuning("thomas_part", 0)
+    try:
+        dt = o.decide_dt()
+        o.potential(0, dt)
+        g.potential(0, dt)
+        s.potential(0, dt)
+        m = read_mask("psi", glsizes)
+        ref = demean("psi", o.get("psi"), glsizes, dxf)[m]
+        part = demean("psi", g.get("psi"), glsizes, dxf)[m]
+        seq = demean("psi", s.get("psi"), glsizes, dxf)[m]
+        assert rel(part, seq) < 1e-12, rel(part, seq)
+        assert rel(part, ref) < 1e-12, rel(part, ref)
+        assert g.integrate() == s.integrate()
+        for n in ["ux", "uy"] + (["uz"] if len(glsizes) == 3 else []):
+            assert rel(g.get(n), s.get(n)) < 1e-10, n
+    finally:
+        g.close()
+        s.close()
 
 
 @pytest.mark.parametrize("uniformx", [True, False])
