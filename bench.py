@@ -234,6 +234,19 @@ def drift_2d(sva, steps, oracle_steps=100):
         g.close()
 
 
+def plain_rate(g, cells, steps):
+    """Mcell-steps/s of the plain FP64 step from the solver's current state
+    (the forms below are timed after the drift run, so they are compared
+    with the plain step at that same state)."""
+    g.integrate()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.integrate()
+    g.synchronize()
+    return cells * steps / (time.perf_counter() - t0) / 1e6
+
+
 def mixed_form(g, cells, steps, plain_value, refine):
     """The same solver with the FP32-transform Poisson solve followed by
     `refine` FP64 residual refinements, timed over `steps` steps."""
@@ -439,6 +452,8 @@ def batched_form(sva, ncases, steps, autotune=2):
                 b.integrate()
             b.synchronize()
             el32 = time.perf_counter() - t0
+            for s in solvers:
+                s.set_poisson_mixed(-1)
         finally:
             b.close()
     finally:
@@ -451,6 +466,7 @@ def batched_form(sva, ncases, steps, autotune=2):
     return {"value": round(v, 3), "unit": "Mcell-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
             "sequential_value": round(seq, 3), "vs_sequential": round(v / seq, 3), "cases": ncases,
             "cells": cells, "steps": steps, "autotuned": tuned,
+            "parts": 4 if ncases >= 16 else 2 if ncases >= 8 else 1,
             "fp32_poisson": {"value": round(v32, 3), "ms_per_step": round(el32 / steps * 1e3, 4),
                              "vs_fp64_batch": round(v32 / v, 4), "refinements": 1},
             "workload": f"{ncases} 2D cases, nx in {{64,96,128,192,256}}, ny = 2 nx, 1-4 droplets, "
@@ -667,15 +683,24 @@ def main(argv=None) -> int:
     copy_gbs = g.copy_probe(1 << 27, 10)
     forms = {}
     want = set() if args.no_forms else set(args.forms.split(","))
+    plain_now = value
+    if want & {"mixed", "autotuned"} and args.poisson_mixed < 0:
+        plain_now = plain_rate(g, cells, args.steps)
+        forms["plain_same_state"] = {"value": round(plain_now, 3), "unit": "Mcell-steps/s",
+                                     "steps_before": steps_drift + 1,
+                                     "note": "the plain step re-timed at the state the mixed and autotuned forms "
+                                             "start from; their vs_plain is against this value"}
+        print(f"[bench] plain step after the drift run: {plain_now:.1f} Mcell-steps/s", file=sys.stderr,
+              flush=True)
     if "mixed" in want and args.poisson_mixed < 0:
         try:
-            forms["mixed_precision"] = mixed_form(g, cells, args.steps, value, 1)
-            forms["mixed_precision_fp32_solve"] = mixed_form(g, cells, args.steps, value, 0)
+            forms["mixed_precision"] = mixed_form(g, cells, args.steps, plain_now, 1)
+            forms["mixed_precision_fp32_solve"] = mixed_form(g, cells, args.steps, plain_now, 0)
         except Exception as e:  # reported, never fatal for the plain line
             forms["mixed_precision"] = {"error": repr(e)}
     if "autotuned" in want and args.poisson_mixed < 0 and args.autotune > 0:
         try:
-            forms["autotuned"] = autotuned_form(g, cells, args.steps, value, args.autotune)
+            forms["autotuned"] = autotuned_form(g, cells, args.steps, plain_now, args.autotune)
         except Exception as e:
             forms["autotuned"] = {"error": repr(e)}
     if args.kernels:

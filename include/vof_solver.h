@@ -321,8 +321,12 @@ int vof_set_poisson_mixed(domain_t * domain, int refinements);
  * each kernel of the step runs as one launch covering every case's
  * work-groups, with results bit-identical to integrate() on each case.  The
  * step is asynchronous on the batch's stream: vof_batch_synchronize before
- * touching a case through any other entry point.  No counterpart in the
- * reference (one case per process there). */
+ * touching a case through any other entry point.  A batch of 8 cases or more
+ * is run as 2 batches of alternate cases (4 from 16 cases on: case c in
+ * part c % 4), each on its own stream, stepped in turn by
+ * vof_batch_integrate: the host's work of one part (reading back its dt,
+ * recording its first stage) then overlaps the other parts' kernels.  No
+ * counterpart in the reference (one case per process there). */
 typedef struct vof_batch vof_batch_t;
 int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
                      interface_t * const interfaces[], vof_batch_t ** batch);
@@ -335,7 +339,8 @@ int vof_batch_integrate(vof_batch_t * batch, double dts[]);
  * it changes, e.g. after tuning); on = 0 issues the launches directly.
  * Results are the same either way. */
 int vof_batch_set_graphs(vof_batch_t * batch, int on);
-/* graphs captured and graph launches issued so far */
+/* graphs captured and graph launches issued so far (both parts of a split
+ * batch together) */
 int vof_batch_graph_counts(const vof_batch_t * batch, size_t * captures, size_t * launches);
 int vof_batch_synchronize(vof_batch_t * batch);
 int vof_batch_destroy(vof_batch_t * batch);

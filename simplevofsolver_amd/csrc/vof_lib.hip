@@ -543,7 +543,16 @@ struct vof_batch {
   double * d_dt;
   double * h_dt;
   const double ** d_dt_src;
+  // a batch of 8 cases or more is 2 batches of alternate cases (4 from 16
+  // cases on: case c in part c % 4), each with its own stream, tables and
+  // graphs, stepped in turn: while the host reads one part's dt and records
+  // its first stage, the device runs the other parts' stages.
+  // part_cases[p][i] is the case number of case i of part p.  A batch with
+  // parts holds no stream itself.
+  std::vector<vof_batch *> parts;
+  std::vector<std::vector<int>> part_cases;
 };
+static int batch_parts(int ncases) { return ncases >= 16 ? 4 : ncases >= 8 ? 2 : 1; }
 
 // build (dev == nullptr: size only / fill host) or launch (dev != nullptr)
 // the merged launch number idx of the listed cases (which all recorded the
@@ -2587,9 +2596,8 @@ static int batch_run(vof_batch * B, int ph, const std::function<int()> & pre, co
   return 0;
 }
 
-extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
-                                interface_t * const interfaces[], vof_batch_t ** out) {
-  if (ncases < 1 || !domains || !fluids || !interfaces || !out) return fail("vof_batch_create: bad arguments");
+static int batch_create_one(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
+                            interface_t * const interfaces[], vof_batch_t ** out) {
   vof_batch * B = new vof_batch();
   B->graphs = true;
   for (int c = 0; c < ncases; c++) {
@@ -2671,8 +2679,63 @@ extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fl
   return 0;
 }
 
+extern "C" int vof_batch_create(int ncases, const domain_t * const domains[], fluid_t * const fluids[],
+                                interface_t * const interfaces[], vof_batch_t ** out) {
+  if (ncases < 1 || !domains || !fluids || !interfaces || !out) return fail("vof_batch_create: bad arguments");
+  const int np = batch_parts(ncases);
+  if (np == 1) return batch_create_one(ncases, domains, fluids, interfaces, out);
+  vof_batch * B = new vof_batch();
+  for (int c = 0; c < ncases; c++) {
+    vof_ctx * ctx = ctx_of(domains[c]);
+    if (!ctx) {
+      delete B;
+      return fail("vof_batch_create: case %d has no initialised domain", c);
+    }
+    for (int d = 0; d < c; d++)
+      if (B->ctx[d] == ctx) {
+        delete B;
+        return fail("vof_batch_create: case %d repeats case %d", c, d);
+      }
+    B->ctx.push_back(ctx);
+    B->domain.push_back(domains[c]);
+    B->fluid.push_back(fluids[c]);
+    B->interface.push_back(interfaces[c]);
+  }
+  B->part_cases.resize(np);
+  for (int c = 0; c < ncases; c++) B->part_cases[c % np].push_back(c);
+  for (int p = 0; p < np; p++) {
+    std::vector<const domain_t *> d;
+    std::vector<fluid_t *> f;
+    std::vector<interface_t *> itf;
+    for (int c : B->part_cases[p]) {
+      d.push_back(domains[c]);
+      f.push_back(fluids[c]);
+      itf.push_back(interfaces[c]);
+    }
+    vof_batch_t * P = nullptr;
+    if (batch_create_one((int)d.size(), d.data(), f.data(), itf.data(), &P)) {
+      const std::string err = g_err;
+      vof_batch_destroy(B);
+      g_err = err;
+      return 1;
+    }
+    B->parts.push_back(P);
+  }
+  *out = B;
+  return 0;
+}
+
 extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
   if (!B) return fail("vof_batch_integrate: batch is NULL");
+  if (!B->parts.empty()) {
+    for (size_t p = 0; p < B->parts.size(); p++) {
+      std::vector<double> d(B->part_cases[p].size());
+      if (vof_batch_integrate(B->parts[p], d.data())) return 1;
+      if (dts)
+        for (size_t i = 0; i < d.size(); i++) dts[B->part_cases[p][i]] = d[i];
+    }
+    return 0;
+  }
   const size_t n = B->ctx.size();
   // each case's launches are recorded on a worker thread; an error message
   // (thread-local) is carried back to this thread
@@ -2735,6 +2798,11 @@ extern "C" int vof_batch_integrate(vof_batch_t * B, double dts[]) {
 
 extern "C" int vof_batch_set_graphs(vof_batch_t * B, int on) {
   if (!B) return fail("vof_batch_set_graphs: batch is NULL");
+  if (!B->parts.empty()) {
+    for (auto * P : B->parts)
+      if (vof_batch_set_graphs(P, on)) return 1;
+    return 0;
+  }
   HIPCHK(hipStreamSynchronize(B->stream));
   B->graphs = on != 0;
   if (!B->graphs)
@@ -2744,6 +2812,18 @@ extern "C" int vof_batch_set_graphs(vof_batch_t * B, int on) {
 
 extern "C" int vof_batch_graph_counts(const vof_batch_t * B, size_t * captures, size_t * launches) {
   if (!B || !captures || !launches) return fail("vof_batch_graph_counts: NULL argument");
+  if (!B->parts.empty()) {
+    size_t c = 0, l = 0;
+    for (auto * P : B->parts) {
+      size_t a = 0, b = 0;
+      if (vof_batch_graph_counts(P, &a, &b)) return 1;
+      c += a;
+      l += b;
+    }
+    *captures = c;
+    *launches = l;
+    return 0;
+  }
   *captures = B->graph_captures;
   *launches = B->graph_launches;
   return 0;
@@ -2751,12 +2831,24 @@ extern "C" int vof_batch_graph_counts(const vof_batch_t * B, size_t * captures, 
 
 extern "C" int vof_batch_synchronize(vof_batch_t * B) {
   if (!B) return fail("vof_batch_synchronize: batch is NULL");
+  if (!B->parts.empty()) {
+    for (auto * P : B->parts)
+      if (vof_batch_synchronize(P)) return 1;
+    return 0;
+  }
   HIPCHK(hipStreamSynchronize(B->stream));
   return 0;
 }
 
 extern "C" int vof_batch_destroy(vof_batch_t * B) {
   if (!B) return 0;
+  if (!B->parts.empty() || !B->part_cases.empty()) {
+    int rc = 0;
+    for (auto * P : B->parts)
+      if (vof_batch_destroy(P)) rc = 1;
+    delete B;
+    return rc;
+  }
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   // cases keep working on their own after the batch: give them private
   // tables.  A case whose tables cannot be allocated is left without any
@@ -3008,7 +3100,33 @@ extern "C" int vof_autotune(domain_t * domain, fluid_t * fluid, interface_t * in
 extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, size_t len) {
   if (!B) return fail("vof_batch_autotune: batch is NULL");
   if (steps < 1 || steps > 1000) return fail("vof_batch_autotune: steps must be in 1..1000");
-  HIPCHK(hipStreamSynchronize(B->stream));
+  // a batch in parts is timed as a whole: the snapshots and events go to the
+  // first part's stream, and each tuning step joins the parts' streams on
+  // both sides, so that a trial's events cover both parts' work
+  const bool split = !B->parts.empty();
+  hipStream_t st = split ? B->parts[0]->stream : B->stream;
+  hipEvent_t j0 = nullptr, j1 = nullptr;
+  if (split) {
+    for (auto * P : B->parts) HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK(hipEventCreateWithFlags(&j0, hipEventDisableTiming));
+    if (hipEventCreateWithFlags(&j1, hipEventDisableTiming) != hipSuccess) {
+      (void)hipEventDestroy(j0);
+      return fail("vof_batch_autotune: event creation failed");
+    }
+  } else {
+    HIPCHK(hipStreamSynchronize(B->stream));
+  }
+  auto step = [&]() -> int {
+    if (!split) return vof_batch_integrate(B, nullptr);
+    HIPCHK(hipEventRecord(j0, st));
+    for (size_t p = 1; p < B->parts.size(); p++) HIPCHK(hipStreamWaitEvent(B->parts[p]->stream, j0, 0));
+    if (vof_batch_integrate(B, nullptr)) return 1;
+    for (size_t p = 1; p < B->parts.size(); p++) {
+      HIPCHK(hipEventRecord(j1, B->parts[p]->stream));
+      HIPCHK(hipStreamWaitEvent(st, j1, 0));
+    }
+    return 0;
+  };
   std::vector<std::unique_ptr<TuneSnap>> snaps;
   std::vector<bool> prof(B->ctx.size());
   int rc = 0;
@@ -3022,11 +3140,15 @@ extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, siz
       break;
     }
     snaps.emplace_back(new TuneSnap());
-    rc = tune_take(B->ctx[c], B->fluid[c], B->interface[c], B->stream, *snaps.back());
+    rc = tune_take(B->ctx[c], B->fluid[c], B->interface[c], st, *snaps.back());
   }
-  if (!rc)
-    rc = tune_run(B->ctx, snaps, B->stream, steps, [&]() { return vof_batch_integrate(B, nullptr); }, report, len);
-  if (rc) (void)hipStreamSynchronize(B->stream);
+  if (!rc) rc = tune_run(B->ctx, snaps, st, steps, step, report, len);
+  if (rc) (void)hipStreamSynchronize(st);
+  if (split) {
+    for (auto * P : B->parts) (void)hipStreamSynchronize(P->stream);
+    (void)hipEventDestroy(j0);
+    (void)hipEventDestroy(j1);
+  }
   for (size_t c = 0; c < B->ctx.size(); c++) B->ctx[c]->prof_on = prof[c];
   return rc;
 }

@@ -109,3 +109,64 @@ def test_batch_graph_replays_match_direct_launches(dim, tmp_path):
         batch_d.close()
         for s in singles + members + direct:
             s.close()
+
+
+def _cases_split():
+    # 10 cases: a batch of 8 or more is run as two batches of alternate cases
+    # (5 + 5 here), each on its own stream, stepped in turn
+    return _cases_2d() + _cases_mixed_transforms() + [random_fields((40, 24), seed=78, uniformx=False)]
+
+
+def test_split_batch_is_bit_identical_to_single_cases(tmp_path):
+    """A batch of 10 cases (two parts of alternate cases stepped in turn)
+    gives the single cases' fields and time steps, with graphs and without,
+    returns every case's dt in case order, counts the graphs of both parts,
+    and its autotuner (both parts timed together) leaves the state as it was."""
+    cases = _cases_split()
+    singles = [sva.solver_from_ic(ic, str(tmp_path / f"a{c}")) for c, ic in enumerate(cases)]
+    members = [sva.solver_from_ic(ic, str(tmp_path / f"b{c}")) for c, ic in enumerate(cases)]
+    direct = [sva.solver_from_ic(ic, str(tmp_path / f"c{c}")) for c, ic in enumerate(cases)]
+    batch = sva.Batch(members)
+    batch_d = sva.Batch(direct)
+    batch_d.set_graphs(False)
+    try:
+        batch.integrate()
+        batch_d.integrate()
+        for s in singles:
+            s.integrate()
+        batch.synchronize()
+        before = [[m.get(n) for n in _names(2)] for m in members]
+        rep = batch.autotune(1)
+        assert any(k.startswith("best:") for k, _, _ in rep)
+        # the picks apply to the compared cases too (fft_radix changes results
+        # by rounding)
+        for k, v, _ in rep:
+            if k.startswith("best:"):
+                for s in singles + direct:
+                    s.set_tuning(k[5:], v)
+        for c, m in enumerate(members):
+            for n, want in zip(_names(2), before[c]):
+                assert np.array_equal(m.get(n), want), ("autotune changed the state", c, n)
+        captures0, launches0 = batch.graph_counts()
+        for step in range(3):
+            dts = batch.integrate()
+            dts_d = batch_d.integrate()
+            ref = [s.integrate() for s in singles]
+            assert dts == ref and dts_d == ref, (step, dts, dts_d, ref)
+        batch.synchronize()
+        batch_d.synchronize()
+        for c, (a, d, s) in enumerate(zip(members, direct, singles)):
+            for n in _names(2):
+                want = s.get(n)
+                assert np.array_equal(a.get(n), want), (c, n)
+                assert np.array_equal(d.get(n), want), (c, n)
+        captures, launches = batch.graph_counts()
+        # 4 phases per part and step, both parts
+        assert launches - launches0 == 3 * 4 * 2, (launches0, launches)
+        assert captures >= 8, captures
+        assert batch_d.graph_counts() == (0, 0)
+    finally:
+        batch.close()
+        batch_d.close()
+        for s in singles + members + direct:
+            s.close()
