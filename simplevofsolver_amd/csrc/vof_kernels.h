@@ -3377,9 +3377,15 @@ __device__ __forceinline__ void lds_barrier() {
 // is computed.  For cell plane k: the registers (plane k+1) go to LDS, the
 // loads of plane k+2 are issued, the corner plane between k and k+1 is
 // formed (the one between k-1 and k is kept from the previous plane), then
-// the cells of plane k.  Same arithmetic as k_iface_fused.
+// the cells of plane k.  Same arithmetic as k_iface_fused.  The intercepts
+// (a Newton solve at each interface cell) are not solved in the march, where
+// a wavefront would run them with only its few interface lanes active: the
+// cells' vof and normals go to a list in LDS, which the work-group's lanes
+// solve one cell each after the last plane (cells beyond the list's
+// kMList entries are solved in place).
 constexpr int kMX = 32, kMY = 8;
 constexpr int kMThreads = kMX * kMY;
+constexpr int kMList = 256;
 template <int KZ>
 __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, const double * __restrict__ vof,
                                                    double * __restrict__ normal, double * __restrict__ curv) {
@@ -3388,6 +3394,9 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
   constexpr int PF = (VP + kMThreads - 1) / kMThreads;   // prefetched values per thread
   __shared__ double sv[3][VP];
   __shared__ double sc[2][3][CP];
+  __shared__ double lv[4][kMList];   // listed interface cells: vof, normal x, y, z
+  __shared__ int lc[kMList];         // their (a, b, k - kb)
+  __shared__ int ln;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int i0 = 1 + (int)vb.x * kMX, j0 = 1 + (int)vb.y * kMY, kb = 1 + (int)vb.z * KZ;
   const int ke = min(kb + KZ - 1, nz);
@@ -3437,6 +3446,7 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
       o2[e] = gz * ninv;
     }
   };
+  if (tid == 0) ln = 0;
   fetch(kb - 1);
   put(kb - 1);
   fetch(kb);
@@ -3493,12 +3503,33 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
           nv[1] *= ninv;
           nv[2] *= ninv;
         }
-        nv[3] = intercept(lvof, nv);
-        for (int d = 0; d <= 3; d++) {
-          double * base = normal + d;
-          store_periodic(base, (i - 1) * 4, nx * 4, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
+        const int slot = atomicAdd(&ln, 1);
+        if (slot < kMList) {
+          lv[0][slot] = lvof;
+          lv[1][slot] = nv[0];
+          lv[2][slot] = nv[1];
+          lv[3][slot] = nv[2];
+          lc[slot] = a | (b << 8) | ((k - kb) << 16);
+        } else {
+          nv[3] = intercept(lvof, nv);
+          for (int d = 0; d <= 3; d++) {
+            double * base = normal + d;
+            store_periodic(base, (i - 1) * 4, nx * 4, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
+          }
         }
       }
+    }
+  }
+  lds_barrier();
+  const int nl = min(ln, kMList);
+  for (int e = tid; e < nl; e += kMThreads) {
+    const int c = lc[e];
+    const int ci = i0 + (c & 255), cj = j0 + ((c >> 8) & 255), ck = kb + (c >> 16);
+    double nv[4] = {lv[1][e], lv[2][e], lv[3][e], 0.};
+    nv[3] = intercept(lv[0][e], nv);
+    for (int d = 0; d <= 3; d++) {
+      double * base = normal + d;
+      store_periodic(base, (ci - 1) * 4, nx * 4, ny + 2, cj, PLANE1(ck), ny, nz, 1, nv[d]);
     }
   }
 }
