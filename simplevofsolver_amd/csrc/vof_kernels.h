@@ -3175,6 +3175,12 @@ __device__ __forceinline__ void k_iface_fused_fz_body(const VBlock & vb, Grid g,
 #endif
   __shared__ double sv[VX * VY * VZ];
   __shared__ double sc[NDIMS][CX * CY * CZ];
+  // interface cells of the tile (vof, unit normal), solved for their
+  // intercepts one per lane after every cell is classified: in place, a
+  // wavefront would run the Newton solve with only its few interface lanes
+  __shared__ double lv[NDIMS + 1][kIThreads];
+  __shared__ int lc[kIThreads];
+  __shared__ int ln;
   const int nx = g.nx, ny = g.ny;
   const Tile3 tile = g.bands ? band_tile(vb) : plain_tile(vb);
   const int i0 = 1 + tile.x * kIX, j0 = 1 + tile.y * kIY;
@@ -3186,6 +3192,7 @@ __device__ __forceinline__ void k_iface_fused_fz_body(const VBlock & vb, Grid g,
   const int k0 = 1;
 #endif
   const int tid = threadIdx.x;
+  if (tid == 0) ln = 0;
   // vof tile: cells (i0-1 .. i0+kIX, j0-1 .., k0-1 ..)
   for (int e = tid; e < VX * VY * VZ; e += kIThreads) {
     const int a = e % VX, r = e / VX;
@@ -3258,84 +3265,106 @@ __device__ __forceinline__ void k_iface_fused_fz_body(const VBlock & vb, Grid g,
 #if NDIMS == 3
   const int c = tid / (kIX * kIY);
   const int k = k0 + c;
-  if (i > nx || j > ny || k > nz) return;
+  const bool own = i <= nx && j <= ny && k <= nz;
 #else
   const int c = 0;
   const int k = 1;
-  if (i > nx || j > ny) return;
+  const bool own = i <= nx && j <= ny;
 #endif
 #define SC(D, A, B, C) sc[D][(A) + CX * ((B) + CY * (C))]
-  const double dx = g.dxf[i - 1];
-  const double rdx = g.xt[kXT * i + XT_RXF];   // 1 / DXF
-  double cv;
+  if (own) {
+    const double dx = g.dxf[i - 1];
+    const double rdx = g.xt[kXT * i + XT_RXF];   // 1 / DXF
+    double cv;
 #if NDIMS == 2
-  {
-    const double dnxdx = rdx * (-SC(0, a, b, 0) + SC(0, a + 1, b, 0) - SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0));
-    const double dnydy = g.rdy * (-SC(1, a, b, 0) - SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0));
-    cv = 0.5 * (-dnxdx - dnydy);
-  }
+    {
+      const double dnxdx = rdx * (-SC(0, a, b, 0) + SC(0, a + 1, b, 0) - SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0));
+      const double dnydy = g.rdy * (-SC(1, a, b, 0) - SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0));
+      cv = 0.5 * (-dnxdx - dnydy);
+    }
 #else
-  {
-    const double dnxdx = rdx * ((-SC(0, a, b, c) + SC(0, a + 1, b, c) - SC(0, a, b + 1, c) + SC(0, a + 1, b + 1, c))
-                                    + (-SC(0, a, b, c + 1) + SC(0, a + 1, b, c + 1) - SC(0, a, b + 1, c + 1) + SC(0, a + 1, b + 1, c + 1)));
-    const double dnydy = g.rdy * ((-SC(1, a, b, c) - SC(1, a + 1, b, c) + SC(1, a, b + 1, c) + SC(1, a + 1, b + 1, c))
-                                      + (-SC(1, a, b, c + 1) - SC(1, a + 1, b, c + 1) + SC(1, a, b + 1, c + 1) + SC(1, a + 1, b + 1, c + 1)));
-    const double dnzdz = g.rdz * (-(SC(2, a, b, c) + SC(2, a + 1, b, c) + SC(2, a, b + 1, c) + SC(2, a + 1, b + 1, c))
-                                      + (SC(2, a, b, c + 1) + SC(2, a + 1, b, c + 1) + SC(2, a, b + 1, c + 1) + SC(2, a + 1, b + 1, c + 1)));
-    cv = 0.25 * (-dnxdx - dnydy - dnzdz);
-  }
+    {
+      const double dnxdx = rdx * ((-SC(0, a, b, c) + SC(0, a + 1, b, c) - SC(0, a, b + 1, c) + SC(0, a + 1, b + 1, c))
+                                      + (-SC(0, a, b, c + 1) + SC(0, a + 1, b, c + 1) - SC(0, a, b + 1, c + 1) + SC(0, a + 1, b + 1, c + 1)));
+      const double dnydy = g.rdy * ((-SC(1, a, b, c) - SC(1, a + 1, b, c) + SC(1, a, b + 1, c) + SC(1, a + 1, b + 1, c))
+                                        + (-SC(1, a, b, c + 1) - SC(1, a + 1, b, c + 1) + SC(1, a, b + 1, c + 1) + SC(1, a + 1, b + 1, c + 1)));
+      const double dnzdz = g.rdz * (-(SC(2, a, b, c) + SC(2, a + 1, b, c) + SC(2, a, b + 1, c) + SC(2, a + 1, b + 1, c))
+                                        + (SC(2, a, b, c + 1) + SC(2, a + 1, b, c + 1) + SC(2, a, b + 1, c + 1) + SC(2, a + 1, b + 1, c + 1)));
+      cv = 0.25 * (-dnxdx - dnydy - dnzdz);
+    }
 #endif
-  store_periodic(curv, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, cv);
-  const double lvof = sv[(a + 1) + VX * ((b + 1) + VY * (NDIMS == 3 ? c + 1 : 0))];
-  bool iface;
-  if (FZ == 2) {
-    iface = (fz.cell[I_C1(i, j, k)] & 0x80u) != 0;
-  } else {
-    iface = !(lvof < kVofMin || 1. - kVofMin < lvof);
-    if (FZ == 1) fz.cell[I_C1(i, j, k)] = iface ? 0x80u : 0u;
-  }
-  if (!iface) return;
-  double nv[NDIMS + 1];
+    store_periodic(curv, i, nx + 2, ny + 2, j, PLANE1(k), ny, nz, 1, cv);
+    const double lvof = sv[(a + 1) + VX * ((b + 1) + VY * (NDIMS == 3 ? c + 1 : 0))];
+    bool iface;
+    if (FZ == 2) {
+      iface = (fz.cell[I_C1(i, j, k)] & 0x80u) != 0;
+    } else {
+      iface = !(lvof < kVofMin || 1. - kVofMin < lvof);
+      if (FZ == 1) fz.cell[I_C1(i, j, k)] = iface ? 0x80u : 0u;
+    }
+    if (iface) {
+      double nv[NDIMS];
 #if NDIMS == 2
-  {
-    double nxv = SC(0, a, b, 0) + SC(0, a + 1, b, 0) + SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0);
-    double nyv = SC(1, a, b, 0) + SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0);
-    nxv /= dx;
-    nyv /= g.dy;
-    const double norm = sqrt(nxv * nxv + nyv * nyv);
-    const double ninv = 1. / fmax(norm, DBL_EPSILON);
-    nv[0] = nxv * ninv;
-    nv[1] = nyv * ninv;
-  }
+      {
+        double nxv = SC(0, a, b, 0) + SC(0, a + 1, b, 0) + SC(0, a, b + 1, 0) + SC(0, a + 1, b + 1, 0);
+        double nyv = SC(1, a, b, 0) + SC(1, a + 1, b, 0) + SC(1, a, b + 1, 0) + SC(1, a + 1, b + 1, 0);
+        nxv /= dx;
+        nyv /= g.dy;
+        const double norm = sqrt(nxv * nxv + nyv * nyv);
+        const double ninv = 1. / fmax(norm, DBL_EPSILON);
+        nv[0] = nxv * ninv;
+        nv[1] = nyv * ninv;
+      }
 #else
-  for (int d = 0; d < 3; d++) {
-    nv[d] = (SC(d, a, b, c) + SC(d, a + 1, b, c) + SC(d, a, b + 1, c) + SC(d, a + 1, b + 1, c))
-          + (SC(d, a, b, c + 1) + SC(d, a + 1, b, c + 1) + SC(d, a, b + 1, c + 1) + SC(d, a + 1, b + 1, c + 1));
-  }
-  nv[0] /= dx;
-  nv[1] /= g.dy;
-  nv[2] /= g.dz;
-  {
-    const double norm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-    const double ninv = 1. / fmax(norm, DBL_EPSILON);
-    nv[0] *= ninv;
-    nv[1] *= ninv;
-    nv[2] *= ninv;
-  }
+      for (int d = 0; d < 3; d++) {
+        nv[d] = (SC(d, a, b, c) + SC(d, a + 1, b, c) + SC(d, a, b + 1, c) + SC(d, a + 1, b + 1, c))
+              + (SC(d, a, b, c + 1) + SC(d, a + 1, b, c + 1) + SC(d, a, b + 1, c + 1) + SC(d, a + 1, b + 1, c + 1));
+      }
+      nv[0] /= dx;
+      nv[1] /= g.dy;
+      nv[2] /= g.dz;
+      {
+        const double norm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const double ninv = 1. / fmax(norm, DBL_EPSILON);
+        nv[0] *= ninv;
+        nv[1] *= ninv;
+        nv[2] *= ninv;
+      }
 #endif
+      const int slot = atomicAdd(&ln, 1);   // at most one entry per lane
+      lv[0][slot] = lvof;
+#pragma unroll
+      for (int d = 0; d < NDIMS; d++) lv[1 + d][slot] = nv[d];
+      lc[slot] = tid;
+    }
+  }
 #undef SC
-  {
-    int it = FZ == 2 ? (int)(fz.cell[I_C1(i, j, k)] & 15u) : 0;
-    nv[NDIMS] = intercept<FZ>(lvof, nv, &it);
-    if (FZ == 1) fz.cell[I_C1(i, j, k)] = (unsigned char)(0x80u | (unsigned)it);
-  }
-  // NORMAL rows are 0..ny+1 like cell arrays but x runs 1..nx without padding
-  for (int d = 0; d <= NDIMS; d++) {
-    // component d of each image: the component array is interleaved, so
-    // store through a strided view
-    double * base = normal + d;
-    const int n0 = nx * (NDIMS + 1);
-    store_periodic(base, (i - 1) * (NDIMS + 1), n0, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
+  __syncthreads();
+  const int nl = ln;
+  for (int e = tid; e < nl; e += kIThreads) {
+    const int t = lc[e];
+    const int ia = i0 + t % kIX, jb = j0 + (t / kIX) % kIY;
+#if NDIMS == 3
+    const int kc = k0 + t / (kIX * kIY);
+#else
+    const int kc = 1;
+#endif
+    double nv[NDIMS + 1];
+#pragma unroll
+    for (int d = 0; d < NDIMS; d++) nv[d] = lv[1 + d][e];
+    {
+      int it = FZ == 2 ? (int)(fz.cell[I_C1(ia, jb, kc)] & 15u) : 0;
+      nv[NDIMS] = intercept<FZ>(lv[0][e], nv, &it);
+      if (FZ == 1) fz.cell[I_C1(ia, jb, kc)] = (unsigned char)(0x80u | (unsigned)it);
+    }
+    // NORMAL rows are 0..ny+1 like cell arrays but x runs 1..nx without padding
+    for (int d = 0; d <= NDIMS; d++) {
+      // component d of each image: the component array is interleaved, so
+      // store through a strided view
+      double * base = normal + d;
+      const int n0 = nx * (NDIMS + 1);
+      store_periodic(base, (ia - 1) * (NDIMS + 1), n0, ny + 2, jb, PLANE1(kc), ny, nz, 1, nv[d]);
+    }
   }
 }
 
