@@ -2135,52 +2135,46 @@ __device__ void thomas_line_seq(int n, int nlines, int line, double eval, const 
   }
 }
 
-// thomas_line_seq on a line staged in LDS (ql: the rows, vl: n factors)
+// The elimination of k_thomas_spec for one line staged in LDS (ql), both
+// components by one lane, with the line's factors given (tab: c0, rlast,
+// flag; val and v of rows 0 .. n-1 in vl, fl; the rows' lower coefficients
+// in pl): per component the operations of k_thomas_spec with the divisions
+// done in advance, its gauge choice for the last row included
 template <class T2>
-__device__ void thomas_line_lds(int n, double eval, const double * __restrict__ pl, const double * __restrict__ pu,
-                                double2 * ql, double * vl) {
-  double cc = -pl[0] - pu[0] + eval;
-  cc += pl[0];
-  double vprev = pu[0] / cc;
+__device__ void thomas_line_tab(int n, const double * tab, const double * pl, const double * vl,
+                                const double * fl, double2 * ql) {
+  const double c0 = tab[0], rlast = tab[1];
+  const bool flag = tab[2] != 0.;
   double2 qprev = ql[0];
-  qprev.x = qprev.x / cc;
-  qprev.y = qprev.y / cc;
+  qprev.x = qprev.x / c0;
+  qprev.y = qprev.y / c0;
   ql[0] = to_d2(from_d2<T2>(qprev));
-  vl[0] = vprev;
+#pragma unroll 4
   for (int i = 1; i < n - 1; i++) {
-    const double c = -pl[i] - pu[i] + eval;
-    const double val = 1. / (c - pl[i] * vprev);
-    const double vi = val * (pu[i]);
     double2 qi = ql[i];
-    qi.x = val * (qi.x - pl[i] * qprev.x);
-    qi.y = val * (qi.y - pl[i] * qprev.y);
+    qi.x = vl[i] * (qi.x - pl[i] * qprev.x);
+    qi.y = vl[i] * (qi.y - pl[i] * qprev.y);
     ql[i] = to_d2(from_d2<T2>(qi));
-    vl[i] = vi;
     qprev = qi;
-    vprev = vi;
   }
   {
     const int i = n - 1;
-    double c = -pl[i] - pu[i] + eval;
-    c += pu[i];
-    const double val = c - pl[i] * vprev;
     double2 qi = ql[i];
-    const bool gauge = sizeof(T2) == sizeof(float2) && eval == 0.;
-    if (!gauge && fabs(val) > DBL_EPSILON) {
-      const double r = 1. / val;
-      qi.x = r * (qi.x - pl[i] * qprev.x);
-      qi.y = r * (qi.y - pl[i] * qprev.y);
+    const bool gauge = sizeof(T2) == sizeof(float2);
+    if (!gauge && flag) {
+      qi.x = rlast * (qi.x - pl[i] * qprev.x);
+      qi.y = rlast * (qi.y - pl[i] * qprev.y);
     } else {
       qi = make_double2(0., 0.);
     }
     ql[i] = to_d2(from_d2<T2>(qi));
     qprev = qi;
   }
+#pragma unroll 4
   for (int i = n - 2; i >= 0; i--) {
     double2 qi = ql[i];
-    const double vi = vl[i];
-    qi.x -= vi * qprev.x;
-    qi.y -= vi * qprev.y;
+    qi.x -= fl[i] * qprev.x;
+    qi.y -= fl[i] * qprev.y;
     ql[i] = to_d2(from_d2<T2>(qi));
     qprev = qi;
   }
@@ -2206,8 +2200,10 @@ __device__ void thomas_line_lds(int n, double eval, const double * __restrict__ 
 // the sequential order's by rounding (a few 1e-14 of the line's largest
 // entry).  The (ky, kz) = (0, 0) line (lambda = 0) is singular, Neumann in
 // x; it keeps the sequential elimination and its choice of the constant: its
-// segment lanes stage the line in LDS, where one lane eliminates it
-// (thomas_line_lds) while the other lines solve their reduced systems.  Only
+// segment lanes stage the line in LDS, where one lane eliminates it while
+// the other lines solve their reduced systems, with the factors of that
+// line's sweep (stab, built once on the host by the same operations) staged
+// beside it (thomas_line_tab: no division left on the recurrence).  Only
 // line 0 has lambda = 0 (every other eigenvalue is negative), and it is line
 // t = 0 of the first work-group; a singular line elsewhere would take
 // thomas_line_seq on global memory.
@@ -2219,13 +2215,16 @@ template <class T2>
 __device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, int nlines, int nh,
                                                    const double * __restrict__ pl, const double * __restrict__ pu,
                                                    const double * __restrict__ ey, const double * __restrict__ ez,
-                                                   T2 * __restrict__ spec, double * __restrict__ vbuf) {
+                                                   const double * __restrict__ stab, T2 * __restrict__ spec,
+                                                   double * __restrict__ vbuf) {
   // reduced rows 2 seg (first row of the segment) and 2 seg + 1 (last row):
   // lower, diagonal, upper, right-hand side (re, im); the solution replaces
   // the right-hand side
   __shared__ double red[5][2 * kTpSeg][kTpLines];
   __shared__ double2 sline[kTpSeg * kTpRows];   // the staged singular line
+  __shared__ double sval[kTpSeg * kTpRows];      // its factors and lower coefficients
   __shared__ double sfac[kTpSeg * kTpRows];
+  __shared__ double spl[kTpSeg * kTpRows];
   const int tid = (int)threadIdx.x;
   const int t = tid % kTpLines, seg = tid / kTpLines;   // 8 consecutive lines per segment: 128-byte rows
   const int line = (int)vb.x * kTpLines + t;
@@ -2243,6 +2242,24 @@ __device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, in
   }
   const bool sing = eval == 0.;
   const bool stage = has_line && sing && t == 0 && seg < P;   // rows of the staged singular line
+  // the work-group of the singular line (its line t = 0) stages its factors
+  {
+    const int l0 = (int)vb.x * kTpLines;
+    bool wsing = false;
+    if (l0 < nlines) {
+#if NDIMS == 2
+      wsing = ey[l0 % nh] == 0.;
+#else
+      wsing = ey[l0 % nh] + ez[l0 / nh] == 0.;
+#endif
+    }
+    if (wsing)
+      for (int e = tid; e < n; e += kTpLines * kTpSeg) {
+        sval[e] = stab[3 + e];
+        sfac[e] = stab[3 + n + e];
+        spl[e] = pl[e];
+      }
+  }
   const bool act = has_line && !sing && seg < P;
   const int s = seg * n / P, e = (seg + 1) * n / P - 1;
   const int m = e - s + 1;
@@ -2341,7 +2358,7 @@ __device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, in
     red[4][2 * seg + 1][t] = de.y;
   }
   __syncthreads();
-  if (tid == 0 && has_line && sing) thomas_line_lds<T2>(n, eval, pl, pu, sline, sfac);
+  if (tid == 0 && has_line && sing) thomas_line_tab<T2>(n, stab, spl, sval, sfac, sline);
   // cyclic reduction of the reduced systems (rows seg and seg + 32 of line t
   // per lane): step h combines row j with rows j - h and j + h so that it
   // couples j - 2h, j, j + 2h; after ceil(log2 2P) steps every row stands
@@ -2427,8 +2444,9 @@ __global__ __launch_bounds__(kTpLines * kTpSeg) void k_thomas_part(int nx, int n
                                                                     const double * __restrict__ pu,
                                                                     const double * __restrict__ ey,
                                                                     const double * __restrict__ ez,
+                                                                    const double * __restrict__ stab,
                                                                     T2 * __restrict__ spec, double * __restrict__ vbuf) {
-  k_thomas_part_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, spec, vbuf);
+  k_thomas_part_body<T2>(hw_block(), nx, nlines, nh, pl, pu, ey, ez, stab, spec, vbuf);
 }
 
 // inverse real transform in y (c2r: the imaginary parts of the k = 0 and

@@ -268,6 +268,7 @@ struct vof_ctx {
   double * d_pu;
   double * d_ey;                         // eigenvalues in y, z (dft.c:263-291)
   double * d_ez;
+  double * d_stab;                       // factors of the (0, 0) line: c0, rlast, flag, val[nx], v[nx]
   double2 * d_spec;
   // mixed-precision Poisson (refinements < 0: plain FP64)
   int poisson_refine;
@@ -1068,6 +1069,34 @@ static int ctx_create(domain_t * domain) {
     HIPCHK(hipMalloc(&ctx->d_pu, sizeof(double) * nx));
     HIPCHK(hipMemcpy(ctx->d_pl, l.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_pu, u.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+    // elimination factors of the (ky, kz) = (0, 0) line, whose eigenvalue is
+    // 0 in every step: the operations of k_thomas_spec's sweep for it, once
+    // (the partitioned solve eliminates that line with them)
+    std::vector<double> st(3 + 2 * (size_t)nx, 0.);
+    if (nx >= 2) {
+      const double eval = 0.;
+      double cc = -l[0] - u[0] + eval;
+      cc += l[0];
+      st[0] = cc;
+      double vprev = u[0] / cc;
+      st[3 + nx] = vprev;
+      for (int i = 1; i < nx - 1; i++) {
+        const double c = -l[i] - u[i] + eval;
+        const double val = 1. / (c - l[i] * vprev);
+        const double vi = val * (u[i]);
+        st[3 + i] = val;
+        st[3 + nx + i] = vi;
+        vprev = vi;
+      }
+      const int i = nx - 1;
+      double c = -l[i] - u[i] + eval;
+      c += u[i];
+      const double val = c - l[i] * vprev;
+      st[2] = fabs(val) > DBL_EPSILON ? 1. : 0.;
+      st[1] = st[2] != 0. ? 1. / val : 0.;
+    }
+    HIPCHK(hipMalloc(&ctx->d_stab, sizeof(double) * st.size()));
+    HIPCHK(hipMemcpy(ctx->d_stab, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
   }
 #undef DXF
 #undef DXC
@@ -1197,7 +1226,7 @@ extern "C" int domain_finalise(domain_t * domain) {
     }
     for (int m = 0; m < 12; m++) (void)hipEventDestroy(ctx->tab_done[m]);
     dev_free(ctx->d_twy); if (ctx->d_twz) dev_free(ctx->d_twz);
-    dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez);
+    dev_free(ctx->d_pl); dev_free(ctx->d_pu); dev_free(ctx->d_ey); dev_free(ctx->d_ez); dev_free(ctx->d_stab);
     dev_free(ctx->d_spec32); dev_free(ctx->d_twy32); dev_free(ctx->d_twz32);
     dev_free(ctx->d_spec); dev_free(ctx->d_vbuf); dev_free(ctx->d_delta); dev_free(ctx->d_part);
     if (ctx->d_vof_spare) dev_free(ctx->d_vof_spare);
@@ -1779,7 +1808,7 @@ static int launch_thomas(vof_ctx * ctx, const char * name, T2 * spec, int nlines
   if (ctx->thomas_part && ctx->nx >= 4 && ctx->nx <= 256) {
     const dim3 g((unsigned)((nlines + vofk::kTpLines - 1) / vofk::kTpLines)), b(vofk::kTpLines * vofk::kTpSeg);
     KLB(ctx, name, vofk::kTpLines * vofk::kTpSeg, (vofk::k_thomas_part_body<T2>), (vofk::k_thomas_part<T2>), g, b, 0,
-        ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, spec, ctx->d_vbuf);
+        ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_stab, spec, ctx->d_vbuf);
     return 0;
   }
   // two lanes per line (real and imaginary parts)
