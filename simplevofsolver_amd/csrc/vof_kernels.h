@@ -2190,8 +2190,9 @@ __device__ void thomas_line_tab(int n, const double * tab, const double * pl, co
 //     interior row reads x_i = d_i - a_i x_s - c_i x_e in the segment's first
 //     and last unknowns x_s, x_e;
 //  2. the first and last rows of the P segments of a line form a
-//     tri-diagonal system in those 2P unknowns; the lanes write it to LDS and
-//     solve it there by cyclic reduction (log2 2P steps, 2 rows per lane);
+//     tri-diagonal system in those 2P unknowns; the lanes write it to LDS,
+//     where one wavefront per two lines condenses it to the P unknowns x_e
+//     and solves that by cyclic reduction (log2 P steps, a row per lane);
 //  3. every lane forms its interior rows from x_s, x_e and stores its rows.
 // The spectrum crosses HBM once each way, where the sequential elimination
 // also stores and re-reads the modified right-hand side and reads its
@@ -2217,10 +2218,15 @@ __device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, in
                                                    const double * __restrict__ ey, const double * __restrict__ ez,
                                                    const double * __restrict__ stab, T2 * __restrict__ spec,
                                                    double * __restrict__ vbuf) {
-  // reduced rows 2 seg (first row of the segment) and 2 seg + 1 (last row):
-  // lower, diagonal, upper, right-hand side (re, im); the solution replaces
-  // the right-hand side
-  __shared__ double red[5][2 * kTpSeg][kTpLines];
+  // the end rows of every segment: r1 (first row: lower, diagonal, upper,
+  // right-hand side re, im), r2 (last row: lower, upper, right-hand side re,
+  // im; its diagonal is 1).  Segments of a line are contiguous, lines 33
+  // apart (the segment lanes of 8 lines then fall in distinct banks).  The
+  // reduction then publishes its rows in r1 and leaves each segment's x_s
+  // and x_e (re, im) in r2.
+  constexpr int RS = kTpSeg + 1;
+  __shared__ double r1s[5][kTpLines][RS];
+  __shared__ double r2s[4][kTpLines][RS];
   __shared__ double2 sline[kTpSeg * kTpRows];   // the staged singular line
   __shared__ double sval[kTpSeg * kTpRows];      // its factors and lower coefficients
   __shared__ double sfac[kTpSeg * kTpRows];
@@ -2346,78 +2352,133 @@ __device__ __forceinline__ void k_thomas_part_body(const VBlock & vb, int nx, in
       r1.x = D[0].x - up * D[1].x;
       r1.y = D[0].y - up * D[1].y;
     }
-    red[0][2 * seg][t] = lo;
-    red[1][2 * seg][t] = d1;
-    red[2][2 * seg][t] = u1;
-    red[3][2 * seg][t] = r1.x;
-    red[4][2 * seg][t] = r1.y;
-    red[0][2 * seg + 1][t] = ae;
-    red[1][2 * seg + 1][t] = 1.;
-    red[2][2 * seg + 1][t] = ce;
-    red[3][2 * seg + 1][t] = de.x;
-    red[4][2 * seg + 1][t] = de.y;
+    r1s[0][t][seg] = lo;
+    r1s[1][t][seg] = d1;
+    r1s[2][t][seg] = u1;
+    r1s[3][t][seg] = r1.x;
+    r1s[4][t][seg] = r1.y;
+    r2s[0][t][seg] = ae;
+    r2s[1][t][seg] = ce;
+    r2s[2][t][seg] = de.x;
+    r2s[3][t][seg] = de.y;
   }
   __syncthreads();
   if (tid == 0 && has_line && sing) thomas_line_tab<T2>(n, stab, spl, sval, sfac, sline);
-  // cyclic reduction of the reduced systems (rows seg and seg + 32 of line t
-  // per lane): step h combines row j with rows j - h and j + h so that it
-  // couples j - 2h, j, j + 2h; after ceil(log2 2P) steps every row stands
-  // alone.  The rows are diagonally dominant as the segments' rows are.
-  const int nr = 2 * P;
-  const bool red_on = has_line && !sing;
-  for (int h = 1; h < nr; h *= 2) {
-    double nw[2][5];
-#pragma unroll
-    for (int k2 = 0; k2 < 2; k2++) {
-      const int j = seg + kTpSeg * k2;
-      if (red_on && j < nr) {
-        const double lo = red[0][j][t], up = red[2][j][t];
-        double nlo = 0., nup = 0., ndg = red[1][j][t], nre = red[3][j][t], nim = red[4][j][t];
-        if (j - h >= 0) {
-          const double al = -lo / red[1][j - h][t];
-          nlo = al * red[0][j - h][t];
-          ndg += al * red[2][j - h][t];
-          nre += al * red[3][j - h][t];
-          nim += al * red[4][j - h][t];
+  // the reduced system of a line, one wavefront per two lines (lanes 0-31
+  // line 2w, 32-63 line 2w + 1; lane k for segment k).  Segment k's rows
+  //   r1:  l1 x_e(k-1) + d1 x_s(k) + u1 x_e(k) = r1
+  //   r2:  a2 x_s(k) + x_e(k) + c2 x_s(k+1) = r2
+  // with x_s(k) and x_s(k+1) taken from the r1 rows of segments k and k + 1
+  // leave a tri-diagonal system in the P unknowns x_e(k), solved by cyclic
+  // reduction: step h combines row k with rows k - h and k + h so that it
+  // couples k - 2h, k, k + 2h; after ceil(log2 P) steps every row stands
+  // alone.  Then x_s(k) follows from r1.  The rows are diagonally dominant as
+  // the segments' rows are.  A lane keeps its row in registers and publishes
+  // it, with the reciprocal of its diagonal, for its neighbours each step.  A
+  // wavefront's LDS accesses complete in order, so its steps need no
+  // barrier: the wavefront-scope fences only keep the compiler from moving
+  // one step's reads across the previous step's writes.
+  {
+    const int lw = tid & 63;
+    const int tl = 2 * (tid >> 6) + (lw >> 5);
+    const int k = lw & 31;
+    const int lr = (int)vb.x * kTpLines + tl;
+    bool on = false;
+    if (lr < nlines && k < P) {
+#if NDIMS == 2
+      on = ey[lr % nh] != 0.;
+#else
+      on = ey[lr % nh] + ez[lr / nh] != 0.;
+#endif
+    }
+    double l1 = 0., d1 = 1., u1 = 0., q1x = 0., q1y = 0.;
+    double own[5] = {0., 1., 0., 0., 0.};
+    double rd = 1.;
+    if (on) {
+      l1 = r1s[0][tl][k];
+      d1 = r1s[1][tl][k];
+      u1 = r1s[2][tl][k];
+      q1x = r1s[3][tl][k];
+      q1y = r1s[4][tl][k];
+      const double a2 = r2s[0][tl][k], c2 = r2s[1][tl][k];
+      const double w = a2 / d1;
+      own[0] = -w * l1;
+      own[1] = 1. - w * u1;
+      own[3] = r2s[2][tl][k] - w * q1x;
+      own[4] = r2s[3][tl][k] - w * q1y;
+      if (k + 1 < P) {
+        const double g = c2 / r1s[1][tl][k + 1];
+        own[1] -= g * r1s[0][tl][k + 1];
+        own[2] = -g * r1s[2][tl][k + 1];
+        own[3] -= g * r1s[3][tl][k + 1];
+        own[4] -= g * r1s[4][tl][k + 1];
+      }
+      rd = 1. / own[1];
+    }
+    for (int h = 1; h < P; h *= 2) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+      if (on) {
+        r1s[0][tl][k] = own[0];
+        r1s[1][tl][k] = rd;
+        r1s[2][tl][k] = own[2];
+        r1s[3][tl][k] = own[3];
+        r1s[4][tl][k] = own[4];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+      if (on) {
+        double nlo = 0., nup = 0., ndg = own[1], nre = own[3], nim = own[4];
+        if (k - h >= 0) {
+          const double al = -own[0] * r1s[1][tl][k - h];
+          nlo = al * r1s[0][tl][k - h];
+          ndg += al * r1s[2][tl][k - h];
+          nre += al * r1s[3][tl][k - h];
+          nim += al * r1s[4][tl][k - h];
         }
-        if (j + h < nr) {
-          const double ga = -up / red[1][j + h][t];
-          nup = ga * red[2][j + h][t];
-          ndg += ga * red[0][j + h][t];
-          nre += ga * red[3][j + h][t];
-          nim += ga * red[4][j + h][t];
+        if (k + h < P) {
+          const double ga = -own[2] * r1s[1][tl][k + h];
+          nup = ga * r1s[2][tl][k + h];
+          ndg += ga * r1s[0][tl][k + h];
+          nre += ga * r1s[3][tl][k + h];
+          nim += ga * r1s[4][tl][k + h];
         }
-        nw[k2][0] = nlo;
-        nw[k2][1] = ndg;
-        nw[k2][2] = nup;
-        nw[k2][3] = nre;
-        nw[k2][4] = nim;
+        own[0] = nlo;
+        own[1] = ndg;
+        own[2] = nup;
+        own[3] = nre;
+        own[4] = nim;
+        rd = 1. / ndg;
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int k2 = 0; k2 < 2; k2++) {
-      const int j = seg + kTpSeg * k2;
-      if (red_on && j < nr) {
-#pragma unroll
-        for (int c = 0; c < 5; c++) red[c][j][t] = nw[k2][c];
-      }
+    // x_e(k), then x_s(k) = (r1 - l1 x_e(k-1) - u1 x_e(k)) / d1
+    const double yx = own[3] * rd, yy = own[4] * rd;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    if (on) {
+      r2s[2][tl][k] = yx;
+      r2s[3][tl][k] = yy;
     }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k2 = 0; k2 < 2; k2++) {
-    const int j = seg + kTpSeg * k2;
-    if (red_on && j < nr) {
-      const double dg = red[1][j][t];
-      red[3][j][t] = red[3][j][t] / dg;
-      red[4][j][t] = red[4][j][t] / dg;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    if (on) {
+      double px = 0., py = 0.;
+      if (k > 0) {
+        px = r2s[2][tl][k - 1];
+        py = r2s[3][tl][k - 1];
+      }
+      r2s[0][tl][k] = (q1x - l1 * px - u1 * yx) / d1;
+      r2s[1][tl][k] = (q1y - l1 * py - u1 * yy) / d1;
     }
   }
   __syncthreads();
   if (act) {
-    xs = make_double2(red[3][2 * seg][t], red[4][2 * seg][t]);
-    xe = make_double2(red[3][2 * seg + 1][t], red[4][2 * seg + 1][t]);
+    xs = make_double2(r2s[0][t][seg], r2s[1][t][seg]);
+    xe = make_double2(r2s[2][t][seg], r2s[3][t][seg]);
     q[(size_t)s * st] = from_d2<T2>(xs);
 #pragma unroll
     for (int r = 1; r < kTpRows - 1; r++) {
