@@ -3442,9 +3442,17 @@ extern "C" int vof_adjoint_gradient(const domain_t * domain, fluid_t * fluid, in
                              dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y, (const double2 *)ctx->d_twy,
                              (const double *)b, 1. / (double)ny, ctx->d_spec);
         }
-        hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((2 * ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
-                           ctx->nh, (const double *)ctx->d_pl, (const double *)ctx->d_pu,
-                           (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf, vbuf_store(ctx));
+        if (ctx->thomas_part && nx >= 4 && nx <= 256) {
+          hipLaunchKernelGGL(vofk::k_thomas_part<double2>, dim3((ctx->nh + vofk::kTpLines - 1) / vofk::kTpLines),
+                             dim3(vofk::kTpLines * vofk::kTpSeg), 0, st, nx, ctx->nh, ctx->nh,
+                             (const double *)ctx->d_pl, (const double *)ctx->d_pu, (const double *)ctx->d_ey,
+                             (const double *)ctx->d_ez, (const double *)ctx->d_stab, ctx->d_spec, ctx->d_vbuf);
+        } else {
+          hipLaunchKernelGGL(vofk::k_thomas_spec<double2>, dim3((2 * ctx->nh + 63) / 64), dim3(64), 0, st, nx, ctx->nh,
+                             ctx->nh, (const double *)ctx->d_pl, (const double *)ctx->d_pu,
+                             (const double *)ctx->d_ey, (const double *)ctx->d_ez, ctx->d_spec, ctx->d_vbuf,
+                             vbuf_store(ctx));
+        }
         if (use1024) {
           hipLaunchKernelGGL(vofk::k_irfft_y_psi<1024>, dim3(nbx), dim3(vofk::kFftThreads), 0, st, g, ctx->plan_y,
                              (const double2 *)ctx->d_twy, (const double2 *)ctx->d_spec, x, 1);
