@@ -2532,15 +2532,16 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
   const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
-  const int total = NC * N;
+  // each stored bin kk = 0 .. N/2 is read once and placed at ky = kk and,
+  // conjugated, at ky = N - kk
+  const int total = NC * nh;
   for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
     double2 As[kFftLoadU], Bs[kFftLoadU];
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
-      const int c = e / N, ky = e % N;
+      const int c = e / nh, kk = e % nh;
       const int xa = x0 + 2 * c, xb = xa + 1;
-      const int kk = ky <= N / 2 ? ky : N - ky;
       As[u] = make_double2(0., 0.);
       Bs[u] = make_double2(0., 0.);
       if (e < total && xa < nx) As[u] = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
@@ -2550,15 +2551,15 @@ __device__ __forceinline__ void k_irfft_y_psi_body(const VBlock & vb,
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
       if (e >= total) break;
-      const int c = e / N, ky = e % N;
-      const int kk = ky <= N / 2 ? ky : N - ky;
-      const double cs = ky <= N / 2 ? 1. : -1.;   // conjugate for the upper half
+      const int c = e / nh, kk = e % nh;
       double2 A = As[u], B = Bs[u];
       if (kk == 0 || 2 * kk == N) { A.y = 0.; B.y = 0.; }
-      A.y *= cs;
-      B.y *= cs;
-      // Z = A + i B
-      sa[c * CS + fft_in<F256>(ky)] = make_double2(A.x - B.y, A.y + B.x);
+      // Z = A + i B at ky = kk; the conjugates (imaginary parts times -1) at N - kk
+      sa[c * CS + fft_in<F256>(kk)] = make_double2(A.x - B.y, A.y + B.x);
+      if (kk != 0 && 2 * kk != N) {
+        const double ay = A.y * -1., by = B.y * -1.;
+        sa[c * CS + fft_in<F256>(N - kk)] = make_double2(A.x - by, ay + B.x);
+      }
     }
   }
   __syncthreads();
@@ -2749,15 +2750,16 @@ __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb,
   const int k = (int)tile.y + 1;
   const int nz = g.nz;
 #endif
-  const int total = NC * N;
+  // each stored bin kk = 0 .. N/2 is read once and placed at ky = kk and,
+  // conjugated, at ky = N - kk
+  const int total = NC * nh;
   for (int e0 = threadIdx.x; e0 < total; e0 += kFftLoadU * blockDim.x) {
     float2 As[kFftLoadU], Bs[kFftLoadU];
 #pragma unroll
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
-      const int c = e / N, ky = e % N;
+      const int c = e / nh, kk = e % nh;
       const int xa = x0 + 2 * c, xb = xa + 1;
-      const int kk = ky <= N / 2 ? ky : N - ky;
       As[u] = make_float2(0.f, 0.f);
       Bs[u] = make_float2(0.f, 0.f);
       if (e < total && xa < nx) As[u] = spec[((size_t)xa * nz + (k - 1)) * nh + kk];
@@ -2767,14 +2769,14 @@ __device__ __forceinline__ void k_irfft32_y_psi_body(const VBlock & vb,
     for (int u = 0; u < kFftLoadU; u++) {
       const int e = e0 + u * (int)blockDim.x;
       if (e >= total) break;
-      const int c = e / N, ky = e % N;
-      const int kk = ky <= N / 2 ? ky : N - ky;
-      const float cs = ky <= N / 2 ? 1.f : -1.f;
+      const int c = e / nh, kk = e % nh;
       float2 A = As[u], B = Bs[u];
       if (kk == 0 || 2 * kk == N) { A.y = 0.f; B.y = 0.f; }
-      A.y *= cs;
-      B.y *= cs;
-      sa[c * CS + fft_in<F256>(ky)] = make_float2(A.x - B.y, A.y + B.x);
+      sa[c * CS + fft_in<F256>(kk)] = make_float2(A.x - B.y, A.y + B.x);
+      if (kk != 0 && 2 * kk != N) {
+        const float ay = A.y * -1.f, by = B.y * -1.f;
+        sa[c * CS + fft_in<F256>(N - kk)] = make_float2(A.x - by, ay + B.x);
+      }
     }
   }
   __syncthreads();
