@@ -601,7 +601,7 @@ def main(argv=None) -> int:
     ap.add_argument("--drift-2d-steps", type=int, default=1000,
                     help="steps of the 2D 128x256 droplet workload for its drift report (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=96)
-    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=30)
     args = ap.parse_args(argv)
     if args.cpu_worker:
         return cpu_worker(args.n, args.cpu_reps, args.cpu_sync, max(1, args.cpu_workers))
@@ -715,7 +715,9 @@ def main(argv=None) -> int:
     g.close()
     if "batched" in want:
         try:
-            forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps, args.autotune)
+            # a batched step takes ~2 ms: the tuner times 10 steps per candidate
+            forms["batched"] = batched_form(sva, args.batch_cases, args.batch_steps,
+                                            max(args.autotune, 10) if args.autotune > 0 else 0)
         except Exception as e:
             forms["batched"] = {"error": repr(e)}
     if "precision" in want and args.precision_steps > 0:

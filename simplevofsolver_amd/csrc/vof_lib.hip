@@ -14,6 +14,7 @@
 #include <sys/stat.h>
 
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <memory>
@@ -3035,8 +3036,12 @@ static std::vector<TuneKnob> tune_knobs(const std::vector<vof_ctx *> & cs) {
   return k;
 }
 
+// sync_all (may be null): the steps run on several streams (a batch in
+// parts); a trial then waits for all of them and is timed on the host clock
+// from the end of its untimed step to the end of its last step
 static int tune_run(const std::vector<vof_ctx *> & cs, std::vector<std::unique_ptr<TuneSnap>> & snaps,
-                    hipStream_t st, int steps, const std::function<int()> & step, char * report, size_t len) {
+                    hipStream_t st, int steps, const std::function<int()> & step, char * report, size_t len,
+                    const std::function<int()> * sync_all = nullptr) {
   std::string rep;
   char line[160];
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -3046,6 +3051,16 @@ static int tune_run(const std::vector<vof_ctx *> & cs, std::vector<std::unique_p
   auto trial = [&](double * ms) -> int {
     for (auto & S : snaps)
       if (tune_restore(*S, st)) return 1;
+    if (sync_all) {
+      if ((*sync_all)()) return 1;
+      if (step() || (*sync_all)()) return 1;
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int n = 0; n < steps; n++)
+        if (step()) return 1;
+      if ((*sync_all)()) return 1;
+      *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+      return 0;
+    }
     if (step()) return 1;
     HIPCHK(hipEventRecord(e0, st));
     for (int n = 0; n < steps; n++)
@@ -3088,7 +3103,12 @@ static int tune_run(const std::vector<vof_ctx *> & cs, std::vector<std::unique_p
   }
   // leave every case as it was (also after a failed trial: the snapshot is
   // put back whatever the trial left)
-  if (rc) (void)hipStreamSynchronize(st);
+  if (rc) {
+    if (sync_all)
+      (void)(*sync_all)();
+    else
+      (void)hipStreamSynchronize(st);
+  }
   for (auto & S : snaps)
     if (tune_restore(*S, st)) rc = 1;
   if (hipStreamSynchronize(st) != hipSuccess) rc = fail("autotune: stream failed");
@@ -3129,33 +3149,21 @@ extern "C" int vof_autotune(domain_t * domain, fluid_t * fluid, interface_t * in
 extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, size_t len) {
   if (!B) return fail("vof_batch_autotune: batch is NULL");
   if (steps < 1 || steps > 1000) return fail("vof_batch_autotune: steps must be in 1..1000");
-  // a batch in parts is timed as a whole: the snapshots and events go to the
-  // first part's stream, and each tuning step joins the parts' streams on
-  // both sides, so that a trial's events cover both parts' work
+  // a batch in parts is timed as it runs, its parts stepped in turn on their
+  // own streams: on the host clock, with every part's stream drained around
+  // the timed steps; the snapshots go to the first part's stream
   const bool split = !B->parts.empty();
   hipStream_t st = split ? B->parts[0]->stream : B->stream;
-  hipEvent_t j0 = nullptr, j1 = nullptr;
-  if (split) {
+  const std::function<int()> sync_all = [&]() -> int {
     for (auto * P : B->parts) HIPCHK(hipStreamSynchronize(P->stream));
-    HIPCHK(hipEventCreateWithFlags(&j0, hipEventDisableTiming));
-    if (hipEventCreateWithFlags(&j1, hipEventDisableTiming) != hipSuccess) {
-      (void)hipEventDestroy(j0);
-      return fail("vof_batch_autotune: event creation failed");
-    }
+    return 0;
+  };
+  if (split) {
+    if (sync_all()) return 1;
   } else {
     HIPCHK(hipStreamSynchronize(B->stream));
   }
-  auto step = [&]() -> int {
-    if (!split) return vof_batch_integrate(B, nullptr);
-    HIPCHK(hipEventRecord(j0, st));
-    for (size_t p = 1; p < B->parts.size(); p++) HIPCHK(hipStreamWaitEvent(B->parts[p]->stream, j0, 0));
-    if (vof_batch_integrate(B, nullptr)) return 1;
-    for (size_t p = 1; p < B->parts.size(); p++) {
-      HIPCHK(hipEventRecord(j1, B->parts[p]->stream));
-      HIPCHK(hipStreamWaitEvent(st, j1, 0));
-    }
-    return 0;
-  };
+  auto step = [&]() -> int { return vof_batch_integrate(B, nullptr); };
   std::vector<std::unique_ptr<TuneSnap>> snaps;
   std::vector<bool> prof(B->ctx.size());
   int rc = 0;
@@ -3171,13 +3179,10 @@ extern "C" int vof_batch_autotune(vof_batch_t * B, int steps, char * report, siz
     snaps.emplace_back(new TuneSnap());
     rc = tune_take(B->ctx[c], B->fluid[c], B->interface[c], st, *snaps.back());
   }
-  if (!rc) rc = tune_run(B->ctx, snaps, st, steps, step, report, len);
+  if (!rc) rc = tune_run(B->ctx, snaps, st, steps, step, report, len, split ? &sync_all : nullptr);
   if (rc) (void)hipStreamSynchronize(st);
-  if (split) {
+  if (split)
     for (auto * P : B->parts) (void)hipStreamSynchronize(P->stream);
-    (void)hipEventDestroy(j0);
-    (void)hipEventDestroy(j1);
-  }
   for (size_t c = 0; c < B->ctx.size(); c++) B->ctx[c]->prof_on = prof[c];
   return rc;
 }
