@@ -218,13 +218,20 @@ def drift_2d(sva, steps, oracle_steps=100):
         for name in ("ux", "uy", "vof"):
             a, b = g.get(name), o.get(name)
             diff[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-        for _ in range(max(0, steps - oracle_steps)):
+        t1 = time.perf_counter()
+        ngpu = max(0, steps - oracle_steps)
+        for _ in range(ngpu):
             g.integrate()
         g.synchronize()
-        el = time.perf_counter() - t0
+        t2 = time.perf_counter()
+        el = t2 - t0
         rec = drift_record(d0, g.diagnostics(), max(steps, oracle_steps), ndims=2)
         rec["vs_oracle_after"] = {"steps": oracle_steps, **diff}
+        # wall_s includes the oracle's steps beside the first oracle_steps
         rec["wall_s"] = round(el, 2)
+        if ngpu:
+            rec["gpu_ms_per_step"] = round((t2 - t1) / ngpu * 1e3, 4)
+            rec["gpu_mcell_steps_per_s"] = round(ic.glsizes[0] * ic.glsizes[1] * ngpu / (t2 - t1) / 1e6, 3)
         rec["workload"] = "2D 128x256 on 1x2, droplet r=0.25, perturbation 1e-3, Re=100, We=10, FP64"
         print(f"[bench] 2D drift over {rec['steps']} steps: KE {rec['kinetic_energy_rel']:+.3e}, interface "
               f"{rec['interface_area_rel']:+.3e}, volume {rec['vof_volume_rel']:.1e}; vs oracle after "
