@@ -3491,8 +3491,13 @@ __device__ __forceinline__ void lds_barrier() {
 // (a Newton solve at each interface cell) are not solved in the march, where
 // a wavefront would run them with only its few interface lanes active: the
 // cells' vof and normals go to a list in LDS, which the work-group's lanes
-// solve one cell each after the last plane (cells beyond the list's
-// kMList entries are solved in place).
+// solve one cell each after the last plane.  THINC fringes make interfaces
+// several cells thick, so a work-group can meet more interface cells than
+// the list holds: those wait in their own entries of the output (normal and
+// vof), and after the list their (a, b, k - kb) are compacted into the
+// corner ring, free by then, and solved one cell per lane as well.  (Solving
+// them in the march holds the Newton solve's registers across the whole
+// plane loop and costs a wave of occupancy.)
 constexpr int kMX = 32, kMY = 8;
 constexpr int kMThreads = kMX * kMY;
 constexpr int kMList = 256;
@@ -3507,6 +3512,7 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
   __shared__ double lv[4][kMList];   // listed interface cells: vof, normal x, y, z
   __shared__ int lc[kMList];         // their (a, b, k - kb)
   __shared__ int ln;
+  __shared__ int lnov;               // cells beyond the list
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int i0 = 1 + (int)vb.x * kMX, j0 = 1 + (int)vb.y * kMY, kb = 1 + (int)vb.z * KZ;
   const int ke = min(kb + KZ - 1, nz);
@@ -3556,7 +3562,10 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
       o2[e] = gz * ninv;
     }
   };
-  if (tid == 0) ln = 0;
+  if (tid == 0) {
+    ln = 0;
+    lnov = 0;
+  }
   fetch(kb - 1);
   put(kb - 1);
   fetch(kb);
@@ -3569,6 +3578,11 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
   const bool own = i <= nx && j <= ny;
   const double dx = own ? g.dxf[i - 1] : 1.;
   const double rdx = own ? g.xt[kXT * i + XT_RXF] : 0.;   // 1 / DXF
+  // first of the 4 entries of cell (ii, jj, kk) in `normal`
+  auto park = [&](int ii, int jj, int kk) {
+    return (size_t)(ii - 1) * 4 + (size_t)nx * 4 * ((size_t)jj + (size_t)(ny + 2) * (size_t)kk);
+  };
+  unsigned ovm = 0u;   // planes (bit k - kb) whose cell of this lane waits outside the list
   for (int k = kb; k <= ke; k++) {
     put(k + 1);
     if (k + 2 <= ke + 1) fetch(k + 2);
@@ -3621,11 +3635,14 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
           lv[3][slot] = nv[2];
           lc[slot] = a | (b << 8) | ((k - kb) << 16);
         } else {
-          nv[3] = intercept(lvof, nv);
-          for (int d = 0; d <= 3; d++) {
-            double * base = normal + d;
-            store_periodic(base, (i - 1) * 4, nx * 4, ny + 2, j, PLANE1(k), ny, nz, 1, nv[d]);
-          }
+          // list full: the cell's normal and vof wait in its own entry of
+          // `normal` (x, y, z, intercept) and bit k - kb of ovm
+          double * ent = normal + park(i, j, k);
+          ent[0] = nv[0];
+          ent[1] = nv[1];
+          ent[2] = nv[2];
+          ent[3] = lvof;
+          ovm |= 1u << (k - kb);
         }
       }
     }
@@ -3641,6 +3658,36 @@ __device__ __forceinline__ void k_iface_march_body(const VBlock & vb, Grid g, co
       double * base = normal + d;
       store_periodic(base, (ci - 1) * 4, nx * 4, ny + 2, cj, PLANE1(ck), ny, nz, 1, nv[d]);
     }
+  }
+  // the cells beyond the list: lane offsets by atomic add, then windows of
+  // kOv cells compacted into the corner ring and solved a lane each
+  constexpr int kOv = (int)(sizeof(sc) / sizeof(int));
+  const int cnt = __builtin_popcount(ovm);
+  const int off = cnt > 0 ? atomicAdd(&lnov, cnt) : 0;
+  __syncthreads();   // the parked entries (global) and lnov (LDS)
+  const int tot = lnov;
+  int * ovl = reinterpret_cast<int *>(&sc[0][0][0]);
+  for (int w = 0; w < tot; w += kOv) {
+    unsigned m = ovm;
+    for (int e = off; m != 0u; e++) {
+      const int kk = __builtin_ctz(m);
+      m &= m - 1u;
+      if (e >= w && e < w + kOv) ovl[e - w] = a | (b << 8) | (kk << 16);
+    }
+    lds_barrier();
+    const int n = min(tot - w, kOv);
+    for (int x = tid; x < n; x += kMThreads) {
+      const int c = ovl[x];
+      const int ci = i0 + (c & 255), cj = j0 + ((c >> 8) & 255), ck = kb + (c >> 16);
+      const double * ent = normal + park(ci, cj, ck);
+      double nv[4] = {ent[0], ent[1], ent[2], 0.};
+      nv[3] = intercept(ent[3], nv);
+      for (int d = 0; d <= 3; d++) {
+        double * base = normal + d;
+        store_periodic(base, (ci - 1) * 4, nx * 4, ny + 2, cj, PLANE1(ck), ny, nz, 1, nv[d]);
+      }
+    }
+    lds_barrier();
   }
 }
 
