@@ -62,6 +62,7 @@ def algorithmic_bytes(kernel: str, nx: int, ny: int, nz: int, iface_cells: int |
         "k_div_rfft_y": c * 24 + s * 16,
         "k_fft_z": s * 32,
         "k_thomas_spec": s * 32,
+        "k_thomas_part": s * 32,
         "k_irfft_y_psi": s * 16 + c * 8,
         "k_correct": c * (8 + 24 + 24),
         "k_pressure": c * (8 + 8 + 8),

@@ -1807,8 +1807,10 @@ extern "C" int fluid_predict_field(const domain_t * domain, const size_t rkstep,
 template <class T2>
 static int launch_thomas(vof_ctx * ctx, const char * name, T2 * spec, int nlines) {
   if (ctx->thomas_part && ctx->nx >= 4 && ctx->nx <= 256) {
+    // timing label of the kernel that runs
+    const char * pname = sizeof(T2) == sizeof(float2) ? "k_thomas_part32" : "k_thomas_part";
     const dim3 g((unsigned)((nlines + vofk::kTpLines - 1) / vofk::kTpLines)), b(vofk::kTpLines * vofk::kTpSeg);
-    KLB(ctx, name, vofk::kTpLines * vofk::kTpSeg, (vofk::k_thomas_part_body<T2>), (vofk::k_thomas_part<T2>), g, b, 0,
+    KLB(ctx, pname, vofk::kTpLines * vofk::kTpSeg, (vofk::k_thomas_part_body<T2>), (vofk::k_thomas_part<T2>), g, b, 0,
         ctx->nx, nlines, ctx->nh, ctx->d_pl, ctx->d_pu, ctx->d_ey, ctx->d_ez, ctx->d_stab, spec, ctx->d_vbuf);
     return 0;
   }
