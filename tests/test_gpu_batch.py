@@ -35,15 +35,24 @@ def _cases_mixed_transforms_3d():
             random_fields((12, 16, 8), seed=77)]
 
 
+def _cases_overflow_3d():
+    # every cell of the first case an interface cell: each work-group of the
+    # 3D interface march meets more cells than its intercept list holds
+    from simplevofsolver_amd.ic import _pad_cells
+    ic = random_fields((40, 20, 24), seed=79, uniformx=False)
+    ic.fields["vof"] = _pad_cells(np.random.default_rng(79).uniform(0.02, 0.98, size=(24, 20, 40)))
+    return [ic, random_fields((16, 12, 20), seed=80)]
+
+
 def _names(nd):
     return ["ux", "uy", "t", "vof", "p", "psi"] + (["uz"] if nd == 3 else [])
 
 
 @pytest.mark.parametrize("mixed", [-1, 2])
-@pytest.mark.parametrize("dim", [2, 3, "2-256", "3-256"])
+@pytest.mark.parametrize("dim", [2, 3, "2-256", "3-256", "3-overflow"])
 def test_batch_is_bit_identical_to_single_cases(dim, mixed, tmp_path):
     cases = {2: _cases_2d, 3: _cases_3d, "2-256": _cases_mixed_transforms,
-             "3-256": _cases_mixed_transforms_3d}[dim]()
+             "3-256": _cases_mixed_transforms_3d, "3-overflow": _cases_overflow_3d}[dim]()
     dim = len(cases[0].glsizes)
     singles = [sva.solver_from_ic(ic, str(tmp_path / f"a{c}")) for c, ic in enumerate(cases)]
     members = [sva.solver_from_ic(ic, str(tmp_path / f"b{c}")) for c, ic in enumerate(cases)]
