@@ -27,6 +27,9 @@ def test_algorithmic_bytes_per_cell():
     assert bench.algorithmic_bytes("k_lines_lds", n, n, n) == (4 * n - 1) * n * n * 24
     nh = n // 2 + 1
     assert bench.algorithmic_bytes("k_fft_z", n, n, n) == n * nh * n * 32
+    # the spectrum read and written once by either spectral x solver
+    for k in ("k_thomas_part", "k_thomas_spec"):
+        assert bench.algorithmic_bytes(k, n, n, n) == n * nh * n * 32
     assert bench.algorithmic_bytes("no_such_kernel", n, n, n) is None
 
 
