@@ -3701,6 +3701,129 @@ __global__ __launch_bounds__(kMThreads) void k_iface_march(Grid g, const double 
 // interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
 // (compute_flux_*), the flux divergence and the RK update, written to the
 // other VOF buffer with its halo images (walls stay zero).
+// face_flux's THINC integral with the face direction D chosen at run time
+// (the same quadrature points and operation order as face_flux<D>)
+__device__ inline double thinc_face(const double * n, int D, double c) {
+  double flux = 0.;
+#if NDIMS == 2
+  for (int q = 0; q < NGAUSS; q++) {
+    const double pt[2] = {D == 0 ? c : kGaussP[q], D == 0 ? kGaussP[q] : c};
+    flux += kGaussW[q] * indicator(n, pt);
+  }
+#else
+  for (int q = 0; q < NGAUSS; q++) {
+    double sr = 0.;
+    for (int r = 0; r < NGAUSS; r++) {
+      const double pt[3] = {D == 0 ? c : kGaussP[q], D == 0 ? kGaussP[q] : D == 1 ? c : kGaussP[r],
+                            D == 2 ? c : kGaussP[r]};
+      sr += kGaussW[r] * indicator(n, pt);
+    }
+    flux += kGaussW[q] * sr;
+  }
+#endif
+  return flux;
+}
+
+// k_vof_fused without decision record / replay: the THINC integrals of the
+// work-group's interface faces (upwind cell holding an interface) are not
+// formed by each cell's lane, where a wavefront runs them with its few
+// interface lanes active, but one face per lane from a list in LDS.  Every
+// face of the work-group has a slot (lane * NF + face) holding its flux, or
+// for an interface face its velocity until the listed pass replaces it by
+// velocity * integral.  Fluxes and their sum are those of face_flux, in the
+// same order.  At most 256 lanes (launch bounds), so the slots never run out.
+constexpr int kVLanes = 256;
+__device__ __forceinline__ void vof_fused_listed(const VBlock & vb, Grid g, const double * __restrict__ ux,
+                                                 const double * __restrict__ uy, const double * __restrict__ uz,
+                                                 const double * __restrict__ vof, const double * __restrict__ normal,
+                                                 double * __restrict__ srca, const double * __restrict__ srcb,
+                                                 double * __restrict__ vof_new, double dta, double dtb, int use_b,
+                                                 int keep_a) {
+  constexpr int NC = NDIMS + 1;
+  constexpr int NF = 2 * NDIMS;   // x lower, x upper, y lower, y upper (, z lower, z upper)
+  __shared__ double fv[kVLanes * NF];           // flux (velocity of a listed face before the listed pass)
+  __shared__ unsigned fi[kVLanes * NF];         // listed face: the upwind cell's normal (entry of I_NRM)
+  __shared__ unsigned char fd[kVLanes * NF];    // listed face: 2 D + (upwind cell is the upper one)
+  __shared__ unsigned short fl[kVLanes * NF];   // the listed faces' slots
+  __shared__ int fn;
+  const int nx = g.nx, ny = g.ny;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nt = blockDim.x * blockDim.y;
+  if (tid == 0) fn = 0;
+  __syncthreads();
+  THREAD_IJK(1, 1, 1)
+  const bool own = i <= nx && j <= ny;
+#if NDIMS == 3
+  const int nz = g.nz;
+#else
+  const int nz = 1;
+  (void)uz;
+#endif
+  double vc = 0., sb = 0.;
+  size_t is = 0;
+  if (own) {
+    is = I_SC(i, j, k);
+    const double uxm = ux[I_UX(i, j, k)], uxp = ux[I_UX(i + 1, j, k)];
+    const double uym = uy[I_C1(i, j, k)], uyp = uy[I_C1(i, j + 1, k)];
+    vc = vof[I_VOF(i, j, k)];
+    const double vxm = vof[I_VOF(i - 1, j, k)], vxp = vof[I_VOF(i + 1, j, k)];
+    const double vym = vof[I_VOF(i, j - 1, k)], vyp = vof[I_VOF(i, j + 1, k)];
+    sb = use_b ? srcb[is] : 0.;
+#if NDIMS == 3
+    const double uzm = uz[I_C1(i, j, k)], uzp = uz[I_C1(i, j, k + 1)];
+    const double vzm = vof[I_VOF(i, j, k - 1)], vzp = vof[I_VOF(i, j, k + 1)];
+#endif
+    const unsigned cc = (unsigned)I_NRM(i, j, k);
+    // face f between its lower cell (vof vm, normal nm) and upper cell (vp, np)
+    auto face = [&](int f, double vel, double vm, double vp, unsigned nm, unsigned np) {
+      const int sl = tid * NF + f;
+      if ((f == 0 && i == 1) || (f == 1 && i == nx)) {   // wall faces carry no flux
+        fv[sl] = 0.;
+        return;
+      }
+      const bool up = vel < 0.;
+      const double lv = up ? vp : vm;
+      if (lv < kVofMin || 1. - kVofMin < lv) {
+        fv[sl] = vel * lv;
+        return;
+      }
+      fv[sl] = vel;
+      fi[sl] = up ? np : nm;
+      fd[sl] = (unsigned char)(2 * (f >> 1) + (up ? 1 : 0));
+      fl[atomicAdd(&fn, 1)] = (unsigned short)sl;
+    };
+    face(0, uxm, vxm, vc, (unsigned)I_NRM(i - 1, j, k), cc);
+    face(1, uxp, vc, vxp, cc, (unsigned)I_NRM(i + 1, j, k));
+    face(2, uym, vym, vc, (unsigned)I_NRM(i, j - 1, k), cc);
+    face(3, uyp, vc, vyp, cc, (unsigned)I_NRM(i, j + 1, k));
+#if NDIMS == 3
+    face(4, uzm, vzm, vc, (unsigned)I_NRM(i, j, k - 1), cc);
+    face(5, uzp, vc, vzp, cc, (unsigned)I_NRM(i, j, k + 1));
+#endif
+  }
+  __syncthreads();
+  const int n = fn;
+  for (int e = tid; e < n; e += nt) {
+    const int sl = fl[e];
+    const int d = fd[sl];
+    fv[sl] = fv[sl] * thinc_face(normal + NC * (size_t)fi[sl], d >> 1, (d & 1) ? -0.5 : +0.5);
+  }
+  __syncthreads();
+  if (!own) return;
+  const double * f = fv + tid * NF;
+  double s = 0.;
+  s += g.xt[kXT * i + XT_RXF] * (f[0] - f[1]);   // 1 / DXF(i)
+  s += g.rdy * (f[2] - f[3]);
+#if NDIMS == 3
+  s += g.rdz * (f[4] - f[5]);
+#endif
+  if (keep_a) st_stream(&srca[is], s);
+  double v = vc;
+  v += dta * s;
+  if (use_b) v += dtb * sb;
+  store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
+}
+
 template <int FZ>
 __device__ __forceinline__ void k_vof_fused_fz_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                             const double * __restrict__ uz, const double * __restrict__ vof,
@@ -3762,7 +3885,7 @@ __device__ __forceinline__ void k_vof_fused_body(const VBlock & vb, Grid g, cons
                             const double * __restrict__ normal, double * __restrict__ srca,
                             const double * __restrict__ srcb, double * __restrict__ vof_new,
                             double dta, double dtb, int use_b, int keep_a) {
-  k_vof_fused_fz_body<0>(vb, g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a, Freeze{});
+  vof_fused_listed(vb, g, ux, uy, uz, vof, normal, srca, srcb, vof_new, dta, dtb, use_b, keep_a);
 }
 
 __global__ __launch_bounds__(256) void k_vof_fused(Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
