@@ -3701,6 +3701,7 @@ __global__ __launch_bounds__(kMThreads) void k_iface_march(Grid g, const double 
 // interface_update_vof fused: the 2 x NDIMS face fluxes of each cell
 // (compute_flux_*), the flux divergence and the RK update, written to the
 // other VOF buffer with its halo images (walls stay zero).
+
 // face_flux's THINC integral with the face direction D chosen at run time
 // (the same quadrature points and operation order as face_flux<D>)
 __device__ inline double thinc_face(const double * n, int D, double c) {
@@ -3824,6 +3825,8 @@ __device__ __forceinline__ void vof_fused_listed(const VBlock & vb, Grid g, cons
   store_periodic(vof_new, i, nx + 2, ny + 4, j + 1, PLANE2(k), ny, nz, 2, v);
 }
 
+// The same with each cell's lane forming its own faces' integrals, as the
+// 2D adjoint's decision record (FZ = 1) and replay (FZ = 2) need.
 template <int FZ>
 __device__ __forceinline__ void k_vof_fused_fz_body(const VBlock & vb, Grid g, const double * __restrict__ ux, const double * __restrict__ uy,
                             const double * __restrict__ uz, const double * __restrict__ vof,
